@@ -1,0 +1,160 @@
+/*
+ * sopg.h -- C ABI of the MI355X (gfx950) SOP Langevin-dynamics hot path.
+ *
+ * Drop-in boundary for SOP-CPU's hot path.  Each entry point below replaces
+ * one reference function (cited file:line in SOP-CPU); the C++ shim in
+ * sop-cpu_amd/host/ defines the reference's own signatures (energy.h,
+ * neighbor_list.h, cell_list.h, sop.h) on top of these.
+ *
+ * Conventions
+ *   - Every function returns an int status: SOPG_OK (0) or a negative code;
+ *     sopg_last_error() gives the message.  No exceptions cross the ABI.
+ *   - Plain pointers and sizes only.  Bead indices in topology arrays are
+ *     1-based, as in the reference's files; arrays themselves are 0-based
+ *     (pass &ibead_bnd[1] for a reference 1-based array).
+ *   - Coordinates are AoS x,y,z triples per bead (the reference's `coord`
+ *     layout, global.h:6-13): pass &pos[1].x.
+ *   - A context owns all device memory of one GPU stream and holds one or more
+ *     chains (a ragged batch).  One host thread per context.
+ *   - FP64 throughout.  Device code is built without FMA contraction and with
+ *     IEEE division/sqrt so that forces, lists and trajectories reproduce the
+ *     reference's arithmetic.
+ */
+#ifndef SOPG_H
+#define SOPG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SOPG_OK 0
+#define SOPG_ERR_ARG -1       /* invalid argument / state */
+#define SOPG_ERR_HIP -2       /* HIP runtime error */
+#define SOPG_ERR_OVERFLOW -3  /* a per-bead neighbor list exceeded its capacity */
+#define SOPG_ERR_UNSUPPORTED -4
+
+#define SOPG_CUTOFF_NONE 0    /* no cutoff: the loader's full master lists (params.cpp:210-230) */
+#define SOPG_CUTOFF_NEIGHBOR 1 /* cutofftype neighborlist (neighbor_list.cpp) */
+#define SOPG_CUTOFF_CELL 2    /* cutofftype celllist (cell_list.cpp) */
+
+#define SOPG_NOISE_REPLAY 0   /* caller supplies the 3N gaussians of each force_eval */
+#define SOPG_NOISE_PHILOX 1   /* device Philox4x32-10 + Box-Muller, keyed by chain seed */
+
+/* Which list sopg_get_list materialises (reference order). */
+#define SOPG_LIST_NL_ATT 0    /* ibead/jbead_neighbor_list_att, nnl_att */
+#define SOPG_LIST_NL_REP 1    /* ibead/jbead_neighbor_list_rep, nnl_rep */
+#define SOPG_LIST_PL_ATT 2    /* ibead/jbead_pair_list_att, nil_att */
+#define SOPG_LIST_PL_REP 3    /* ibead/jbead_pair_list_rep, nil_rep */
+
+typedef struct sopg_ctx sopg_ctx;
+
+/* Type tables and force-field constants (global.cpp:60-92, params.cpp:268-312).
+ * sopg_default_tables() fills the reference's values. */
+typedef struct {
+  double coeff_att[3][3], coeff_rep[3][3];
+  double force_coeff_att[3][3], force_coeff_rep[3][3];
+  double sigma_rep[3][3];
+  double k_bnd, R0sq, e_bnd_coeff;     /* FENE: k=20, R0^2=4, kR0^2/2 */
+  double e_ang_ss_coeff, f_ang_ss_coeff; /* eps*sigma_ss^6, 6*eps*sigma_ss^6 */
+} sopg_tables;
+
+/* Per-chain run parameters (params.cpp:12-101 semantics are applied by the
+ * caller: pass the already-derived a1..a5 exactly as set_params left them). */
+typedef struct {
+  double T, zeta, h;                   /* temperature, friction, time step */
+  double a1, a2, a3, a4, a5;           /* integrator coefficients */
+  double boxl, ncell, lcell;           /* periodic box, cells per edge, cell edge */
+  int cutoff;                          /* SOPG_CUTOFF_* */
+  int nnlup;                           /* list rebuild period (steps) */
+  int nup;                             /* observable/record period (steps) */
+  int noise_mode;                      /* SOPG_NOISE_* */
+  uint64_t seed;                       /* Philox key (SOPG_NOISE_PHILOX) */
+} sopg_chain_params;
+
+/* One chain's topology.  Arrays are 0-based with 1-based bead indices.
+ * rep_* may be NULL with ncon_rep = -1: the non-native master list is then the
+ * complement of the native list among all pairs j-i>2 in (i,j) lexicographic
+ * order -- exactly the list a complete lexicographic vdw file yields -- and
+ * bead_type (may be NULL: all type 1) gives itype/jtype. */
+typedef struct {
+  int nbead;
+  int nbnd; const int* ibnd; const int* jbnd; const double* r0bnd;
+  int nang; const int* iang; const int* jang; const int* kang;
+  int ncon_att; const int* att_i; const int* att_j; const int* att_it; const int* att_jt;
+  const double* att_r;                 /* lj_nat_pdb_dist */
+  int ncon_rep; const int* rep_i; const int* rep_j; const int* rep_it; const int* rep_jt;
+  const int* bead_type;
+} sopg_topology;
+
+typedef struct { double e_bnd, e_ang_ss, e_vdw_rr_att, e_vdw_rr_rep, e_vdw_rr, rna_etot; } sopg_energies;
+typedef struct { double Q; int contct_nat; double rgsq, kinT; } sopg_observables;
+
+/* One `update.out` record (sop.cpp:176-179) produced by sopg_run. */
+typedef struct {
+  int chain; double istep, T, kinT, e_bnd, e_ang_ss, e_vdw_rr, rna_etot, Q; int contct_nat; double rgsq;
+} sopg_record;
+
+/* ---------------------------------------------------------------- context */
+int sopg_create(int device, sopg_ctx** out);
+int sopg_destroy(sopg_ctx* ctx);
+const char* sopg_last_error(const sopg_ctx* ctx);
+int sopg_default_tables(sopg_tables* t);
+int sopg_set_tables(sopg_ctx* ctx, const sopg_tables* t);
+/* Lanes per bead for the list and force kernels (1..64, power of two; 0 =
+ * automatic).  Results do not depend on it. */
+int sopg_set_lanes(sopg_ctx* ctx, int lanes);
+/* Per-bead capacity of the non-native neighbor list (0 = automatic). */
+int sopg_set_capacity(sopg_ctx* ctx, int cap);
+
+/* ---------------------------------------------------------------- chains */
+/* Topology copy of the loaders (params.cpp:110-260); returns chain id in *id. */
+int sopg_add_chain(sopg_ctx* ctx, const sopg_topology* topo, const sopg_chain_params* p, int* id);
+int sopg_set_chain_params(sopg_ctx* ctx, int chain, const sopg_chain_params* p);
+/* State of one chain: pos, unc_pos, vel, force as AoS 3N doubles (any may be NULL). */
+int sopg_put_state(sopg_ctx* ctx, int chain, const double* pos, const double* unc_pos,
+                   const double* vel, const double* force);
+int sopg_get_state(sopg_ctx* ctx, int chain, double* pos, double* unc_pos, double* vel, double* force);
+int sopg_nchains(const sopg_ctx* ctx);
+int sopg_chain_nbead(const sopg_ctx* ctx, int chain);
+
+/* ------------------------------------------- reference entry points (one chain) */
+/* update_neighbor_list()  neighbor_list.cpp:6-117 */
+int sopg_update_neighbor_list(sopg_ctx* ctx, int chain, int* nnl_att, int* nnl_rep);
+/* update_cell_list()      cell_list.cpp:6-98 */
+int sopg_update_cell_list(sopg_ctx* ctx, int chain, int* nnl_att, int* nnl_rep);
+/* update_pair_list()      sop.cpp:473-553 */
+int sopg_update_pair_list(sopg_ctx* ctx, int chain, int* nil_att, int* nil_rep);
+/* force_eval()            energy.cpp:21-33 (clear, random, FENE, angular, vdw).
+ * noise: the 3N gaussians this call consumes (bead-major x,y,z), or NULL for
+ * the chain's device generator. */
+int sopg_force_eval(sopg_ctx* ctx, int chain, const double* noise);
+/* energy_eval()           energy.cpp:6-19 */
+int sopg_energy_eval(sopg_ctx* ctx, int chain, sopg_energies* out);
+/* underdamped_iteration() sop.cpp:282-329 */
+int sopg_underdamped_iteration(sopg_ctx* ctx, int chain, const double* noise);
+/* calculate_observables() sop.cpp:200-280 */
+int sopg_calculate_observables(sopg_ctx* ctx, int chain, sopg_observables* out);
+/* The current NL / PL of a chain in the reference's order (1-based bead ids).
+ * Writes at most cap entries; *n = list length. */
+int sopg_get_list(sopg_ctx* ctx, int chain, int which, int* ibead, int* jbead, long cap, long* n);
+
+/* ---------------------------------------------------------------- hot loop */
+/* Advance every chain of the context by nsteps steps of the underdamped_ctrl
+ * loop body (sop.cpp:153-189): list rebuild every nnlup steps, pair list every
+ * step, underdamped_iteration, and an energy + observables record every nup
+ * steps.  Step counters continue across calls (sopg_set_step).  noise: NULL
+ * (device generator) or nsteps*3*Ntot gaussians, step-major, beads in chain
+ * order.  Records go to rec (up to rec_cap); *nrec = count produced. */
+int sopg_run(sopg_ctx* ctx, long nsteps, const double* noise, sopg_record* rec, long rec_cap, long* nrec);
+/* Set the step counters of a chain: istep of the next step, and the
+ * reference's inlup / iup counters as they stand before that step. */
+int sopg_set_step(sopg_ctx* ctx, int chain, double istep, int inlup, int iup);
+/* Block until all work queued on the context's stream has finished. */
+int sopg_sync(sopg_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SOPG_H */

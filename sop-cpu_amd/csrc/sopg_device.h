@@ -1,0 +1,105 @@
+// sopg_device.h -- FP64 helpers shared by the gfx950 kernels (and by host
+// checks of the same routines).  Compiled with -ffp-contract=off: every
+// expression below is evaluated exactly as written, one IEEE rounding per
+// operation, which is what makes the kernels reproduce SOP-CPU's arithmetic.
+#pragma once
+#include <cmath>
+#include <cstdint>
+
+#if defined(__HIPCC__)
+#define SOPG_HD __host__ __device__ __forceinline__
+#else
+#define SOPG_HD inline
+#endif
+
+namespace sopg {
+
+// rnd(x): round half away from zero through floor/ceil (global.cpp:223-227).
+SOPG_HD double rnd(double x) { return (x > 0) ? floor(x + 0.5) : ceil(x - 0.5); }
+
+// Min-image component d - boxl*rnd(d/boxl) (e.g. energy.cpp:287-289).
+// Exact fast path: for |d| <= boxl/4 the quotient lies in [-1/4, 1/4]
+// (division is monotone and boxl/4 is exact), rnd returns +0 or -0, and
+// d - boxl*(+-0) equals d + 0.0 (which maps -0 to +0, as the full form does).
+// The division is skipped only there; everything else takes the full form.
+SOPG_HD double mimg(double d, double boxl, double qbox) {
+  if (fabs(d) <= qbox) return d + 0.0;
+  return d - boxl * rnd(d / boxl);
+}
+
+// Correctly rounded d^8 and d^6 by double-double squaring.  The reference
+// calls glibc pow(d,8.0) / pow(d,6.0) (energy.cpp:382, 180); glibc's pow is
+// accurate to ~0.52 ulp, so it differs from the correctly rounded value by one
+// ulp on ~0.08% of inputs.  fma gives the exact low part of each product.
+SOPG_HD void two_prod(double a, double b, double& p, double& e) { p = a * b; e = fma(a, b, -p); }
+SOPG_HD void dd_sq(double h, double l, double& oh, double& ol) {
+  double p, e;
+  two_prod(h, h, p, e);
+  double c = h * l;
+  e += c + c;
+  double s = p + e;
+  ol = e - (s - p);
+  oh = s;
+}
+SOPG_HD double pow8_cr(double d) {
+  double h2, l2, h4, l4, h8, l8;
+  dd_sq(d, 0.0, h2, l2);
+  dd_sq(h2, l2, h4, l4);
+  dd_sq(h4, l4, h8, l8);
+  return h8;
+}
+SOPG_HD double pow6_cr(double d) {
+  double h2, l2, h4, l4, p, e;
+  dd_sq(d, 0.0, h2, l2);
+  dd_sq(h2, l2, h4, l4);
+  two_prod(h4, h2, p, e);
+  e += h4 * l2 + l4 * h2;
+  return p + e;
+}
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): counter-based, so every
+// (chain seed, force evaluation, bead, pair) has its own independent block.
+SOPG_HD void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+  for (int r = 0; r < 10; r++) {
+    uint64_t p0 = (uint64_t)M0 * c[0], p1 = (uint64_t)M1 * c[2];
+    uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+    k0 += W0; k1 += W1;
+  }
+}
+
+// Two standard normal deviates (Box-Muller) from one Philox block.
+SOPG_HD void philox_gauss2(uint64_t seed, uint64_t ctr, uint32_t bead, uint32_t pair, double& g0, double& g1) {
+  uint32_t c[4] = {(uint32_t)ctr, (uint32_t)(ctr >> 32), bead, pair};
+  philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  uint64_t a = ((uint64_t)c[0] << 32) | c[1], b = ((uint64_t)c[2] << 32) | c[3];
+  double u1 = (double)((a >> 11) + 1) * (1.0 / 9007199254740992.0);  // (0, 1]
+  double u2 = (double)(b >> 11) * (1.0 / 9007199254740992.0);        // [0, 1)
+  double r = sqrt(-2.0 * log(u1));
+  double s, co;
+#if defined(__HIP_DEVICE_COMPILE__)
+  sincospi(2.0 * u2, &s, &co);
+#else
+  s = sin(6.283185307179586 * u2);
+  co = cos(6.283185307179586 * u2);
+#endif
+  g0 = r * co;
+  g1 = r * s;
+}
+
+// Slot word of the per-bead non-native neighbor list:
+//   bits  0-24 partner bead (chain-local, 0-based)
+//   bit     25 side: 0 = this bead is the pair's i (first) bead, 1 = its j bead
+//   bits 26-27 itype, bits 28-29 jtype
+//   bits 30-31 pair-list membership, one bit per parity (double-buffered)
+constexpr uint32_t kPartnerMask = 0x1FFFFFFu;
+constexpr int kMaxChainBeads = 1 << 25;
+SOPG_HD uint32_t slot_make(uint32_t partner, uint32_t side, uint32_t it, uint32_t jt) {
+  return partner | (side << 25) | (it << 26) | (jt << 28);
+}
+
+}  // namespace sopg

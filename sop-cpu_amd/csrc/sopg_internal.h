@@ -1,0 +1,74 @@
+// sopg_internal.h -- device-side data layout of a context (a ragged batch of
+// chains) shared by the kernels and the C ABI implementation.
+//
+// Layout in HBM (SoA, FP64, all chains concatenated; global bead index g =
+// chain offset + local bead index):
+//   unc_pos x/y/z  two buffers (the step kernel reads one, writes the next)
+//   pos x/y/z      two buffers (wrapped copy, output only)
+//   vel, force, incr x/y/z
+//   per-bead incidence lists (CSR, in the reference's list order):
+//     bonds, angles, native (attractive) contacts, and -- for explicit
+//     non-native master lists -- non-native contacts
+//   per-bead non-native neighbor slots: cap words per bead + count
+//   master native-contact arrays (sigma powers, cutoffs, NL / PL flags)
+#pragma once
+#include <cstdint>
+
+namespace sopg {
+
+constexpr double kTol = 1.0e-7;  // energy.cpp:271
+
+struct Tables {
+  double fatt[3][3], frep[3][3], catt[3][3], crep[3][3];
+  double s2[3][3], s6[3][3], s12[3][3];
+  double nlrc2[3][3], plrc2[3][3];  // (3.2 sigma_rep)^2, (2.5 sigma_rep)^2
+  double k_bnd, R0sq, e_bnd_coeff, e_ang_ss_coeff, f_ang_ss_coeff;
+  double thr_rep;                   // max over types of sigma_rep2 * tol
+};
+
+struct ChainP {
+  int off, n;          // first global bead, bead count
+  int att_off, natt;   // native master entries [att_off, att_off+natt)
+  int cutoff;          // SOPG_CUTOFF_*
+  int rep_explicit;    // 1: non-native master list stored (CSR); 0: implicit
+  int nnlup, nup;
+  int phase, iup0;     // reference inlup / iup before the first step of a run
+  int vel_on;          // T >= 1e-5 (sop.cpp:286, 316)
+  int noise_mode;      // SOPG_NOISE_*
+  double boxl, qbox, lcell, ncell, qncell;
+  double a1, a2, a3, a4, a5, var, T;
+  double thr_att;      // tol * max lj_nat_pdb_dist2 of the chain
+  uint64_t seed, noise_base;  // Philox key, force evaluations done so far
+};
+
+// Per-bead reduction fields (red[g * kRed + field]).
+enum { R_EBND = 0, R_EANG, R_EATT, R_EREP, R_CONT, R_RG, R_V2, R_NLA, R_PLA, R_NLR, R_PLR, kRed = 12 };
+
+struct Dev {
+  int nb;    // total beads
+  int cap;   // non-native slots per bead
+  double *ux[2], *uy[2], *uz[2];
+  double *px[2], *py[2], *pz[2];
+  double *vx, *vy, *vz, *fx, *fy, *fz, *ix, *iy, *iz;
+  int* chain;
+  int8_t* type;
+  // bonds: incidence code = bond << 1 | side (0: bead is i, 1: bead is j)
+  int *bond_ptr, *bond_code, *bi, *bj; double* br0;
+  // angles: code = angle << 1 | side (0: bead is i, 1: bead is k)
+  int *ang_ptr, *ang_code, *gi, *gk;
+  // natives: code = master << 1 | side; partner = the other bead (global)
+  int *att_ptr, *att_code, *att_partner; uint8_t* att_flag;  // flag: bit0 NL, bit1+p PL(parity p)
+  int *am_i, *am_j; int8_t* am_t;  // am_t = itype | jtype << 2
+  double *am_r2, *am_r6, *am_r12, *am_nlrc2, *am_plrc2; float* am_rf;
+  uint8_t *am_nl, *am_pl0, *am_pl1;
+  // explicit non-natives
+  int *rep_ptr, *rep_code, *rm_i, *rm_j; int8_t* rm_t;
+  // non-native neighbor slots
+  uint32_t* slot; int* slot_k; int* cnt;
+  double *cx, *cy, *cz;  // cell coordinates (cell list)
+  double* red;           // per-bead reduction fields
+  int* err;              // bit0: slot overflow
+  ChainP* cp;
+};
+
+}  // namespace sopg

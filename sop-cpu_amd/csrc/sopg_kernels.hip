@@ -1,0 +1,652 @@
+// sopg_kernels.hip -- gfx950 kernels of the SOP hot path (FP64, no MFMA).
+//
+// Work decomposition: a group of L lanes (1..64, inside one wave64) owns one
+// bead.  Every per-bead list (bond, angle, native and non-native incidences)
+// is walked L entries at a time; each lane evaluates one pair, and the group
+// folds the pair forces into the bead's force strictly in list order with
+// ballot + cross-lane shuffles.  That order -- noise, bonds, angles, native
+// pairs, non-native pairs, each in the reference's list order -- is the order
+// in which SOP-CPU's force_eval accumulates force[bead] (energy.cpp:21-33,
+// 259-465), so the sums are bit-identical and independent of L, of the batch
+// composition and of scheduling.  No floating-point atomics anywhere.
+//
+// Each pair is evaluated by both of its beads (the reference evaluates it once
+// and applies +f / -f); both evaluations use the pair's own (i, j) orientation
+// and therefore produce the same bits.
+#include <hip/hip_runtime.h>
+#include "sopg_device.h"
+#include "sopg_internal.h"
+#include "../../include/sopg.h"
+
+namespace sopg {
+
+// ------------------------------------------------------------------ group ops
+
+template <int L>
+__device__ __forceinline__ uint64_t group_mask(bool c) {
+  if constexpr (L == 1) {
+    return c ? 1ull : 0ull;
+  } else {
+    uint64_t m = __ballot(c);
+    const int gb = (threadIdx.x & 63) & ~(L - 1);
+    if constexpr (L == 64) return m;
+    else return (m >> gb) & ((1ull << L) - 1ull);
+  }
+}
+
+// F += contributions of the group's lanes, in lane order (= list order).
+template <int L>
+__device__ __forceinline__ void fold3(double& Fx, double& Fy, double& Fz, bool c, double fx, double fy, double fz) {
+  if constexpr (L == 1) {
+    if (c) { Fx += fx; Fy += fy; Fz += fz; }
+  } else {
+    const uint64_t gm = group_mask<L>(c);
+#pragma unroll
+    for (int k = 0; k < L; k++) {
+      const double ax = __shfl(fx, k, L), ay = __shfl(fy, k, L), az = __shfl(fz, k, L);
+      if ((gm >> k) & 1ull) { Fx += ax; Fy += ay; Fz += az; }
+    }
+  }
+}
+
+// Deterministic group sum (fixed butterfly), used for energy partials only.
+template <int L>
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+  for (int s = L / 2; s >= 1; s >>= 1) v += __shfl_xor(v, s, L);
+  return v;
+}
+
+// ------------------------------------------------------------------ rare paths
+
+// energy.cpp:293: the native skip test reads lj_nat_pdb_dist2[i] with i the
+// PAIR-LIST position of the entry, not its master index.  Only reachable when
+// d2 < tol * max(lj_nat_pdb_dist2); then the position is counted exactly.
+__device__ __noinline__ bool att_tol_skip(const Dev& D, const ChainP& P, int k, int pr, double d2) {
+  const uint8_t* pl = pr ? D.am_pl1 : D.am_pl0;
+  int p = 0;
+  for (int kk = P.att_off; kk < k; kk++) p += pl[kk];
+  return d2 < kTol * D.am_r2[P.att_off + p];
+}
+
+// energy.cpp:292, 331: rep_tol comes from the LAST native pair-list entry.
+// With an empty native pair list the reference reads it uninitialised; here no
+// non-native pair is skipped in that case.
+__device__ __noinline__ bool rep_tol_skip(const Dev& D, const Tables& T, const ChainP& P, int pr, double d2) {
+  const uint8_t* pl = pr ? D.am_pl1 : D.am_pl0;
+  for (int kk = P.att_off + P.natt - 1; kk >= P.att_off; kk--) {
+    if (pl[kk]) {
+      const int tt = D.am_t[kk];
+      return d2 < T.s2[tt & 3][tt >> 2] * kTol;
+    }
+  }
+  return false;
+}
+
+__device__ __forceinline__ bool cell_adjacent(const Dev& D, const ChainP& P, int i, int j) {
+  // cell_list.cpp:24-44 with the cell indices precomputed per bead
+  const double cx = mimg(D.cx[i] - D.cx[j], P.ncell, P.qncell);
+  const double cy = mimg(D.cy[i] - D.cy[j], P.ncell, P.qncell);
+  const double cz = mimg(D.cz[i] - D.cz[j], P.ncell, P.qncell);
+  return cx >= -1.0 && cx <= 1.0 && cy >= -1.0 && cy <= 1.0 && cz >= -1.0 && cz <= 1.0;
+}
+
+// ------------------------------------------------------------------ kernels
+
+// Cell index per bead: floor(unc_pos / lcell) (cell_list.cpp:24-31).
+__global__ void k_cellcoords(Dev D, int cur, int g0, int g1) {
+  const int g = g0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= g1) return;
+  const ChainP& P = D.cp[D.chain[g]];
+  D.cx[g] = floor(D.ux[cur][g] / P.lcell);
+  D.cy[g] = floor(D.uy[cur][g] / P.lcell);
+  D.cz[g] = floor(D.uz[cur][g] / P.lcell);
+}
+
+// Neighbor-list (distance) or cell-list rebuild plus the pair-list filter at
+// the same positions.  mode < 0: each chain's own cutoff, only chains whose
+// rebuild is due at relative step rel; mode >= 0: that criterion, all chains.
+template <int L>
+__global__ void __launch_bounds__(256) k_rebuild(Dev D, Tables T, int cur, int par, int rel, int mode, int g0, int g1) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = g0 + t / L;
+  const int l = t & (L - 1);
+  if (g >= g1) return;
+  const ChainP& P = D.cp[D.chain[g]];
+  int crit = mode;
+  if (mode < 0) {
+    if (P.cutoff == SOPG_CUTOFF_NONE || (P.phase + rel) % P.nnlup != 0) return;
+    crit = P.cutoff;
+  }
+  const double* ux = D.ux[cur];
+  const double* uy = D.uy[cur];
+  const double* uz = D.uz[cur];
+  const double boxl = P.boxl, qbox = P.qbox;
+  uint8_t* aplw = par ? D.am_pl1 : D.am_pl0;
+
+  // native contacts (neighbor_list.cpp:17-67 / cell_list.cpp:16-57)
+  const int a0 = D.att_ptr[g], a1 = D.att_ptr[g + 1];
+  for (int q = a0 + l; q < a1; q += L) {
+    const int code = D.att_code[q], k = code >> 1, side = code & 1, o = D.att_partner[q];
+    const int i = side ? o : g, j = side ? g : o;
+    const double dx = mimg(ux[j] - ux[i], boxl, qbox);
+    const double dy = mimg(uy[j] - uy[i], boxl, qbox);
+    const double dz = mimg(uz[j] - uz[i], boxl, qbox);
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    bool nl, pl;
+    if (crit == SOPG_CUTOFF_NEIGHBOR) { nl = d2 < D.am_nlrc2[k]; pl = nl && d2 < D.am_plrc2[k]; }
+    else if (crit == SOPG_CUTOFF_CELL) { nl = cell_adjacent(D, P, i, j); pl = nl && d2 < D.am_plrc2[k]; }
+    else { nl = true; pl = true; }
+    D.att_flag[q] = (uint8_t)((nl ? 1 : 0) | (pl ? (2 << par) : 0));
+    if (side == 0) { D.am_nl[k] = nl; aplw[k] = pl; }
+  }
+
+  // non-native contacts: stable (ballot + prefix) compaction into the slots
+  const int b = g - P.off;
+  uint32_t* S = D.slot + (size_t)g * D.cap;
+  int* SK = D.slot_k ? D.slot_k + (size_t)g * D.cap : nullptr;
+  int cnt = 0;
+  if (!P.rep_explicit) {
+    // implicit master list: all j-i>2 pairs in (i,j) order minus natives; for
+    // bead b that is every partner c (|c-b|>2, not native) in ascending order.
+    int np = a0;  // per-lane cursor into b's natives (ascending partners)
+    for (int base = 0; base < P.n; base += L) {
+      const int c = base + l;
+      bool keep = false;
+      uint32_t w = 0;
+      if (c < P.n && (c - b > 2 || b - c > 2)) {
+        const int gc = P.off + c;
+        while (np < a1 && D.att_partner[np] < gc) np++;
+        if (!(np < a1 && D.att_partner[np] == gc)) {
+          const int side = (gc < g) ? 1 : 0;
+          const int i = side ? gc : g, j = side ? g : gc;
+          const int it = D.type[i], jt = D.type[j];
+          const double dx = mimg(ux[j] - ux[i], boxl, qbox);
+          const double dy = mimg(uy[j] - uy[i], boxl, qbox);
+          const double dz = mimg(uz[j] - uz[i], boxl, qbox);
+          const double d2 = dx * dx + dy * dy + dz * dz;
+          bool pl;
+          if (crit == SOPG_CUTOFF_NEIGHBOR) { keep = d2 < T.nlrc2[it][jt]; pl = keep && d2 < T.plrc2[it][jt]; }
+          else if (crit == SOPG_CUTOFF_CELL) { keep = cell_adjacent(D, P, i, j); pl = keep && d2 < T.plrc2[it][jt]; }
+          else { keep = true; pl = true; }
+          w = slot_make((uint32_t)c, (uint32_t)side, (uint32_t)it, (uint32_t)jt) | (pl ? (1u << (30 + par)) : 0u);
+        }
+      }
+      const uint64_t m = group_mask<L>(keep);
+      if (keep) {
+        const int idx = cnt + __popcll(m & ((1ull << l) - 1ull));
+        if (idx < D.cap) S[idx] = w;
+      }
+      cnt += __popcll(m);
+    }
+  } else {
+    const int r0 = D.rep_ptr[g], r1 = D.rep_ptr[g + 1];
+    for (int base = r0; base < r1; base += L) {
+      const int q = base + l;
+      bool keep = false;
+      uint32_t w = 0;
+      int kk = 0;
+      if (q < r1) {
+        const int code = D.rep_code[q], side = code & 1;
+        kk = code >> 1;
+        const int i = D.rm_i[kk], j = D.rm_j[kk];
+        const int tt = D.rm_t[kk], it = tt & 3, jt = tt >> 2;
+        const double dx = mimg(ux[j] - ux[i], boxl, qbox);
+        const double dy = mimg(uy[j] - uy[i], boxl, qbox);
+        const double dz = mimg(uz[j] - uz[i], boxl, qbox);
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        bool pl;
+        if (crit == SOPG_CUTOFF_NEIGHBOR) { keep = d2 < T.nlrc2[it][jt]; pl = keep && d2 < T.plrc2[it][jt]; }
+        else if (crit == SOPG_CUTOFF_CELL) { keep = cell_adjacent(D, P, i, j); pl = keep && d2 < T.plrc2[it][jt]; }
+        else { keep = true; pl = true; }
+        const int o = side ? i : j;
+        w = slot_make((uint32_t)(o - P.off), (uint32_t)side, (uint32_t)it, (uint32_t)jt) | (pl ? (1u << (30 + par)) : 0u);
+      }
+      const uint64_t m = group_mask<L>(keep);
+      if (keep) {
+        const int idx = cnt + __popcll(m & ((1ull << l) - 1ull));
+        if (idx < D.cap) { S[idx] = w; if (SK) SK[idx] = kk; }
+      }
+      cnt += __popcll(m);
+    }
+  }
+  if (l == 0) {
+    D.cnt[g] = cnt < D.cap ? cnt : D.cap;
+    if (cnt > D.cap) atomicOr(D.err, 1);
+  }
+}
+
+// Pair-list filter at the current positions (sop.cpp:473-553), parity par.
+template <int L>
+__global__ void __launch_bounds__(256) k_pairfilter(Dev D, Tables T, int cur, int par, int g0, int g1) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = g0 + t / L;
+  const int l = t & (L - 1);
+  if (g >= g1) return;
+  const ChainP& P = D.cp[D.chain[g]];
+  const double* ux = D.ux[cur];
+  const double* uy = D.uy[cur];
+  const double* uz = D.uz[cur];
+  const double boxl = P.boxl, qbox = P.qbox;
+  uint8_t* aplw = par ? D.am_pl1 : D.am_pl0;
+  const uint8_t pbit = (uint8_t)(2 << par);
+  const int a0 = D.att_ptr[g], a1 = D.att_ptr[g + 1];
+  for (int q = a0 + l; q < a1; q += L) {
+    const int code = D.att_code[q], k = code >> 1, side = code & 1, o = D.att_partner[q];
+    const uint8_t f = D.att_flag[q];
+    bool pl = false;
+    if (f & 1) {
+      const int i = side ? o : g, j = side ? g : o;
+      const double dx = mimg(ux[j] - ux[i], boxl, qbox);
+      const double dy = mimg(uy[j] - uy[i], boxl, qbox);
+      const double dz = mimg(uz[j] - uz[i], boxl, qbox);
+      const double d2 = dx * dx + dy * dy + dz * dz;
+      pl = (P.cutoff == SOPG_CUTOFF_NONE) ? true : d2 < D.am_plrc2[k];
+    }
+    D.att_flag[q] = (uint8_t)((f & ~pbit) | (pl ? pbit : 0));
+    if (side == 0) aplw[k] = pl;
+  }
+  const int cnt = D.cnt[g];
+  uint32_t* S = D.slot + (size_t)g * D.cap;
+  const uint32_t sbit = 1u << (30 + par);
+  for (int q = l; q < cnt; q += L) {
+    const uint32_t w = S[q];
+    const int o = P.off + (int)(w & kPartnerMask), side = (w >> 25) & 1, it = (w >> 26) & 3, jt = (w >> 28) & 3;
+    const int i = side ? o : g, j = side ? g : o;
+    const double dx = mimg(ux[j] - ux[i], boxl, qbox);
+    const double dy = mimg(uy[j] - uy[i], boxl, qbox);
+    const double dz = mimg(uz[j] - uz[i], boxl, qbox);
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    const bool pl = (P.cutoff == SOPG_CUTOFF_NONE) ? true : d2 < T.plrc2[it][jt];
+    S[q] = (w & ~sbit) | (pl ? sbit : 0u);
+  }
+}
+
+// Position half of underdamped_iteration (sop.cpp:288-310): incr, pos, unc_pos.
+__global__ void k_posupdate(Dev D, int src, int dst, int g0, int g1) {
+  const int g = g0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= g1) return;
+  const ChainP& P = D.cp[D.chain[g]];
+  const double ix = P.a1 * D.vx[g] + P.a2 * D.fx[g];
+  const double iy = P.a1 * D.vy[g] + P.a2 * D.fy[g];
+  const double iz = P.a1 * D.vz[g] + P.a2 * D.fz[g];
+  D.ix[g] = ix; D.iy[g] = iy; D.iz[g] = iz;
+  D.px[dst][g] = mimg(D.px[src][g] + ix, P.boxl, P.qbox);
+  D.py[dst][g] = mimg(D.py[src][g] + iy, P.boxl, P.qbox);
+  D.pz[dst][g] = mimg(D.pz[src][g] + iz, P.boxl, P.qbox);
+  D.ux[dst][g] = D.ux[src][g] + ix;
+  D.uy[dst][g] = D.uy[src][g] + iy;
+  D.uz[dst][g] = D.uz[src][g] + iz;
+}
+
+// Velocity half of underdamped_iteration (sop.cpp:316-328).
+__global__ void k_velupdate(Dev D, int g0, int g1) {
+  const int g = g0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= g1) return;
+  const ChainP& P = D.cp[D.chain[g]];
+  if (!P.vel_on) return;
+  D.vx[g] = P.a3 * D.ix[g] + P.a4 * D.fx[g];
+  D.vy[g] = P.a3 * D.iy[g] + P.a4 * D.fy[g];
+  D.vz[g] = P.a3 * D.iz[g] + P.a4 * D.fz[g];
+}
+
+// One force evaluation (force_eval, energy.cpp:21-33) at unc_pos[cur] with the
+// pair list of parity pr, fused with what the step loop needs around it.
+// flags: F_STORE   store forces (else energy only)
+//        F_VEL     velocity update (sop.cpp:320-328) when the chain's T allows
+//        F_ENERGY  energy partials (energy.cpp:115-257) on chains whose record
+//                  falls on this step (all chains if F_ENERGY_ALL)
+// NEXT:  also the next step's pair list (parity pr^1) from these positions
+//        and the next position update into buffer cur^1.
+enum { F_STORE = 1, F_VEL = 2, F_ENERGY = 4, F_ENERGY_ALL = 8 };
+
+template <int L, bool NEXT>
+__global__ void __launch_bounds__(256) k_force(Dev D, Tables T, int cur, int pr, int rel, int flags,
+                                               const double* __restrict__ noise, int g0, int g1) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = g0 + t / L;
+  const int l = t & (L - 1);
+  if (g >= g1) return;
+  const ChainP& P = D.cp[D.chain[g]];
+  const double* __restrict__ ux = D.ux[cur];
+  const double* __restrict__ uy = D.uy[cur];
+  const double* __restrict__ uz = D.uz[cur];
+  const double boxl = P.boxl, qbox = P.qbox;
+  const int pw = pr ^ 1;
+  const bool energy = (flags & F_ENERGY_ALL) || ((flags & F_ENERGY) && ((P.iup0 + rel) % P.nup == 0));
+  double eb = 0.0, ea = 0.0, eatt = 0.0, erep = 0.0;
+
+  // random force (energy.cpp:449-465): force = 0 + var * gasdev(), x, y, z
+  double gx = 0.0, gy = 0.0, gz = 0.0;
+  if (!(flags & F_STORE)) {
+    // energy only: no random force
+  } else if (P.noise_mode == SOPG_NOISE_REPLAY) {
+    gx = noise[3 * (size_t)g];
+    gy = noise[3 * (size_t)g + 1];
+    gz = noise[3 * (size_t)g + 2];
+  } else {
+    const uint64_t ctr = P.noise_base + (uint64_t)rel;
+    const uint32_t bead = (uint32_t)(g - P.off);
+    if constexpr (L == 1) {
+      double u;
+      philox_gauss2(P.seed, ctr, bead, 0, gx, gy);
+      philox_gauss2(P.seed, ctr, bead, 1, gz, u);
+    } else {
+      double a, c;
+      philox_gauss2(P.seed, ctr, bead, (uint32_t)(l & 1), a, c);
+      gx = __shfl(a, 0, L);
+      gy = __shfl(c, 0, L);
+      gz = __shfl(a, 1, L);
+    }
+  }
+  double Fx = 0.0 + P.var * gx;
+  double Fy = 0.0 + P.var * gy;
+  double Fz = 0.0 + P.var * gz;
+
+  // FENE bonds (energy.cpp:402-447): F_i -= f, F_j += f
+  {
+    const int p0 = D.bond_ptr[g], p1 = D.bond_ptr[g + 1];
+    for (int base = p0; base < p1; base += L) {
+      const int q = base + l;
+      const bool has = q < p1;
+      double fx = 0.0, fy = 0.0, fz = 0.0;
+      if (has) {
+        const int code = D.bond_code[q], kb = code >> 1, side = code & 1;
+        const int i = D.bi[kb], j = D.bj[kb];
+        const double dx = mimg(ux[j] - ux[i], boxl, qbox);
+        const double dy = mimg(uy[j] - uy[i], boxl, qbox);
+        const double dz = mimg(uz[j] - uz[i], boxl, qbox);
+        const double d = sqrt(dx * dx + dy * dy + dz * dz);
+        const double dev = d - D.br0[kb];
+        const double dev2 = dev * dev;
+        const double temp = -T.k_bnd * dev / d / (1.0 - dev2 / T.R0sq);
+        fx = temp * dx; fy = temp * dy; fz = temp * dz;
+        if (side == 0) { fx = -fx; fy = -fy; fz = -fz; }
+        if (energy && side == 0) eb += log1p(-dev * dev / T.R0sq);
+      }
+      fold3<L>(Fx, Fy, Fz, has, fx, fy, fz);
+    }
+  }
+  // soft-sphere (i, i+2) repulsion (energy.cpp:354-400): F_i -= f, F_k += f
+  {
+    const int p0 = D.ang_ptr[g], p1 = D.ang_ptr[g + 1];
+    for (int base = p0; base < p1; base += L) {
+      const int q = base + l;
+      const bool has = q < p1;
+      double fx = 0.0, fy = 0.0, fz = 0.0;
+      if (has) {
+        const int code = D.ang_code[q], ka = code >> 1, side = code & 1;
+        const int i = D.gi[ka], k = D.gk[ka];
+        const double dx = mimg(ux[k] - ux[i], boxl, qbox);
+        const double dy = mimg(uy[k] - uy[i], boxl, qbox);
+        const double dz = mimg(uz[k] - uz[i], boxl, qbox);
+        const double d = sqrt(dx * dx + dy * dy + dz * dz);
+        const double d8 = pow8_cr(d);
+        const double co1 = T.f_ang_ss_coeff / d8;
+        fx = co1 * dx; fy = co1 * dy; fz = co1 * dz;
+        if (side == 0) { fx = -fx; fy = -fy; fz = -fz; }
+        if (energy && side == 0) ea += T.e_ang_ss_coeff / pow6_cr(d);
+      }
+      fold3<L>(Fx, Fy, Fz, has, fx, fy, fz);
+    }
+  }
+  // native LJ 12-6 over the pair list (energy.cpp:274-311): F_i += f, F_j -= f
+  {
+    const int p0 = D.att_ptr[g], p1 = D.att_ptr[g + 1];
+    uint8_t* aplw = pw ? D.am_pl1 : D.am_pl0;
+    for (int base = p0; base < p1; base += L) {
+      const int q = base + l;
+      bool contrib = false;
+      double fx = 0.0, fy = 0.0, fz = 0.0;
+      if (q < p1) {
+        const uint8_t f = D.att_flag[q];
+        const int code = D.att_code[q], k = code >> 1, side = code & 1;
+        if (f & 1) {
+          const int o = D.att_partner[q];
+          const int i = side ? o : g, j = side ? g : o;
+          const double dx = mimg(ux[j] - ux[i], boxl, qbox);
+          const double dy = mimg(uy[j] - uy[i], boxl, qbox);
+          const double dz = mimg(uz[j] - uz[i], boxl, qbox);
+          const double d2 = dx * dx + dy * dy + dz * dz;
+          if ((f >> (1 + pr)) & 1) {
+            const int tt = D.am_t[k], it = tt & 3, jt = tt >> 2;
+            const double d6 = d2 * d2 * d2;
+            const double d12 = d6 * d6;
+            const bool skip = (d2 < P.thr_att) && att_tol_skip(D, P, k, pr, d2);
+            if (!skip) {
+              const double co1 = T.fatt[it][jt] / d2 * ((D.am_r12[k] / d12) - (D.am_r6[k] / d6));
+              fx = co1 * dx; fy = co1 * dy; fz = co1 * dz;
+              if (side == 1) { fx = -fx; fy = -fy; fz = -fz; }
+              contrib = true;
+            }
+            if (energy && side == 0)
+              eatt += T.catt[it][jt] * (D.am_r12[k] / d12) - 2.0 * (D.am_r6[k] / d6);
+          }
+          if constexpr (NEXT) {
+            const bool npl = (P.cutoff == SOPG_CUTOFF_NONE) || d2 < D.am_plrc2[k];
+            const uint8_t nf = (uint8_t)((f & ~(2 << pw)) | (npl ? (2 << pw) : 0));
+            if (nf != f) D.att_flag[q] = nf;
+            if (side == 0) aplw[k] = npl;
+          }
+        } else if constexpr (NEXT) {
+          if (side == 0) aplw[k] = 0;
+        }
+      }
+      fold3<L>(Fx, Fy, Fz, contrib, fx, fy, fz);
+    }
+  }
+  // non-native repulsion over the pair list (energy.cpp:313-350): F_i += f, F_j -= f
+  {
+    const int cnt = D.cnt[g];
+    uint32_t* S = D.slot + (size_t)g * D.cap;
+    const uint32_t rbit = 1u << (30 + pr), wbit = 1u << (30 + pw);
+    for (int base = 0; base < cnt; base += L) {
+      const int q = base + l;
+      bool contrib = false;
+      double fx = 0.0, fy = 0.0, fz = 0.0;
+      if (q < cnt) {
+        const uint32_t w = S[q];
+        const int o = P.off + (int)(w & kPartnerMask), side = (w >> 25) & 1, it = (w >> 26) & 3, jt = (w >> 28) & 3;
+        const int i = side ? o : g, j = side ? g : o;
+        const double dx = mimg(ux[j] - ux[i], boxl, qbox);
+        const double dy = mimg(uy[j] - uy[i], boxl, qbox);
+        const double dz = mimg(uz[j] - uz[i], boxl, qbox);
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        if (w & rbit) {
+          const double d6 = d2 * d2 * d2;
+          const double d12 = d6 * d6;
+          const bool skip = (d2 < T.thr_rep) && rep_tol_skip(D, T, P, pr, d2);
+          if (!skip) {
+            const double co1 = T.frep[it][jt] / d2 * (2.0 * T.s12[it][jt] / d12 + T.s6[it][jt] / d6);
+            fx = co1 * dx; fy = co1 * dy; fz = co1 * dz;
+            if (side == 1) { fx = -fx; fy = -fy; fz = -fz; }
+            contrib = true;
+          }
+          if (energy && side == 0) erep += T.crep[it][jt] * (T.s12[it][jt] / d12 + T.s6[it][jt] / d6);
+        }
+        if constexpr (NEXT) {
+          const bool npl = (P.cutoff == SOPG_CUTOFF_NONE) || d2 < T.plrc2[it][jt];
+          const uint32_t nw = (w & ~wbit) | (npl ? wbit : 0u);
+          if (nw != w) S[q] = nw;
+        }
+      }
+      fold3<L>(Fx, Fy, Fz, contrib, fx, fy, fz);
+    }
+  }
+
+  if (energy) {
+    eb = group_sum<L>(eb); ea = group_sum<L>(ea); eatt = group_sum<L>(eatt); erep = group_sum<L>(erep);
+  }
+  if (l != 0) return;
+  if (energy) {
+    double* r = D.red + (size_t)g * kRed;
+    r[R_EBND] = eb; r[R_EANG] = ea; r[R_EATT] = eatt; r[R_EREP] = erep;
+  }
+  if (!(flags & F_STORE)) return;
+  D.fx[g] = Fx; D.fy[g] = Fy; D.fz[g] = Fz;
+  double vx = D.vx[g], vy = D.vy[g], vz = D.vz[g];
+  if ((flags & F_VEL) && P.vel_on) {
+    vx = P.a3 * D.ix[g] + P.a4 * Fx;
+    vy = P.a3 * D.iy[g] + P.a4 * Fy;
+    vz = P.a3 * D.iz[g] + P.a4 * Fz;
+    D.vx[g] = vx; D.vy[g] = vy; D.vz[g] = vz;
+  }
+  if constexpr (NEXT) {
+    // the next step's position half (sop.cpp:288-310) into buffer cur^1
+    const int nx = cur ^ 1;
+    const double ix = P.a1 * vx + P.a2 * Fx;
+    const double iy = P.a1 * vy + P.a2 * Fy;
+    const double iz = P.a1 * vz + P.a2 * Fz;
+    D.ix[g] = ix; D.iy[g] = iy; D.iz[g] = iz;
+    D.px[nx][g] = mimg(D.px[cur][g] + ix, boxl, qbox);
+    D.py[nx][g] = mimg(D.py[cur][g] + iy, boxl, qbox);
+    D.pz[nx][g] = mimg(D.pz[cur][g] + iz, boxl, qbox);
+    D.ux[nx][g] = ux[g] + ix;
+    D.uy[nx][g] = uy[g] + iy;
+    D.uz[nx][g] = uz[g] + iz;
+  }
+}
+
+// Observables (sop.cpp:200-280) and list counts, per bead.
+enum { OBS_PHYS = 1, OBS_COUNTS = 2 };
+__global__ void k_observe(Dev D, int cur, int par, int rel, int all, int what, int g0, int g1) {
+  const int g = g0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= g1) return;
+  const ChainP& P = D.cp[D.chain[g]];
+  if (!all && (P.iup0 + rel) % P.nup != 0) return;
+  const double* ux = D.ux[cur];
+  const double* uy = D.uy[cur];
+  const double* uz = D.uz[cur];
+  const double boxl = P.boxl, qbox = P.qbox;
+  const uint8_t* apl = par ? D.am_pl1 : D.am_pl0;
+  double* r = D.red + (size_t)g * kRed;
+  int cont = 0, nla = 0, pla = 0, nlr = 0, plr = 0;
+  const int a0 = D.att_ptr[g], a1 = D.att_ptr[g + 1];
+  for (int q = a0; q < a1; q++) {
+    const int code = D.att_code[q];
+    if (code & 1) continue;  // each native pair once, from its i bead
+    const int k = code >> 1, j = D.att_partner[q];
+    nla += D.am_nl[k];
+    pla += apl[k];
+    if (!(what & OBS_PHYS)) continue;
+    // Q: d / (float) r_ij < 1.25 with d from unc_pos[i] - unc_pos[j] (sop.cpp:223-238)
+    const double dx = mimg(ux[g] - ux[j], boxl, qbox);
+    const double dy = mimg(uy[g] - uy[j], boxl, qbox);
+    const double dz = mimg(uz[g] - uz[j], boxl, qbox);
+    const double d = sqrt(dx * dx + dy * dy + dz * dz);
+    if (d / (double)D.am_rf[k] < 1.25) cont++;
+  }
+  const int cnt = D.cnt[g];
+  const uint32_t* S = D.slot + (size_t)g * D.cap;
+  const uint32_t pbit = 1u << (30 + par);
+  for (int q = 0; q < cnt; q++) {
+    const uint32_t w = S[q];
+    if ((w >> 25) & 1) continue;
+    nlr++;
+    if (w & pbit) plr++;
+  }
+  if (what & OBS_COUNTS) { r[R_NLA] = nla; r[R_PLA] = pla; r[R_NLR] = nlr; r[R_PLR] = plr; }
+  if (!(what & OBS_PHYS)) return;
+  // Rg^2 pair sum over j > i (sop.cpp:245-258)
+  double rg = 0.0;
+  const int b = g - P.off;
+  for (int c = b + 1; c < P.n; c++) {
+    const int h = P.off + c;
+    const double dx = mimg(ux[g] - ux[h], boxl, qbox);
+    const double dy = mimg(uy[g] - uy[h], boxl, qbox);
+    const double dz = mimg(uz[g] - uz[h], boxl, qbox);
+    rg += (dx * dx + dy * dy + dz * dz);
+  }
+  const double v2 = D.vx[g] * D.vx[g] + D.vy[g] * D.vy[g] + D.vz[g] * D.vz[g];
+  r[R_CONT] = cont; r[R_RG] = rg; r[R_V2] = v2;
+}
+
+// Per-chain sums of the per-bead fields: one block per chain, fixed order.
+__global__ void __launch_bounds__(256) k_chain_reduce(Dev D, int c0, double* out) {
+  const int c = c0 + blockIdx.x;
+  const ChainP& P = D.cp[c];
+  __shared__ double sh[256];
+  for (int f = 0; f < kRed; f++) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < P.n; b += blockDim.x) s += D.red[(size_t)(P.off + b) * kRed + f];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[(size_t)blockIdx.x * kRed + f] = sh[0];
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------ launchers
+
+static inline unsigned blocks_for(long threads) { return (unsigned)((threads + 255) / 256); }
+
+template <int L>
+static void launch_rebuild_L(const Dev& D, const Tables& T, int cur, int par, int rel, int mode, int g0, int g1,
+                             hipStream_t s) {
+  hipLaunchKernelGGL(k_rebuild<L>, dim3(blocks_for((long)(g1 - g0) * L)), dim3(256), 0, s, D, T, cur, par, rel,
+                     mode, g0, g1);
+}
+template <int L>
+static void launch_pairfilter_L(const Dev& D, const Tables& T, int cur, int par, int g0, int g1, hipStream_t s) {
+  hipLaunchKernelGGL(k_pairfilter<L>, dim3(blocks_for((long)(g1 - g0) * L)), dim3(256), 0, s, D, T, cur, par, g0,
+                     g1);
+}
+template <int L>
+static void launch_force_L(const Dev& D, const Tables& T, int cur, int pr, int rel, int flags, bool next,
+                           const double* noise, int g0, int g1, hipStream_t s) {
+  const dim3 grid(blocks_for((long)(g1 - g0) * L)), block(256);
+  if (next)
+    hipLaunchKernelGGL((k_force<L, true>), grid, block, 0, s, D, T, cur, pr, rel, flags, noise, g0, g1);
+  else
+    hipLaunchKernelGGL((k_force<L, false>), grid, block, 0, s, D, T, cur, pr, rel, flags, noise, g0, g1);
+}
+
+#define SOPG_DISPATCH_L(L, CALL)          \
+  switch (L) {                            \
+    case 1: CALL(1); break;               \
+    case 2: CALL(2); break;               \
+    case 4: CALL(4); break;               \
+    case 8: CALL(8); break;               \
+    case 16: CALL(16); break;             \
+    case 32: CALL(32); break;             \
+    default: CALL(64); break;             \
+  }
+
+void launch_cellcoords(const Dev& D, int cur, int g0, int g1, hipStream_t s) {
+  hipLaunchKernelGGL(k_cellcoords, dim3(blocks_for(g1 - g0)), dim3(256), 0, s, D, cur, g0, g1);
+}
+void launch_rebuild(int L, const Dev& D, const Tables& T, int cur, int par, int rel, int mode, int g0, int g1,
+                    hipStream_t s) {
+#define C_(X) launch_rebuild_L<X>(D, T, cur, par, rel, mode, g0, g1, s)
+  SOPG_DISPATCH_L(L, C_)
+#undef C_
+}
+void launch_pairfilter(int L, const Dev& D, const Tables& T, int cur, int par, int g0, int g1, hipStream_t s) {
+#define C_(X) launch_pairfilter_L<X>(D, T, cur, par, g0, g1, s)
+  SOPG_DISPATCH_L(L, C_)
+#undef C_
+}
+void launch_force(int L, const Dev& D, const Tables& T, int cur, int pr, int rel, int flags, bool next,
+                  const double* noise, int g0, int g1, hipStream_t s) {
+#define C_(X) launch_force_L<X>(D, T, cur, pr, rel, flags, next, noise, g0, g1, s)
+  SOPG_DISPATCH_L(L, C_)
+#undef C_
+}
+void launch_posupdate(const Dev& D, int src, int dst, int g0, int g1, hipStream_t s) {
+  hipLaunchKernelGGL(k_posupdate, dim3(blocks_for(g1 - g0)), dim3(256), 0, s, D, src, dst, g0, g1);
+}
+void launch_velupdate(const Dev& D, int g0, int g1, hipStream_t s) {
+  hipLaunchKernelGGL(k_velupdate, dim3(blocks_for(g1 - g0)), dim3(256), 0, s, D, g0, g1);
+}
+void launch_observe(const Dev& D, int cur, int par, int rel, int all, int what, int g0, int g1, hipStream_t s) {
+  hipLaunchKernelGGL(k_observe, dim3(blocks_for(g1 - g0)), dim3(256), 0, s, D, cur, par, rel, all, what, g0, g1);
+}
+void launch_chain_reduce(const Dev& D, int c0, int nc, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_chain_reduce, dim3(nc), dim3(256), 0, s, D, c0, out);
+}
+
+}  // namespace sopg

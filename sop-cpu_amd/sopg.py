@@ -1,0 +1,367 @@
+"""ctypes binding of libsopg.so (the C ABI declared in include/sopg.h).
+
+This is a thin host-side convenience layer used by the tests and bench.py; the
+product interface is the C ABI itself.  Loading fails loudly when the built
+library is missing: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsopg.so")
+
+OK = 0
+CUTOFF_NONE, CUTOFF_NEIGHBOR, CUTOFF_CELL = 0, 1, 2
+NOISE_REPLAY, NOISE_PHILOX = 0, 1
+LIST_NL_ATT, LIST_NL_REP, LIST_PL_ATT, LIST_PL_REP = 0, 1, 2, 3
+
+_D33 = (C.c_double * 3) * 3
+
+
+class Tables(C.Structure):
+    _fields_ = [("coeff_att", _D33), ("coeff_rep", _D33), ("force_coeff_att", _D33),
+                ("force_coeff_rep", _D33), ("sigma_rep", _D33), ("k_bnd", C.c_double),
+                ("R0sq", C.c_double), ("e_bnd_coeff", C.c_double), ("e_ang_ss_coeff", C.c_double),
+                ("f_ang_ss_coeff", C.c_double)]
+
+
+class ChainParams(C.Structure):
+    _fields_ = [("T", C.c_double), ("zeta", C.c_double), ("h", C.c_double),
+                ("a1", C.c_double), ("a2", C.c_double), ("a3", C.c_double), ("a4", C.c_double),
+                ("a5", C.c_double), ("boxl", C.c_double), ("ncell", C.c_double), ("lcell", C.c_double),
+                ("cutoff", C.c_int), ("nnlup", C.c_int), ("nup", C.c_int), ("noise_mode", C.c_int),
+                ("seed", C.c_uint64)]
+
+
+_IP = C.POINTER(C.c_int)
+_DP = C.POINTER(C.c_double)
+
+
+class Topology(C.Structure):
+    _fields_ = [("nbead", C.c_int),
+                ("nbnd", C.c_int), ("ibnd", _IP), ("jbnd", _IP), ("r0bnd", _DP),
+                ("nang", C.c_int), ("iang", _IP), ("jang", _IP), ("kang", _IP),
+                ("ncon_att", C.c_int), ("att_i", _IP), ("att_j", _IP), ("att_it", _IP), ("att_jt", _IP),
+                ("att_r", _DP),
+                ("ncon_rep", C.c_int), ("rep_i", _IP), ("rep_j", _IP), ("rep_it", _IP), ("rep_jt", _IP),
+                ("bead_type", _IP)]
+
+
+class Energies(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("e_bnd", "e_ang_ss", "e_vdw_rr_att", "e_vdw_rr_rep",
+                                           "e_vdw_rr", "rna_etot")]
+
+
+class Observables(C.Structure):
+    _fields_ = [("Q", C.c_double), ("contct_nat", C.c_int), ("rgsq", C.c_double), ("kinT", C.c_double)]
+
+
+class Record(C.Structure):
+    _fields_ = [("chain", C.c_int), ("istep", C.c_double), ("T", C.c_double), ("kinT", C.c_double),
+                ("e_bnd", C.c_double), ("e_ang_ss", C.c_double), ("e_vdw_rr", C.c_double),
+                ("rna_etot", C.c_double), ("Q", C.c_double), ("contct_nat", C.c_int), ("rgsq", C.c_double)]
+
+
+# every entry point of include/sopg.h and include/sopgen.h
+EXPORTS = [
+    "sopg_create", "sopg_destroy", "sopg_last_error", "sopg_default_tables", "sopg_set_tables",
+    "sopg_set_lanes", "sopg_set_capacity", "sopg_add_chain", "sopg_set_chain_params", "sopg_put_state",
+    "sopg_get_state", "sopg_nchains", "sopg_chain_nbead", "sopg_update_neighbor_list",
+    "sopg_update_cell_list", "sopg_update_pair_list", "sopg_force_eval", "sopg_energy_eval",
+    "sopg_underdamped_iteration", "sopg_calculate_observables", "sopg_get_list", "sopg_run",
+    "sopg_set_step", "sopg_sync",
+    "sopgen_saw", "sopgen_native", "sopgen_maxwell", "sopgen_check", "sopgen_write_topology",
+    "sopgen_write_xyz",
+]
+
+_lib = None
+
+
+def lib():
+    """Load libsopg.so once; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.sopg_create.argtypes = [C.c_int, C.POINTER(vp)]
+        L.sopg_destroy.argtypes = [vp]
+        L.sopg_last_error.argtypes = [vp]
+        L.sopg_last_error.restype = C.c_char_p
+        L.sopg_default_tables.argtypes = [C.POINTER(Tables)]
+        L.sopg_set_tables.argtypes = [vp, C.POINTER(Tables)]
+        L.sopg_set_lanes.argtypes = [vp, C.c_int]
+        L.sopg_set_capacity.argtypes = [vp, C.c_int]
+        L.sopg_add_chain.argtypes = [vp, C.POINTER(Topology), C.POINTER(ChainParams), _IP]
+        L.sopg_set_chain_params.argtypes = [vp, C.c_int, C.POINTER(ChainParams)]
+        L.sopg_put_state.argtypes = [vp, C.c_int, vp, vp, vp, vp]
+        L.sopg_get_state.argtypes = [vp, C.c_int, vp, vp, vp, vp]
+        L.sopg_nchains.argtypes = [vp]
+        L.sopg_chain_nbead.argtypes = [vp, C.c_int]
+        for f in ("sopg_update_neighbor_list", "sopg_update_cell_list", "sopg_update_pair_list"):
+            getattr(L, f).argtypes = [vp, C.c_int, _IP, _IP]
+        L.sopg_force_eval.argtypes = [vp, C.c_int, vp]
+        L.sopg_energy_eval.argtypes = [vp, C.c_int, C.POINTER(Energies)]
+        L.sopg_underdamped_iteration.argtypes = [vp, C.c_int, vp]
+        L.sopg_calculate_observables.argtypes = [vp, C.c_int, C.POINTER(Observables)]
+        L.sopg_get_list.argtypes = [vp, C.c_int, C.c_int, _IP, _IP, C.c_long, C.POINTER(C.c_long)]
+        L.sopg_run.argtypes = [vp, C.c_long, vp, C.POINTER(Record), C.c_long, C.POINTER(C.c_long)]
+        L.sopg_set_step.argtypes = [vp, C.c_int, C.c_double, C.c_int, C.c_int]
+        L.sopg_sync.argtypes = [vp]
+        L.sopgen_saw.argtypes = [C.c_int, C.c_uint64, vp]
+        L.sopgen_native.argtypes = [C.c_int, C.c_uint64, vp]
+        L.sopgen_maxwell.argtypes = [C.c_int, C.c_double, C.c_uint64, vp]
+        L.sopgen_check.argtypes = [C.c_int, vp, _DP, _DP, _DP]
+        L.sopgen_write_topology.argtypes = [C.c_int, vp, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p,
+                                            C.c_double]
+        L.sopgen_write_xyz.argtypes = [C.c_int, vp, C.c_char_p]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _ip(a):
+    return a.ctypes.data_as(_IP)
+
+
+def _dp(a):
+    return a.ctypes.data_as(_DP)
+
+
+class SopgError(RuntimeError):
+    pass
+
+
+# ------------------------------------------------------------------ generator
+
+def gen_native(n, seed):
+    x = np.zeros(3 * n)
+    rc = lib().sopgen_native(n, seed, _ptr(x))
+    if rc:
+        raise SopgError(f"sopgen_native({n}, {seed}) failed: {rc}")
+    return x
+
+
+def gen_saw(n, seed):
+    x = np.zeros(3 * n)
+    rc = lib().sopgen_saw(n, seed, _ptr(x))
+    if rc:
+        raise SopgError(f"sopgen_saw failed: {rc}")
+    return x
+
+
+def gen_maxwell(n, kT, seed):
+    v = np.zeros(3 * n)
+    rc = lib().sopgen_maxwell(n, kT, seed, _ptr(v))
+    if rc:
+        raise SopgError("sopgen_maxwell failed")
+    return v
+
+
+def write_inputs(xyz, directory, vel=None, rcut_nat=8.0):
+    """Write bonds/angles/vdw/init (+ native.xyz, vel.xyz) in the reference's formats."""
+    n = len(xyz) // 3
+    os.makedirs(directory, exist_ok=True)
+    p = lambda f: os.path.join(directory, f).encode()
+    natt = lib().sopgen_write_topology(n, _ptr(xyz), p("bonds.dat"), p("angles.dat"), p("vdw.dat"),
+                                       p("init.dat"), rcut_nat)
+    if natt < 0:
+        raise SopgError("write_topology failed")
+    lib().sopgen_write_xyz(n, _ptr(xyz), p("native.xyz"))
+    if vel is not None:
+        lib().sopgen_write_xyz(n, _ptr(vel), p("vel.xyz"))
+    return natt
+
+
+def chain_topology(xyz, rcut_nat=8.0):
+    """Topology arrays of a generated chain (as the loaders would read its files):
+    bonds (i,i+1), angles (i,i+1,i+2), natives = pairs j-i>2 with r < rcut (lexicographic).
+    The r values are the %.17g round trip of the native distances, exactly like the files."""
+    X = np.asarray(xyz).reshape(-1, 3)
+    n = len(X)
+    d = lambda i, j: float(np.sqrt(((X[j] - X[i]) ** 2).sum()))
+    rt = lambda v: float("%.17g" % v)
+    ib = np.arange(1, n, dtype=np.int32)
+    jb = ib + 1
+    r0 = np.array([rt(_dist(X, i - 1, i)) for i in ib])
+    ia = np.arange(1, n - 1, dtype=np.int32)
+    ja = ia + 1
+    ka = ia + 2
+    I, J = np.triu_indices(n, 3)
+    diff = X[J] - X[I]
+    # same arithmetic as the generator: sqrt(dx*dx + dy*dy + dz*dz) in order
+    dd = np.sqrt(diff[:, 0] * diff[:, 0] + diff[:, 1] * diff[:, 1] + diff[:, 2] * diff[:, 2])
+    nat = dd < rcut_nat
+    att_i = (I[nat] + 1).astype(np.int32)
+    att_j = (J[nat] + 1).astype(np.int32)
+    att_r = dd[nat].astype(np.float64)
+    return dict(n=n, ib=ib, jb=jb, r0=r0, ia=ia, ja=ja, ka=ka, att_i=att_i, att_j=att_j, att_r=att_r)
+
+
+def _dist(X, i, j):
+    dx, dy, dz = X[j] - X[i]
+    return float(np.sqrt(dx * dx + dy * dy + dz * dz))
+
+
+def underdamped_coeffs(h=0.05, zeta=50.0):
+    """a1..a5 as set_params leaves them for `set dynamics underdamped; set zeta Z; set t_step H`
+    (params.cpp:19-32, 84-98)."""
+    halfh = h / 2.0
+    a1 = h * (1.0 - zeta * halfh)
+    a2 = h * halfh
+    a3 = (1.0 - h * zeta / 2.0 + (h * zeta) * (h * zeta) / 4.0) / h
+    a4 = halfh * (1.0 - h * zeta / 2.0)
+    a5 = h / zeta
+    return a1, a2, a3, a4, a5
+
+
+def chain_params(T=0.6, zeta=50.0, h=0.05, boxl=500.0, ncell=55.0, lcell=None, cutoff=CUTOFF_NEIGHBOR,
+                 nnlup=50, nup=1000, noise_mode=NOISE_PHILOX, seed=1, coeffs=None):
+    a = coeffs if coeffs is not None else underdamped_coeffs(h, zeta)
+    p = ChainParams()
+    p.T, p.zeta, p.h = T, zeta, h
+    p.a1, p.a2, p.a3, p.a4, p.a5 = a
+    p.boxl, p.ncell = boxl, ncell
+    p.lcell = boxl / ncell if lcell is None else lcell
+    p.cutoff, p.nnlup, p.nup, p.noise_mode, p.seed = cutoff, nnlup, nup, noise_mode, seed
+    return p
+
+
+class Context:
+    """One device context (a ragged batch of chains) on one GPU."""
+
+    def __init__(self, device=0, lanes=0, capacity=0):
+        self.L = lib()
+        h = C.c_void_p()
+        rc = self.L.sopg_create(device, C.byref(h))
+        if rc:
+            raise SopgError(f"sopg_create failed ({rc}): no usable GPU")
+        self.h = h
+        self._keep = []
+        if lanes:
+            self._chk(self.L.sopg_set_lanes(self.h, lanes))
+        if capacity:
+            self._chk(self.L.sopg_set_capacity(self.h, capacity))
+
+    def _chk(self, rc):
+        if rc != OK:
+            raise SopgError(f"sopg error {rc}: {self.L.sopg_last_error(self.h).decode()}")
+        return rc
+
+    def close(self):
+        if self.h:
+            self.L.sopg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_chain(self, topo, params, implicit=True):
+        """topo: dict from chain_topology() (+ optional 'rep_i','rep_j' explicit list)."""
+        t = Topology()
+        keep = []
+
+        def arr(a, dt):
+            a = np.ascontiguousarray(a, dtype=dt)
+            keep.append(a)
+            return a
+        n = int(topo["n"])
+        t.nbead = n
+        ib, jb, r0 = arr(topo["ib"], np.int32), arr(topo["jb"], np.int32), arr(topo["r0"], np.float64)
+        t.nbnd, t.ibnd, t.jbnd, t.r0bnd = len(ib), _ip(ib), _ip(jb), _dp(r0)
+        ia, ja, ka = arr(topo["ia"], np.int32), arr(topo["ja"], np.int32), arr(topo["ka"], np.int32)
+        t.nang, t.iang, t.jang, t.kang = len(ia), _ip(ia), _ip(ja), _ip(ka)
+        ai, aj, ar = arr(topo["att_i"], np.int32), arr(topo["att_j"], np.int32), arr(topo["att_r"], np.float64)
+        at = arr(topo.get("att_it", np.ones(len(ai))), np.int32)
+        ajt = arr(topo.get("att_jt", np.ones(len(ai))), np.int32)
+        t.ncon_att, t.att_i, t.att_j, t.att_it, t.att_jt, t.att_r = len(ai), _ip(ai), _ip(aj), _ip(at), _ip(ajt), _dp(ar)
+        if "rep_i" in topo and not implicit:
+            ri, rj = arr(topo["rep_i"], np.int32), arr(topo["rep_j"], np.int32)
+            rit = arr(topo.get("rep_it", np.ones(len(ri))), np.int32)
+            rjt = arr(topo.get("rep_jt", np.ones(len(ri))), np.int32)
+            t.ncon_rep, t.rep_i, t.rep_j, t.rep_it, t.rep_jt = len(ri), _ip(ri), _ip(rj), _ip(rit), _ip(rjt)
+        else:
+            t.ncon_rep = -1
+        if "bead_type" in topo:
+            bt = arr(topo["bead_type"], np.int32)
+            t.bead_type = _ip(bt)
+        cid = C.c_int()
+        self._chk(self.L.sopg_add_chain(self.h, C.byref(t), C.byref(params), C.byref(cid)))
+        return cid.value
+
+    def put_state(self, chain, pos=None, unc=None, vel=None, force=None):
+        a = [None if x is None else np.ascontiguousarray(x, dtype=np.float64) for x in (pos, unc, vel, force)]
+        self._chk(self.L.sopg_put_state(self.h, chain, *[_ptr(x) for x in a]))
+
+    def get_state(self, chain):
+        n = self.L.sopg_chain_nbead(self.h, chain)
+        out = [np.zeros(3 * n) for _ in range(4)]
+        self._chk(self.L.sopg_get_state(self.h, chain, *[_ptr(x) for x in out]))
+        return dict(pos=out[0], unc=out[1], vel=out[2], force=out[3])
+
+    def update_neighbor_list(self, chain=0):
+        a, r = C.c_int(), C.c_int()
+        self._chk(self.L.sopg_update_neighbor_list(self.h, chain, C.byref(a), C.byref(r)))
+        return a.value, r.value
+
+    def update_cell_list(self, chain=0):
+        a, r = C.c_int(), C.c_int()
+        self._chk(self.L.sopg_update_cell_list(self.h, chain, C.byref(a), C.byref(r)))
+        return a.value, r.value
+
+    def update_pair_list(self, chain=0):
+        a, r = C.c_int(), C.c_int()
+        self._chk(self.L.sopg_update_pair_list(self.h, chain, C.byref(a), C.byref(r)))
+        return a.value, r.value
+
+    def force_eval(self, chain=0, noise=None):
+        nz = None if noise is None else np.ascontiguousarray(noise, dtype=np.float64)
+        self._chk(self.L.sopg_force_eval(self.h, chain, _ptr(nz)))
+
+    def underdamped_iteration(self, chain=0, noise=None):
+        nz = None if noise is None else np.ascontiguousarray(noise, dtype=np.float64)
+        self._chk(self.L.sopg_underdamped_iteration(self.h, chain, _ptr(nz)))
+
+    def energy_eval(self, chain=0):
+        e = Energies()
+        self._chk(self.L.sopg_energy_eval(self.h, chain, C.byref(e)))
+        return {f: getattr(e, f) for f, _ in Energies._fields_}
+
+    def observables(self, chain=0):
+        o = Observables()
+        self._chk(self.L.sopg_calculate_observables(self.h, chain, C.byref(o)))
+        return dict(Q=o.Q, contct_nat=o.contct_nat, rgsq=o.rgsq, kinT=o.kinT)
+
+    def get_list(self, which, chain=0):
+        n = C.c_long()
+        self._chk(self.L.sopg_get_list(self.h, chain, which, None, None, 0, C.byref(n)))
+        i = np.zeros(max(n.value, 1), np.int32)
+        j = np.zeros(max(n.value, 1), np.int32)
+        self._chk(self.L.sopg_get_list(self.h, chain, which, _ip(i), _ip(j), n.value, C.byref(n)))
+        return i[:n.value], j[:n.value]
+
+    def run(self, nsteps, noise=None, rec_cap=100000):
+        recs = (Record * max(rec_cap, 1))()
+        nr = C.c_long()
+        nz = None if noise is None else np.ascontiguousarray(noise, dtype=np.float64)
+        self._chk(self.L.sopg_run(self.h, nsteps, _ptr(nz), recs, rec_cap, C.byref(nr)))
+        out = []
+        for k in range(min(nr.value, rec_cap)):
+            r = recs[k]
+            out.append({f: getattr(r, f) for f, _ in Record._fields_})
+        return out
+
+    def set_step(self, chain, istep, inlup, iup):
+        self._chk(self.L.sopg_set_step(self.h, chain, istep, inlup, iup))
+
+    def sync(self):
+        self._chk(self.L.sopg_sync(self.h))

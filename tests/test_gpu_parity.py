@@ -1,0 +1,318 @@
+"""GPU parity of the gfx950 hot path against the oracle, through the C ABI.
+
+Bars (from the project's parity contract):
+  * neighbor / cell / pair lists: identical, entry for entry, in the reference's order;
+  * per-bead forces: bit-identical to the oracle running the correctly rounded
+    power (the device's one arithmetic difference from glibc pow), and within
+    1e-12 relative of the oracle with glibc pow (= the reference itself);
+  * energies within 1e-10 relative; observables Q exact, Rg^2 / kinT 1e-12;
+  * trajectories (replayed reference noise) within 1e-9 A after 1000 steps.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import sopg
+from conftest import chain_inputs
+from refs import Oracle, setup_system, std_cmds
+
+pytestmark = pytest.mark.gpu
+
+FORCE_REL = 1e-12
+ENERGY_REL = 1e-10
+TRAJ_ABS = 1e-9
+
+
+def per_bead_rel(a, b):
+    a = a.reshape(-1, 3)
+    b = b.reshape(-1, 3)
+    num = np.linalg.norm(a - b, axis=1)
+    den = np.maximum(np.linalg.norm(b, axis=1), 1e-300)
+    return float((num / den).max())
+
+
+def rel(a, b):
+    return abs(a - b) / max(abs(b), 1e-300)
+
+
+@pytest.fixture
+def pow_cr():
+    Oracle.set_pow_mode(1)
+    yield
+    Oracle.set_pow_mode(0)
+
+
+def make_pair(root, n, seed, cmds, params, coords_vel=True, lanes=0, topo=None):
+    x, v, d = chain_inputs(root, n, seed)
+    orc = Oracle()
+    setup_system(orc, d, cmds)
+    ctx = sopg.Context(lanes=lanes)
+    ctx.add_chain(topo or sopg.chain_topology(x), params)
+    if coords_vel:
+        orc.set_coords(2, v)
+        ctx.put_state(0, pos=x, unc=x, vel=v, force=np.zeros_like(x))
+    else:
+        ctx.put_state(0, pos=x, unc=x, vel=np.zeros_like(x), force=np.zeros_like(x))
+    return x, v, d, orc, ctx
+
+
+def check_lists(ctx, orc, which_dev, which_orc, chain=0):
+    di, dj = ctx.get_list(which_dev, chain)
+    oi, oj = orc.lst(which_orc)
+    assert len(di) == len(oi), (which_dev, len(di), len(oi))
+    assert np.array_equal(di, oi) and np.array_equal(dj, oj), which_dev
+
+
+def run_oracle_ctrl(orc, nsteps, nnlup, nup, cutoff="nl"):
+    """Hot loop of underdamped_ctrl (sop.cpp:153-189) on the oracle; returns records."""
+    recs = []
+    inlup, iup = 1, 1
+    for istep in range(1, nsteps + 1):
+        if inlup % nnlup == 0:
+            orc.call("update_neighbor_list" if cutoff == "nl" else "update_cell_list")
+            inlup = 0
+        inlup += 1
+        orc.call("update_pair_list")
+        orc.call("underdamped_iteration")
+        if iup % nup == 0:
+            orc.call("energy_eval")
+            orc.call("calculate_observables")
+            recs.append(dict(istep=istep, e_bnd=orc.getd("e_bnd"), e_ang_ss=orc.getd("e_ang_ss"),
+                             e_vdw_rr=orc.getd("e_vdw_rr"), rna_etot=orc.getd("rna_etot"),
+                             Q=orc.getd("Q"), contct_nat=orc.geti("contct_nat"),
+                             rgsq=orc.getd("rgsq"), kinT=orc.getd("kinT")))
+            iup = 0
+        iup += 1
+    return recs
+
+
+def compare_records(drecs, orecs):
+    assert len(drecs) == len(orecs)
+    for a, b in zip(drecs, orecs):
+        assert a["istep"] == b["istep"]
+        assert a["contct_nat"] == b["contct_nat"]
+        for k in ("e_bnd", "e_ang_ss", "e_vdw_rr", "rna_etot"):
+            assert rel(a[k], b[k]) <= ENERGY_REL, (k, a[k], b[k])
+        for k in ("rgsq", "kinT", "Q"):
+            assert rel(a[k], b[k]) <= 1e-12, (k, a[k], b[k])
+
+
+@pytest.mark.parametrize("lanes", [1, 8])
+def test_lists_forces_energy_nl76(inputs_root, pow_cr, lanes):
+    n = 76
+    params = sopg.chain_params(cutoff=sopg.CUTOFF_NEIGHBOR, nnlup=50, nup=100, noise_mode=sopg.NOISE_REPLAY)
+    x, v, d, orc, ctx = make_pair(inputs_root, n, 7, std_cmds(nup=100), params, lanes=lanes)
+    print(f"[gpu] N={n} lanes={lanes}: lists", flush=True)
+    na, nr = ctx.update_neighbor_list()
+    orc.call("update_neighbor_list")
+    assert (na, nr) == (orc.geti("nnl_att"), orc.geti("nnl_rep"))
+    check_lists(ctx, orc, sopg.LIST_NL_ATT, 0)
+    check_lists(ctx, orc, sopg.LIST_NL_REP, 1)
+    pa, pr = ctx.update_pair_list()
+    orc.call("update_pair_list")
+    assert (pa, pr) == (orc.geti("nil_att"), orc.geti("nil_rep"))
+    check_lists(ctx, orc, sopg.LIST_PL_ATT, 2)
+    check_lists(ctx, orc, sopg.LIST_PL_REP, 3)
+    e = ctx.energy_eval()
+    orc.call("energy_eval")
+    for k in ("e_bnd", "e_ang_ss", "e_vdw_rr", "rna_etot"):
+        assert rel(e[k], orc.getd(k)) <= ENERGY_REL, k
+    noise = Oracle().gasdev(3 * n)
+    orc.set_noise(noise)
+    orc.call("force_eval")
+    ctx.force_eval(0, noise)
+    fd = ctx.get_state(0)["force"]
+    fo = orc.coords(3)
+    assert np.array_equal(fd, fo), per_bead_rel(fd, fo)
+    ob = ctx.observables()
+    orc.call("calculate_observables")
+    assert ob["contct_nat"] == orc.geti("contct_nat")
+    assert rel(ob["rgsq"], orc.getd("rgsq")) <= 1e-12
+    assert rel(ob["kinT"], orc.getd("kinT")) <= 1e-12
+
+
+def test_force_vs_glibc_pow_reference(inputs_root):
+    """Against the oracle with glibc pow (bit-identical to the reference): 1e-12 relative."""
+    n = 300
+    params = sopg.chain_params(cutoff=sopg.CUTOFF_NEIGHBOR, noise_mode=sopg.NOISE_REPLAY)
+    x, v, d, orc, ctx = make_pair(inputs_root, n, 11, std_cmds(), params)
+    ctx.update_neighbor_list(); ctx.update_pair_list()
+    orc.call("update_neighbor_list"); orc.call("update_pair_list")
+    noise = Oracle().gasdev(3 * n)
+    orc.set_noise(noise)
+    orc.call("force_eval")
+    ctx.force_eval(0, noise)
+    assert per_bead_rel(ctx.get_state(0)["force"], orc.coords(3)) <= FORCE_REL
+
+
+def test_trajectory_1000_steps_nl76(inputs_root, pow_cr):
+    n, steps, nnlup, nup = 76, 1000, 50, 100
+    params = sopg.chain_params(cutoff=sopg.CUTOFF_NEIGHBOR, nnlup=nnlup, nup=nup, noise_mode=sopg.NOISE_REPLAY)
+    x, v, d, orc, ctx = make_pair(inputs_root, n, 7, std_cmds(nnlup=nnlup, nup=nup), params)
+    print(f"[gpu] trajectory N={n} steps={steps}", flush=True)
+    noise = Oracle().gasdev(3 * n * (steps + 1))
+    # underdamped_ctrl preamble (sop.cpp:119-147): lists, energy, force
+    orc.call("update_neighbor_list"); orc.call("update_pair_list")
+    ctx.update_neighbor_list(); ctx.update_pair_list()
+    orc.set_noise(noise)
+    orc.call("energy_eval"); orc.call("force_eval")
+    ctx.force_eval(0, noise[:3 * n])
+    orecs = run_oracle_ctrl(orc, steps, nnlup, nup)
+    drecs = ctx.run(steps, noise[3 * n:])
+    s = ctx.get_state(0)
+    for w, k in ((1, "unc"), (0, "pos"), (2, "vel")):
+        diff = np.abs(s[k] - orc.coords(w)).max()
+        assert diff <= TRAJ_ABS, (k, diff)
+    assert np.array_equal(s["unc"], orc.coords(1)), "expected bit-identical trajectory"
+    assert per_bead_rel(s["force"], orc.coords(3)) <= FORCE_REL
+    compare_records(drecs, orecs)
+    check_lists(ctx, orc, sopg.LIST_PL_REP, 3)
+
+
+def test_cell_list_n300(inputs_root, pow_cr):
+    n, steps, nnlup, nup = 300, 200, 20, 50
+    extra = [("boxl", "504"), ("ncell", "42")]
+    cmds = std_cmds(cutoff="celllist", nnlup=nnlup, nup=nup, extra=extra)
+    params = sopg.chain_params(boxl=504.0, ncell=42.0, cutoff=sopg.CUTOFF_CELL, nnlup=nnlup, nup=nup,
+                               noise_mode=sopg.NOISE_REPLAY)
+    x, v, d, orc, ctx = make_pair(inputs_root, n, 5, cmds, params)
+    print(f"[gpu] cell list N={n}", flush=True)
+    assert orc.getd("lcell") == params.lcell
+    na, nr = ctx.update_cell_list()
+    orc.call("update_cell_list")
+    assert (na, nr) == (orc.geti("nnl_att"), orc.geti("nnl_rep"))
+    check_lists(ctx, orc, sopg.LIST_NL_ATT, 0)
+    check_lists(ctx, orc, sopg.LIST_NL_REP, 1)
+    ctx.update_pair_list(); orc.call("update_pair_list")
+    check_lists(ctx, orc, sopg.LIST_PL_REP, 3)
+    noise = Oracle().gasdev(3 * n * (steps + 1))
+    orc.set_noise(noise)
+    orc.call("energy_eval"); orc.call("force_eval")
+    ctx.force_eval(0, noise[:3 * n])
+    orecs = run_oracle_ctrl(orc, steps, nnlup, nup, cutoff="cl")
+    drecs = ctx.run(steps, noise[3 * n:])
+    s = ctx.get_state(0)
+    assert np.abs(s["unc"] - orc.coords(1)).max() <= TRAJ_ABS
+    compare_records(drecs, orecs)
+    check_lists(ctx, orc, sopg.LIST_NL_REP, 1)
+
+
+def _write_vdw(path, entries, natt, nrep):
+    with open(path, "w") as f:
+        f.write(f"natt {natt} nrep {nrep}\n")
+        for (i, j, r) in entries:
+            f.write(f"{i} {j} {r!r} 1 1\n")
+
+
+def test_explicit_master_lists(inputs_root, pow_cr, tmp_path):
+    """A vdw file in arbitrary order with some reversed pairs: explicit lists."""
+    n, steps, nnlup, nup = 90, 150, 10, 50
+    x, v, d = chain_inputs(inputs_root, n, 3)
+    X = x.reshape(-1, 3)
+    rng = np.random.default_rng(0)
+    ent = []
+    for i in range(n):
+        for j in range(i + 3, n):
+            dx, dy, dz = X[j] - X[i]
+            r = float(np.sqrt(dx * dx + dy * dy + dz * dz))
+            ent.append((i + 1, j + 1, r) if rng.random() < 0.8 else (j + 1, i + 1, r))
+    rng.shuffle(ent)
+    natt = sum(1 for e in ent if e[2] < 8.0)
+    dd = tmp_path / "explicit"
+    dd.mkdir()
+    for f in ("bonds.dat", "angles.dat", "init.dat"):
+        (dd / f).write_bytes(open(os.path.join(d, f), "rb").read())
+    _write_vdw(dd / "vdw.dat", ent, natt, len(ent) - natt)
+    orc = Oracle()
+    setup_system(orc, str(dd), std_cmds(nnlup=nnlup, nup=nup))
+    orc.set_coords(2, v)
+    ai, aj = orc.lst(4)
+    ri, rj = orc.lst(5)
+    att = [e for e in ent if e[2] < 8.0]
+    topo = sopg.chain_topology(x)
+    topo.update(att_i=ai, att_j=aj, att_r=np.array([e[2] for e in att]), rep_i=ri, rep_j=rj)
+    params = sopg.chain_params(cutoff=sopg.CUTOFF_NEIGHBOR, nnlup=nnlup, nup=nup, noise_mode=sopg.NOISE_REPLAY)
+    ctx = sopg.Context()
+    ctx.add_chain(topo, params, implicit=False)
+    ctx.put_state(0, pos=x, unc=x, vel=v, force=np.zeros_like(x))
+    print("[gpu] explicit master lists", flush=True)
+    ctx.update_neighbor_list(); orc.call("update_neighbor_list")
+    for w in (0, 1):
+        check_lists(ctx, orc, w, w)
+    ctx.update_pair_list(); orc.call("update_pair_list")
+    for w in (2, 3):
+        check_lists(ctx, orc, w, w)
+    noise = Oracle().gasdev(3 * n * (steps + 1))
+    orc.set_noise(noise)
+    orc.call("energy_eval"); orc.call("force_eval")
+    ctx.force_eval(0, noise[:3 * n])
+    assert np.array_equal(ctx.get_state(0)["force"], orc.coords(3))
+    orecs = run_oracle_ctrl(orc, steps, nnlup, nup)
+    drecs = ctx.run(steps, noise[3 * n:])
+    assert np.abs(ctx.get_state(0)["unc"] - orc.coords(1)).max() <= TRAJ_ABS
+    compare_records(drecs, orecs)
+
+
+def _philox_ctx(chains, lanes=0):
+    ctx = sopg.Context(lanes=lanes)
+    for (x, v, seed) in chains:
+        ctx.add_chain(sopg.chain_topology(x), sopg.chain_params(cutoff=sopg.CUTOFF_NEIGHBOR, nnlup=20, nup=100,
+                                                                 noise_mode=sopg.NOISE_PHILOX, seed=seed))
+    for c, (x, v, seed) in enumerate(chains):
+        ctx.put_state(c, pos=x, unc=x, vel=v, force=np.zeros_like(x))
+        ctx.update_neighbor_list(c)
+        ctx.update_pair_list(c)
+        ctx.force_eval(c)
+    return ctx
+
+
+def test_batch_equals_alone(inputs_root):
+    """Each chain of a ragged batch evolves exactly as it does alone."""
+    chains = []
+    for n, seed in ((60, 21), (131, 22), (205, 23)):
+        x, v, _ = chain_inputs(inputs_root, n, seed)
+        chains.append((x, v, 1000 + seed))
+    print("[gpu] batch vs alone", flush=True)
+    mix = _philox_ctx(chains)
+    rm = mix.run(300)
+    for c, ch in enumerate(chains):
+        alone = _philox_ctx([ch])
+        ra = alone.run(300)
+        a, b = mix.get_state(c), alone.get_state(0)
+        for k in ("unc", "vel", "force", "pos"):
+            assert np.array_equal(a[k], b[k]), (c, k)
+        assert [r for r in rm if r["chain"] == c] == [dict(r, chain=c) for r in ra]
+
+
+@pytest.mark.parametrize("lanes", [1, 2, 16, 64])
+def test_lanes_invariance(inputs_root, lanes):
+    x, v, _ = chain_inputs(inputs_root, 150, 31)
+    base = _philox_ctx([(x, v, 5)], lanes=8)
+    base.run(120)
+    other = _philox_ctx([(x, v, 5)], lanes=lanes)
+    other.run(120)
+    a, b = base.get_state(0), other.get_state(0)
+    for k in ("unc", "vel", "force"):
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_philox_noise_statistics(inputs_root):
+    """Device noise: zero force field replaced by pure noise statistics check."""
+    n = 2000
+    x = sopg.gen_saw(n, 3)
+    topo = sopg.chain_topology(x)
+    ctx = sopg.Context()
+    p = sopg.chain_params(cutoff=sopg.CUTOFF_NEIGHBOR, noise_mode=sopg.NOISE_PHILOX, seed=99)
+    ctx.add_chain(topo, p)
+    ctx.put_state(0, pos=x, unc=x, vel=np.zeros_like(x), force=np.zeros_like(x))
+    ctx.update_neighbor_list(); ctx.update_pair_list()
+    # energy-free part: the random force alone = force_eval minus the deterministic terms
+    ctx.force_eval(0)
+    f1 = ctx.get_state(0)["force"]
+    ctx.force_eval(0)
+    f2 = ctx.get_state(0)["force"]
+    var = np.sqrt(2.0 * 0.6 * 50.0 / 0.05)
+    g = (f2 - f1) / var  # difference of two independent deviates: N(0, 2)
+    assert abs(g.mean()) < 0.05
+    assert abs(g.var() - 2.0) < 0.1
