@@ -102,9 +102,10 @@ int sopg_destroy(sopg_ctx* ctx);
 const char* sopg_last_error(const sopg_ctx* ctx);
 int sopg_default_tables(sopg_tables* t);
 int sopg_set_tables(sopg_ctx* ctx, const sopg_tables* t);
-/* Lanes per bead for the list and force kernels (1..64, power of two; 0 =
- * automatic).  Results do not depend on it. */
-int sopg_set_lanes(sopg_ctx* ctx, int lanes);
+/* Lanes (1..64, power of two; 0 = default) that cooperate on one bead in the
+ * step (force) kernels and in the list-rebuild kernels.  Results do not depend
+ * on either. */
+int sopg_set_lanes(sopg_ctx* ctx, int step_lanes, int list_lanes);
 /* Per-bead capacity of the non-native neighbor list (0 = automatic). */
 int sopg_set_capacity(sopg_ctx* ctx, int cap);
 
@@ -153,6 +154,20 @@ int sopg_run(sopg_ctx* ctx, long nsteps, const double* noise, sopg_record* rec, 
 int sopg_set_step(sopg_ctx* ctx, int chain, double istep, int inlup, int iup);
 /* Block until all work queued on the context's stream has finished. */
 int sopg_sync(sopg_ctx* ctx);
+
+/* Current list sizes of a chain (nnl_att, nnl_rep, nil_att, nil_rep: the
+ * reference's counters, global.h:96-101) without changing any state. */
+int sopg_list_counts(sopg_ctx* ctx, int chain, int* nnl_att, int* nnl_rep, int* nil_att, int* nil_rep);
+
+/* ------------------------------------------------------------- measurement */
+#define SOPG_KERNEL_STEP 0     /* fused force / velocity / next-position kernel */
+#define SOPG_KERNEL_REBUILD 1  /* neighbor / cell list rebuild */
+#define SOPG_KERNEL_PAIRLIST 2 /* standalone pair-list filter */
+#define SOPG_KERNEL_OBSERVE 3  /* observables + reductions */
+/* enable != 0: bracket every kernel launched by sopg_run with HIP events on
+ * the context's stream and accumulate their durations (reset on enable). */
+int sopg_profile(sopg_ctx* ctx, int enable);
+int sopg_profile_read(sopg_ctx* ctx, int kernel, double* total_ms, long* launches);
 
 #ifdef __cplusplus
 }

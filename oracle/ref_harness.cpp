@@ -24,6 +24,17 @@
 #include "io.h"
 
 static coord* g_incr = 0;
+static int g_incr_n = -1;
+
+// The step scratch array the reference's ctrl loop owns (sop.cpp:104).
+static coord* incr_array() {
+  if (g_incr_n != nbead) {
+    delete[] g_incr;
+    g_incr = new coord[nbead + 1];
+    g_incr_n = nbead;
+  }
+  return g_incr;
+}
 
 static void fill_command(const char* verb, const char* key, const char* val) {
   // One command at slot 1: cmd[1] = verb, options opt[1], opt[2].
@@ -67,17 +78,19 @@ void ref_angular_forces() { soft_sphere_angular_forces(); }
 void ref_vdw_forces() { vdw_forces(); }
 void ref_energy_eval() { energy_eval(); }
 
-void ref_underdamped_iteration() {
-  if (!g_incr) g_incr = new coord[nbead + 1];
-  underdamped_iteration(g_incr);
-}
-void ref_calculate_observables() {
-  if (!g_incr) g_incr = new coord[nbead + 1];
-  calculate_observables(g_incr);
-}
+void ref_underdamped_iteration() { underdamped_iteration(incr_array()); }
+void ref_calculate_observables() { calculate_observables(incr_array()); }
 
 // Reference Ran_Gen (random_generator.cpp:88-146) on the global generator.
 void ref_set_seed(int s) { generator.set_seed(s); }
+// Full generator state from a state file (random_generator.cpp:51-70).
+void ref_rng_restart(const char* path) {
+  char buf[2048];
+  std::strncpy(buf, path, sizeof buf - 1);
+  buf[sizeof buf - 1] = 0;
+  generator.set_fname(buf);
+  generator.restart();
+}
 double ref_ran2() { return generator.ran2(); }
 double ref_gasdev() { return generator.gasdev(); }
 void ref_gasdev_n(double* out, long n) {

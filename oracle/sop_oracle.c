@@ -892,4 +892,5 @@ int orc_get_list(orc_sys* s, int which, int* oi, int* oj) {
 
 void orc_set_noise(orc_sys* s, const double* noise) { s->noise = noise; s->noise_pos = 0; }
 void orc_gasdev_n(orc_sys* s, double* out, long n) { long i; for (i = 0; i < n; i++) out[i] = orc_gasdev(&s->rng); }
+double orc_sys_ran2(orc_sys* s) { return orc_ran2(&s->rng); }
 void orc_set_seed(orc_sys* s, int seed) { s->rng.mseed = seed; }

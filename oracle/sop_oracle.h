@@ -119,6 +119,7 @@ void orc_set_coords(orc_sys* s, int which, const double* in);
 int orc_get_list(orc_sys* s, int which, int* out_i, int* out_j);
 void orc_set_noise(orc_sys* s, const double* noise);
 void orc_gasdev_n(orc_sys* s, double* out, long n);
+double orc_sys_ran2(orc_sys* s);
 void orc_set_seed(orc_sys* s, int seed);
 
 #ifdef __cplusplus

@@ -18,10 +18,10 @@
 
 namespace sopg {
 void launch_cellcoords(const Dev& D, int cur, int g0, int g1, hipStream_t s);
-void launch_rebuild(int L, const Dev& D, const Tables& T, int cur, int par, int rel, int mode, int g0, int g1,
+void launch_rebuild(int L, const Dev& D, const Tables* T, int cur, int par, int rel, int mode, int g0, int g1,
                     hipStream_t s);
-void launch_pairfilter(int L, const Dev& D, const Tables& T, int cur, int par, int g0, int g1, hipStream_t s);
-void launch_force(int L, const Dev& D, const Tables& T, int cur, int pr, int rel, int flags, bool next,
+void launch_pairfilter(int L, const Dev& D, const Tables* T, int cur, int par, int g0, int g1, hipStream_t s);
+void launch_force(int L, const Dev& D, const Tables* T, int cur, int pr, int rel, int flags, bool next,
                   const double* noise, int g0, int g1, hipStream_t s);
 void launch_posupdate(const Dev& D, int src, int dst, int g0, int g1, hipStream_t s);
 void launch_velupdate(const Dev& D, int g0, int g1, hipStream_t s);
@@ -59,8 +59,10 @@ struct sopg_ctx {
   std::string err;
   sopg_tables tab{};
   Tables T{};
-  int lanes = 8;
+  int lanes = 16;       // lanes per bead, step / force kernels
+  int list_lanes = 64;  // lanes per bead, list rebuild
   int cap_user = 0;
+  Tables* d_T = nullptr;
   std::vector<HostChain> chains;
   bool committed = false;
   Dev D{};
@@ -74,6 +76,13 @@ struct sopg_ctx {
   size_t noise_cap = 0;         // doubles
   double* d_rec = nullptr;      // per-record-step reductions
   size_t rec_cap_d = 0;
+  // kernel timing (sopg_profile)
+  bool prof = false;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<int, std::pair<int, int>>> ev_pending;  // kernel, (start, stop) pool indices
+  size_t ev_next = 0;
+  double prof_ms[4] = {0, 0, 0, 0};
+  long prof_n[4] = {0, 0, 0, 0};
 };
 
 namespace {
@@ -89,6 +98,37 @@ int fail(sopg_ctx* c, int code, const std::string& m) {
     if (e_ != hipSuccess)                                                              \
       return fail(ctx, SOPG_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
+
+// Event bracket around one launch when profiling; returns the pool index pair.
+int prof_begin(sopg_ctx* c) {
+  if (!c->prof) return -1;
+  while (c->ev_pool.size() < c->ev_next + 2) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return -1;
+    c->ev_pool.push_back(e);
+  }
+  const int i = (int)c->ev_next;
+  c->ev_next += 2;
+  (void)hipEventRecord(c->ev_pool[i], c->stream);
+  return i;
+}
+void prof_end(sopg_ctx* c, int kernel, int i) {
+  if (i < 0) return;
+  (void)hipEventRecord(c->ev_pool[i + 1], c->stream);
+  c->ev_pending.push_back({kernel, {i, i + 1}});
+}
+// After a stream synchronisation: fold the recorded brackets into the totals.
+void prof_collect(sopg_ctx* c) {
+  for (auto& p : c->ev_pending) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, c->ev_pool[p.second.first], c->ev_pool[p.second.second]) == hipSuccess) {
+      c->prof_ms[p.first] += ms;
+      c->prof_n[p.first]++;
+    }
+  }
+  c->ev_pending.clear();
+  c->ev_next = 0;
+}
 
 template <class T>
 int dalloc(sopg_ctx* c, T** p, size_t n) {
@@ -328,6 +368,7 @@ int commit(sopg_ctx* c) {
   A(D.red, (size_t)nb * kRed);
   A(D.err, 1);
   A(D.cp, c->chains.size());
+  A(c->d_T, 1);
   A(c->d_out, c->chains.size() * kRed);
 #undef A
   hipStream_t s = c->stream;
@@ -363,6 +404,7 @@ int commit(sopg_ctx* c) {
        (rc = upload(c, D.rm_t, rt))))
     return rc;
   if ((rc = upload_chainp(c))) return rc;
+  HIP_OK(c, hipMemcpyAsync(c->d_T, &c->T, sizeof(Tables), hipMemcpyHostToDevice, s));
   HIP_OK(c, hipStreamSynchronize(s));
   c->committed = true;
   // No-cutoff chains interact through their full master lists, as the loader
@@ -370,7 +412,7 @@ int commit(sopg_ctx* c) {
   if (any_none) {
     for (size_t ic = 0; ic < c->chains.size(); ic++)
       if (c->chains[ic].p.cutoff == SOPG_CUTOFF_NONE)
-        launch_rebuild(c->lanes, D, c->T, c->cur, c->par, 0, SOPG_CUTOFF_NONE, c->cp[ic].off,
+        launch_rebuild(c->list_lanes, D, c->d_T, c->cur, c->par, 0, SOPG_CUTOFF_NONE, c->cp[ic].off,
                        c->cp[ic].off + c->cp[ic].n, s);
     HIP_OK(c, hipGetLastError());
   }
@@ -423,7 +465,7 @@ int rebuild_one(sopg_ctx* c, int chain, int mode, int* nnl_att, int* nnl_rep) {
   if (mode == SOPG_CUTOFF_CELL && (!(h->p.ncell > 0.0) || !(h->p.lcell > 0.0)))
     return fail(c, SOPG_ERR_ARG, "cell list needs ncell > 0 and lcell > 0");
   if (mode == SOPG_CUTOFF_CELL) launch_cellcoords(c->D, c->cur, P->off, P->off + P->n, c->stream);
-  launch_rebuild(c->lanes, c->D, c->T, c->cur, c->par, 0, mode, P->off, P->off + P->n, c->stream);
+  launch_rebuild(c->list_lanes, c->D, c->d_T, c->cur, c->par, 0, mode, P->off, P->off + P->n, c->stream);
   HIP_OK(c, hipGetLastError());
   if ((rc = check_err_flag(c))) return rc;
   c->pl_fresh = true;
@@ -478,6 +520,7 @@ int sopg_destroy(sopg_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (void* p : c->allocs) (void)hipFree(p);
+  for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return SOPG_OK;
@@ -515,14 +558,21 @@ int sopg_set_tables(sopg_ctx* c, const sopg_tables* t) {
   if (!c || !t) return SOPG_ERR_ARG;
   c->tab = *t;
   derive_tables(c->tab, c->T);
+  if (c->committed) {
+    HIP_OK(c, hipMemcpyAsync(c->d_T, &c->T, sizeof(Tables), hipMemcpyHostToDevice, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+  }
   return SOPG_OK;
 }
 
-int sopg_set_lanes(sopg_ctx* c, int lanes) {
+int sopg_set_lanes(sopg_ctx* c, int step_lanes, int list_lanes) {
   if (!c) return SOPG_ERR_ARG;
-  if (lanes == 0) lanes = 8;
-  if (lanes < 1 || lanes > 64 || (lanes & (lanes - 1))) return fail(c, SOPG_ERR_ARG, "lanes: power of two in 1..64");
-  c->lanes = lanes;
+  auto ok = [](int l) { return l >= 1 && l <= 64 && !(l & (l - 1)); };
+  if (step_lanes == 0) step_lanes = 16;
+  if (list_lanes == 0) list_lanes = 64;
+  if (!ok(step_lanes) || !ok(list_lanes)) return fail(c, SOPG_ERR_ARG, "lanes: power of two in 1..64");
+  c->lanes = step_lanes;
+  c->list_lanes = list_lanes;
   return SOPG_OK;
 }
 
@@ -693,7 +743,7 @@ int sopg_update_pair_list(sopg_ctx* c, int chain, int* nil_att, int* nil_rep) {
   ChainP* P;
   int rc = get_chain(c, chain, &h, &P);
   if (rc) return rc;
-  launch_pairfilter(c->lanes, c->D, c->T, c->cur, c->par, P->off, P->off + P->n, c->stream);
+  launch_pairfilter(c->list_lanes, c->D, c->d_T, c->cur, c->par, P->off, P->off + P->n, c->stream);
   HIP_OK(c, hipGetLastError());
   c->pl_fresh = true;
   double out[kRed];
@@ -711,7 +761,7 @@ int sopg_force_eval(sopg_ctx* c, int chain, const double* noise) {
   if ((rc = upload_chainp(c))) return rc;
   const double* dn = chain_noise(c, *P, *h, noise, &rc);
   if (rc) return rc;
-  launch_force(c->lanes, c->D, c->T, c->cur, c->par, 0, F_STORE, false, dn, P->off, P->off + P->n, c->stream);
+  launch_force(c->lanes, c->D, c->d_T, c->cur, c->par, 0, F_STORE, false, dn, P->off, P->off + P->n, c->stream);
   HIP_OK(c, hipGetLastError());
   HIP_OK(c, hipStreamSynchronize(c->stream));
   h->noise_base++;
@@ -724,7 +774,7 @@ int sopg_energy_eval(sopg_ctx* c, int chain, sopg_energies* out) {
   int rc = get_chain(c, chain, &h, &P);
   if (rc) return rc;
   if ((rc = upload_chainp(c))) return rc;
-  launch_force(c->lanes, c->D, c->T, c->cur, c->par, 0, F_ENERGY_ALL, false, nullptr, P->off, P->off + P->n,
+  launch_force(c->lanes, c->D, c->d_T, c->cur, c->par, 0, F_ENERGY_ALL, false, nullptr, P->off, P->off + P->n,
                c->stream);
   HIP_OK(c, hipGetLastError());
   double r[kRed];
@@ -749,7 +799,7 @@ int sopg_underdamped_iteration(sopg_ctx* c, int chain, const double* noise) {
   const double* dn = chain_noise(c, *P, *h, noise, &rc);
   if (rc) return rc;
   launch_posupdate(c->D, c->cur, c->cur, P->off, P->off + P->n, c->stream);
-  launch_force(c->lanes, c->D, c->T, c->cur, c->par, 0, F_STORE | F_VEL, false, dn, P->off, P->off + P->n,
+  launch_force(c->lanes, c->D, c->d_T, c->cur, c->par, 0, F_STORE | F_VEL, false, dn, P->off, P->off + P->n,
                c->stream);
   HIP_OK(c, hipGetLastError());
   HIP_OK(c, hipStreamSynchronize(c->stream));
@@ -847,6 +897,36 @@ int sopg_sync(sopg_ctx* c) {
   return SOPG_OK;
 }
 
+int sopg_list_counts(sopg_ctx* c, int chain, int* nnl_att, int* nnl_rep, int* nil_att, int* nil_rep) {
+  HostChain* h;
+  ChainP* P;
+  int rc = get_chain(c, chain, &h, &P);
+  if (rc) return rc;
+  double out[kRed];
+  if ((rc = list_counts(c, chain, out))) return rc;
+  if (nnl_att) *nnl_att = (int)out[R_NLA];
+  if (nnl_rep) *nnl_rep = (int)out[R_NLR];
+  if (nil_att) *nil_att = (int)out[R_PLA];
+  if (nil_rep) *nil_rep = (int)out[R_PLR];
+  return SOPG_OK;
+}
+
+int sopg_profile(sopg_ctx* c, int enable) {
+  if (!c) return SOPG_ERR_ARG;
+  c->prof = enable != 0;
+  c->ev_pending.clear();
+  c->ev_next = 0;
+  for (int k = 0; k < 4; k++) { c->prof_ms[k] = 0.0; c->prof_n[k] = 0; }
+  return SOPG_OK;
+}
+
+int sopg_profile_read(sopg_ctx* c, int kernel, double* total_ms, long* launches) {
+  if (!c || kernel < 0 || kernel > 3) return SOPG_ERR_ARG;
+  if (total_ms) *total_ms = c->prof_ms[kernel];
+  if (launches) *launches = c->prof_n[kernel];
+  return SOPG_OK;
+}
+
 int sopg_run(sopg_ctx* c, long nsteps, const double* noise, sopg_record* rec, long rec_cap, long* nrec) {
   if (!c) return SOPG_ERR_ARG;
   if (nrec) *nrec = 0;
@@ -891,28 +971,41 @@ int sopg_run(sopg_ctx* c, long nsteps, const double* noise, sopg_record* rec, lo
   Dev& D = c->D;
   // step 1 of the run: rebuild if due, pair list, position half
   auto dr = due(0);
+  int pe;
   if (dr.first) {
+    pe = prof_begin(c);
     if (dr.second) launch_cellcoords(D, c->cur, 0, nb, s);
-    launch_rebuild(L, D, c->T, c->cur, c->par, 0, -1, 0, nb, s);
+    launch_rebuild(c->list_lanes, D, c->d_T, c->cur, c->par, 0, -1, 0, nb, s);
+    prof_end(c, SOPG_KERNEL_REBUILD, pe);
   }
-  if (!c->pl_fresh) launch_pairfilter(L, D, c->T, c->cur, c->par, 0, nb, s);
+  if (!c->pl_fresh) {
+    pe = prof_begin(c);
+    launch_pairfilter(c->list_lanes, D, c->d_T, c->cur, c->par, 0, nb, s);
+    prof_end(c, SOPG_KERNEL_PAIRLIST, pe);
+  }
   launch_posupdate(D, c->cur, c->cur, 0, nb, s);
   std::vector<long> rec_steps;
   for (long r = 0; r < nsteps; r++) {
     const bool last = (r == nsteps - 1);
     const double* nz = noise ? c->d_noise + (size_t)r * 3 * nb : nullptr;
-    launch_force(L, D, c->T, c->cur, c->par, (int)r, F_STORE | F_VEL | F_ENERGY, !last, nz, 0, nb, s);
+    pe = prof_begin(c);
+    launch_force(L, D, c->d_T, c->cur, c->par, (int)r, F_STORE | F_VEL | F_ENERGY, !last, nz, 0, nb, s);
+    prof_end(c, SOPG_KERNEL_STEP, pe);
     if (records(r)) {
+      pe = prof_begin(c);
       launch_observe(D, c->cur, c->par, (int)r, 0, OBS_PHYS, 0, nb, s);
       launch_chain_reduce(D, 0, nc, c->d_rec + rec_steps.size() * nc * kRed, s);
+      prof_end(c, SOPG_KERNEL_OBSERVE, pe);
       rec_steps.push_back(r);
     }
     if (!last) {
       c->par ^= 1;
       dr = due(r + 1);
       if (dr.first) {
+        pe = prof_begin(c);
         if (dr.second) launch_cellcoords(D, c->cur, 0, nb, s);
-        launch_rebuild(L, D, c->T, c->cur, c->par, (int)(r + 1), -1, 0, nb, s);
+        launch_rebuild(c->list_lanes, D, c->d_T, c->cur, c->par, (int)(r + 1), -1, 0, nb, s);
+        prof_end(c, SOPG_KERNEL_REBUILD, pe);
       }
       c->cur ^= 1;
     }
@@ -923,6 +1016,7 @@ int sopg_run(sopg_ctx* c, long nsteps, const double* noise, sopg_record* rec, lo
   if (!hr.empty())
     HIP_OK(c, hipMemcpyAsync(hr.data(), c->d_rec, sizeof(double) * hr.size(), hipMemcpyDeviceToHost, s));
   HIP_OK(c, hipStreamSynchronize(s));
+  prof_collect(c);
   if ((rc = check_err_flag(c))) return rc;
   long nr = 0;
   for (size_t q = 0; q < rec_steps.size(); q++) {

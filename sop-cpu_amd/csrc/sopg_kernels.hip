@@ -40,11 +40,15 @@ __device__ __forceinline__ void fold3(double& Fx, double& Fy, double& Fz, bool c
   if constexpr (L == 1) {
     if (c) { Fx += fx; Fy += fy; Fz += fz; }
   } else {
-    const uint64_t gm = group_mask<L>(c);
-#pragma unroll
-    for (int k = 0; k < L; k++) {
-      const double ax = __shfl(fx, k, L), ay = __shfl(fy, k, L), az = __shfl(fz, k, L);
-      if ((gm >> k) & 1ull) { Fx += ax; Fy += ay; Fz += az; }
+    uint64_t gm = group_mask<L>(c);
+    // group-uniform loop over the contributing lanes, lowest lane (= earliest
+    // list entry) first; every source lane belongs to this (active) group
+    while (gm) {
+      const int k = __builtin_ctzll(gm);
+      gm &= gm - 1ull;
+      Fx += __shfl(fx, k, L);
+      Fy += __shfl(fy, k, L);
+      Fz += __shfl(fz, k, L);
     }
   }
 }
@@ -62,22 +66,22 @@ __device__ __forceinline__ double group_sum(double v) {
 // energy.cpp:293: the native skip test reads lj_nat_pdb_dist2[i] with i the
 // PAIR-LIST position of the entry, not its master index.  Only reachable when
 // d2 < tol * max(lj_nat_pdb_dist2); then the position is counted exactly.
-__device__ __noinline__ bool att_tol_skip(const Dev& D, const ChainP& P, int k, int pr, double d2) {
-  const uint8_t* pl = pr ? D.am_pl1 : D.am_pl0;
+__device__ __noinline__ bool att_tol_skip(const uint8_t* __restrict__ pl, const double* __restrict__ r2, int att_off,
+                                          int k, double d2) {
   int p = 0;
-  for (int kk = P.att_off; kk < k; kk++) p += pl[kk];
-  return d2 < kTol * D.am_r2[P.att_off + p];
+  for (int kk = att_off; kk < k; kk++) p += pl[kk];
+  return d2 < kTol * r2[att_off + p];
 }
 
 // energy.cpp:292, 331: rep_tol comes from the LAST native pair-list entry.
 // With an empty native pair list the reference reads it uninitialised; here no
 // non-native pair is skipped in that case.
-__device__ __noinline__ bool rep_tol_skip(const Dev& D, const Tables& T, const ChainP& P, int pr, double d2) {
-  const uint8_t* pl = pr ? D.am_pl1 : D.am_pl0;
-  for (int kk = P.att_off + P.natt - 1; kk >= P.att_off; kk--) {
+__device__ __noinline__ bool rep_tol_skip(const uint8_t* __restrict__ pl, const int8_t* __restrict__ am_t,
+                                          const Tables* __restrict__ T, int att_off, int natt, double d2) {
+  for (int kk = att_off + natt - 1; kk >= att_off; kk--) {
     if (pl[kk]) {
-      const int tt = D.am_t[kk];
-      return d2 < T.s2[tt & 3][tt >> 2] * kTol;
+      const int tt = am_t[kk];
+      return d2 < T->s2[tt & 3][tt >> 2] * kTol;
     }
   }
   return false;
@@ -98,16 +102,16 @@ __global__ void k_cellcoords(Dev D, int cur, int g0, int g1) {
   const int g = g0 + blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= g1) return;
   const ChainP& P = D.cp[D.chain[g]];
-  D.cx[g] = floor(D.ux[cur][g] / P.lcell);
-  D.cy[g] = floor(D.uy[cur][g] / P.lcell);
-  D.cz[g] = floor(D.uz[cur][g] / P.lcell);
+  D.cx[g] = floor((cur ? D.ux[1] : D.ux[0])[g] / P.lcell);
+  D.cy[g] = floor((cur ? D.uy[1] : D.uy[0])[g] / P.lcell);
+  D.cz[g] = floor((cur ? D.uz[1] : D.uz[0])[g] / P.lcell);
 }
 
 // Neighbor-list (distance) or cell-list rebuild plus the pair-list filter at
 // the same positions.  mode < 0: each chain's own cutoff, only chains whose
 // rebuild is due at relative step rel; mode >= 0: that criterion, all chains.
 template <int L>
-__global__ void __launch_bounds__(256) k_rebuild(Dev D, Tables T, int cur, int par, int rel, int mode, int g0, int g1) {
+__global__ void __launch_bounds__(256) k_rebuild(Dev D, const Tables* __restrict__ T, int cur, int par, int rel, int mode, int g0, int g1) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int g = g0 + t / L;
   const int l = t & (L - 1);
@@ -118,9 +122,9 @@ __global__ void __launch_bounds__(256) k_rebuild(Dev D, Tables T, int cur, int p
     if (P.cutoff == SOPG_CUTOFF_NONE || (P.phase + rel) % P.nnlup != 0) return;
     crit = P.cutoff;
   }
-  const double* ux = D.ux[cur];
-  const double* uy = D.uy[cur];
-  const double* uz = D.uz[cur];
+  const double* ux = cur ? D.ux[1] : D.ux[0];
+  const double* uy = cur ? D.uy[1] : D.uy[0];
+  const double* uz = cur ? D.uz[1] : D.uz[0];
   const double boxl = P.boxl, qbox = P.qbox;
   uint8_t* aplw = par ? D.am_pl1 : D.am_pl0;
 
@@ -166,8 +170,8 @@ __global__ void __launch_bounds__(256) k_rebuild(Dev D, Tables T, int cur, int p
           const double dz = mimg(uz[j] - uz[i], boxl, qbox);
           const double d2 = dx * dx + dy * dy + dz * dz;
           bool pl;
-          if (crit == SOPG_CUTOFF_NEIGHBOR) { keep = d2 < T.nlrc2[it][jt]; pl = keep && d2 < T.plrc2[it][jt]; }
-          else if (crit == SOPG_CUTOFF_CELL) { keep = cell_adjacent(D, P, i, j); pl = keep && d2 < T.plrc2[it][jt]; }
+          if (crit == SOPG_CUTOFF_NEIGHBOR) { keep = d2 < T->nlrc2[it][jt]; pl = keep && d2 < T->plrc2[it][jt]; }
+          else if (crit == SOPG_CUTOFF_CELL) { keep = cell_adjacent(D, P, i, j); pl = keep && d2 < T->plrc2[it][jt]; }
           else { keep = true; pl = true; }
           w = slot_make((uint32_t)c, (uint32_t)side, (uint32_t)it, (uint32_t)jt) | (pl ? (1u << (30 + par)) : 0u);
         }
@@ -196,8 +200,8 @@ __global__ void __launch_bounds__(256) k_rebuild(Dev D, Tables T, int cur, int p
         const double dz = mimg(uz[j] - uz[i], boxl, qbox);
         const double d2 = dx * dx + dy * dy + dz * dz;
         bool pl;
-        if (crit == SOPG_CUTOFF_NEIGHBOR) { keep = d2 < T.nlrc2[it][jt]; pl = keep && d2 < T.plrc2[it][jt]; }
-        else if (crit == SOPG_CUTOFF_CELL) { keep = cell_adjacent(D, P, i, j); pl = keep && d2 < T.plrc2[it][jt]; }
+        if (crit == SOPG_CUTOFF_NEIGHBOR) { keep = d2 < T->nlrc2[it][jt]; pl = keep && d2 < T->plrc2[it][jt]; }
+        else if (crit == SOPG_CUTOFF_CELL) { keep = cell_adjacent(D, P, i, j); pl = keep && d2 < T->plrc2[it][jt]; }
         else { keep = true; pl = true; }
         const int o = side ? i : j;
         w = slot_make((uint32_t)(o - P.off), (uint32_t)side, (uint32_t)it, (uint32_t)jt) | (pl ? (1u << (30 + par)) : 0u);
@@ -218,15 +222,15 @@ __global__ void __launch_bounds__(256) k_rebuild(Dev D, Tables T, int cur, int p
 
 // Pair-list filter at the current positions (sop.cpp:473-553), parity par.
 template <int L>
-__global__ void __launch_bounds__(256) k_pairfilter(Dev D, Tables T, int cur, int par, int g0, int g1) {
+__global__ void __launch_bounds__(256) k_pairfilter(Dev D, const Tables* __restrict__ T, int cur, int par, int g0, int g1) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int g = g0 + t / L;
   const int l = t & (L - 1);
   if (g >= g1) return;
   const ChainP& P = D.cp[D.chain[g]];
-  const double* ux = D.ux[cur];
-  const double* uy = D.uy[cur];
-  const double* uz = D.uz[cur];
+  const double* ux = cur ? D.ux[1] : D.ux[0];
+  const double* uy = cur ? D.uy[1] : D.uy[0];
+  const double* uz = cur ? D.uz[1] : D.uz[0];
   const double boxl = P.boxl, qbox = P.qbox;
   uint8_t* aplw = par ? D.am_pl1 : D.am_pl0;
   const uint8_t pbit = (uint8_t)(2 << par);
@@ -257,7 +261,7 @@ __global__ void __launch_bounds__(256) k_pairfilter(Dev D, Tables T, int cur, in
     const double dy = mimg(uy[j] - uy[i], boxl, qbox);
     const double dz = mimg(uz[j] - uz[i], boxl, qbox);
     const double d2 = dx * dx + dy * dy + dz * dz;
-    const bool pl = (P.cutoff == SOPG_CUTOFF_NONE) ? true : d2 < T.plrc2[it][jt];
+    const bool pl = (P.cutoff == SOPG_CUTOFF_NONE) ? true : d2 < T->plrc2[it][jt];
     S[q] = (w & ~sbit) | (pl ? sbit : 0u);
   }
 }
@@ -271,12 +275,12 @@ __global__ void k_posupdate(Dev D, int src, int dst, int g0, int g1) {
   const double iy = P.a1 * D.vy[g] + P.a2 * D.fy[g];
   const double iz = P.a1 * D.vz[g] + P.a2 * D.fz[g];
   D.ix[g] = ix; D.iy[g] = iy; D.iz[g] = iz;
-  D.px[dst][g] = mimg(D.px[src][g] + ix, P.boxl, P.qbox);
-  D.py[dst][g] = mimg(D.py[src][g] + iy, P.boxl, P.qbox);
-  D.pz[dst][g] = mimg(D.pz[src][g] + iz, P.boxl, P.qbox);
-  D.ux[dst][g] = D.ux[src][g] + ix;
-  D.uy[dst][g] = D.uy[src][g] + iy;
-  D.uz[dst][g] = D.uz[src][g] + iz;
+  (dst ? D.px[1] : D.px[0])[g] = mimg((src ? D.px[1] : D.px[0])[g] + ix, P.boxl, P.qbox);
+  (dst ? D.py[1] : D.py[0])[g] = mimg((src ? D.py[1] : D.py[0])[g] + iy, P.boxl, P.qbox);
+  (dst ? D.pz[1] : D.pz[0])[g] = mimg((src ? D.pz[1] : D.pz[0])[g] + iz, P.boxl, P.qbox);
+  (dst ? D.ux[1] : D.ux[0])[g] = (src ? D.ux[1] : D.ux[0])[g] + ix;
+  (dst ? D.uy[1] : D.uy[0])[g] = (src ? D.uy[1] : D.uy[0])[g] + iy;
+  (dst ? D.uz[1] : D.uz[0])[g] = (src ? D.uz[1] : D.uz[0])[g] + iz;
 }
 
 // Velocity half of underdamped_iteration (sop.cpp:316-328).
@@ -301,16 +305,16 @@ __global__ void k_velupdate(Dev D, int g0, int g1) {
 enum { F_STORE = 1, F_VEL = 2, F_ENERGY = 4, F_ENERGY_ALL = 8 };
 
 template <int L, bool NEXT>
-__global__ void __launch_bounds__(256) k_force(Dev D, Tables T, int cur, int pr, int rel, int flags,
+__global__ void __launch_bounds__(256) k_force(Dev D, const Tables* __restrict__ T, int cur, int pr, int rel, int flags,
                                                const double* __restrict__ noise, int g0, int g1) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int g = g0 + t / L;
   const int l = t & (L - 1);
   if (g >= g1) return;
   const ChainP& P = D.cp[D.chain[g]];
-  const double* __restrict__ ux = D.ux[cur];
-  const double* __restrict__ uy = D.uy[cur];
-  const double* __restrict__ uz = D.uz[cur];
+  const double* __restrict__ ux = cur ? D.ux[1] : D.ux[0];
+  const double* __restrict__ uy = cur ? D.uy[1] : D.uy[0];
+  const double* __restrict__ uz = cur ? D.uz[1] : D.uz[0];
   const double boxl = P.boxl, qbox = P.qbox;
   const int pw = pr ^ 1;
   const bool energy = (flags & F_ENERGY_ALL) || ((flags & F_ENERGY) && ((P.iup0 + rel) % P.nup == 0));
@@ -359,10 +363,10 @@ __global__ void __launch_bounds__(256) k_force(Dev D, Tables T, int cur, int pr,
         const double d = sqrt(dx * dx + dy * dy + dz * dz);
         const double dev = d - D.br0[kb];
         const double dev2 = dev * dev;
-        const double temp = -T.k_bnd * dev / d / (1.0 - dev2 / T.R0sq);
+        const double temp = -T->k_bnd * dev / d / (1.0 - dev2 / T->R0sq);
         fx = temp * dx; fy = temp * dy; fz = temp * dz;
         if (side == 0) { fx = -fx; fy = -fy; fz = -fz; }
-        if (energy && side == 0) eb += log1p(-dev * dev / T.R0sq);
+        if (energy && side == 0) eb += log1p(-dev * dev / T->R0sq);
       }
       fold3<L>(Fx, Fy, Fz, has, fx, fy, fz);
     }
@@ -382,10 +386,10 @@ __global__ void __launch_bounds__(256) k_force(Dev D, Tables T, int cur, int pr,
         const double dz = mimg(uz[k] - uz[i], boxl, qbox);
         const double d = sqrt(dx * dx + dy * dy + dz * dz);
         const double d8 = pow8_cr(d);
-        const double co1 = T.f_ang_ss_coeff / d8;
+        const double co1 = T->f_ang_ss_coeff / d8;
         fx = co1 * dx; fy = co1 * dy; fz = co1 * dz;
         if (side == 0) { fx = -fx; fy = -fy; fz = -fz; }
-        if (energy && side == 0) ea += T.e_ang_ss_coeff / pow6_cr(d);
+        if (energy && side == 0) ea += T->e_ang_ss_coeff / pow6_cr(d);
       }
       fold3<L>(Fx, Fy, Fz, has, fx, fy, fz);
     }
@@ -412,15 +416,15 @@ __global__ void __launch_bounds__(256) k_force(Dev D, Tables T, int cur, int pr,
             const int tt = D.am_t[k], it = tt & 3, jt = tt >> 2;
             const double d6 = d2 * d2 * d2;
             const double d12 = d6 * d6;
-            const bool skip = (d2 < P.thr_att) && att_tol_skip(D, P, k, pr, d2);
+            const bool skip = (d2 < P.thr_att) && att_tol_skip(pr ? D.am_pl1 : D.am_pl0, D.am_r2, P.att_off, k, d2);
             if (!skip) {
-              const double co1 = T.fatt[it][jt] / d2 * ((D.am_r12[k] / d12) - (D.am_r6[k] / d6));
+              const double co1 = T->fatt[it][jt] / d2 * ((D.am_r12[k] / d12) - (D.am_r6[k] / d6));
               fx = co1 * dx; fy = co1 * dy; fz = co1 * dz;
               if (side == 1) { fx = -fx; fy = -fy; fz = -fz; }
               contrib = true;
             }
             if (energy && side == 0)
-              eatt += T.catt[it][jt] * (D.am_r12[k] / d12) - 2.0 * (D.am_r6[k] / d6);
+              eatt += T->catt[it][jt] * (D.am_r12[k] / d12) - 2.0 * (D.am_r6[k] / d6);
           }
           if constexpr (NEXT) {
             const bool npl = (P.cutoff == SOPG_CUTOFF_NONE) || d2 < D.am_plrc2[k];
@@ -455,17 +459,17 @@ __global__ void __launch_bounds__(256) k_force(Dev D, Tables T, int cur, int pr,
         if (w & rbit) {
           const double d6 = d2 * d2 * d2;
           const double d12 = d6 * d6;
-          const bool skip = (d2 < T.thr_rep) && rep_tol_skip(D, T, P, pr, d2);
+          const bool skip = (d2 < T->thr_rep) && rep_tol_skip(pr ? D.am_pl1 : D.am_pl0, D.am_t, T, P.att_off, P.natt, d2);
           if (!skip) {
-            const double co1 = T.frep[it][jt] / d2 * (2.0 * T.s12[it][jt] / d12 + T.s6[it][jt] / d6);
+            const double co1 = T->frep[it][jt] / d2 * (2.0 * T->s12[it][jt] / d12 + T->s6[it][jt] / d6);
             fx = co1 * dx; fy = co1 * dy; fz = co1 * dz;
             if (side == 1) { fx = -fx; fy = -fy; fz = -fz; }
             contrib = true;
           }
-          if (energy && side == 0) erep += T.crep[it][jt] * (T.s12[it][jt] / d12 + T.s6[it][jt] / d6);
+          if (energy && side == 0) erep += T->crep[it][jt] * (T->s12[it][jt] / d12 + T->s6[it][jt] / d6);
         }
         if constexpr (NEXT) {
-          const bool npl = (P.cutoff == SOPG_CUTOFF_NONE) || d2 < T.plrc2[it][jt];
+          const bool npl = (P.cutoff == SOPG_CUTOFF_NONE) || d2 < T->plrc2[it][jt];
           const uint32_t nw = (w & ~wbit) | (npl ? wbit : 0u);
           if (nw != w) S[q] = nw;
         }
@@ -493,17 +497,22 @@ __global__ void __launch_bounds__(256) k_force(Dev D, Tables T, int cur, int pr,
   }
   if constexpr (NEXT) {
     // the next step's position half (sop.cpp:288-310) into buffer cur^1
-    const int nx = cur ^ 1;
     const double ix = P.a1 * vx + P.a2 * Fx;
     const double iy = P.a1 * vy + P.a2 * Fy;
     const double iz = P.a1 * vz + P.a2 * Fz;
     D.ix[g] = ix; D.iy[g] = iy; D.iz[g] = iz;
-    D.px[nx][g] = mimg(D.px[cur][g] + ix, boxl, qbox);
-    D.py[nx][g] = mimg(D.py[cur][g] + iy, boxl, qbox);
-    D.pz[nx][g] = mimg(D.pz[cur][g] + iz, boxl, qbox);
-    D.ux[nx][g] = ux[g] + ix;
-    D.uy[nx][g] = uy[g] + iy;
-    D.uz[nx][g] = uz[g] + iz;
+    double* pxs = cur ? D.px[1] : D.px[0];
+    double* pys = cur ? D.py[1] : D.py[0];
+    double* pzs = cur ? D.pz[1] : D.pz[0];
+    double* pxd = cur ? D.px[0] : D.px[1];
+    double* pyd = cur ? D.py[0] : D.py[1];
+    double* pzd = cur ? D.pz[0] : D.pz[1];
+    pxd[g] = mimg(pxs[g] + ix, boxl, qbox);
+    pyd[g] = mimg(pys[g] + iy, boxl, qbox);
+    pzd[g] = mimg(pzs[g] + iz, boxl, qbox);
+    (cur ? D.ux[0] : D.ux[1])[g] = ux[g] + ix;
+    (cur ? D.uy[0] : D.uy[1])[g] = uy[g] + iy;
+    (cur ? D.uz[0] : D.uz[1])[g] = uz[g] + iz;
   }
 }
 
@@ -514,9 +523,9 @@ __global__ void k_observe(Dev D, int cur, int par, int rel, int all, int what, i
   if (g >= g1) return;
   const ChainP& P = D.cp[D.chain[g]];
   if (!all && (P.iup0 + rel) % P.nup != 0) return;
-  const double* ux = D.ux[cur];
-  const double* uy = D.uy[cur];
-  const double* uz = D.uz[cur];
+  const double* ux = cur ? D.ux[1] : D.ux[0];
+  const double* uy = cur ? D.uy[1] : D.uy[0];
+  const double* uz = cur ? D.uz[1] : D.uz[0];
   const double boxl = P.boxl, qbox = P.qbox;
   const uint8_t* apl = par ? D.am_pl1 : D.am_pl0;
   double* r = D.red + (size_t)g * kRed;
@@ -585,18 +594,18 @@ __global__ void __launch_bounds__(256) k_chain_reduce(Dev D, int c0, double* out
 static inline unsigned blocks_for(long threads) { return (unsigned)((threads + 255) / 256); }
 
 template <int L>
-static void launch_rebuild_L(const Dev& D, const Tables& T, int cur, int par, int rel, int mode, int g0, int g1,
+static void launch_rebuild_L(const Dev& D, const Tables* T, int cur, int par, int rel, int mode, int g0, int g1,
                              hipStream_t s) {
   hipLaunchKernelGGL(k_rebuild<L>, dim3(blocks_for((long)(g1 - g0) * L)), dim3(256), 0, s, D, T, cur, par, rel,
                      mode, g0, g1);
 }
 template <int L>
-static void launch_pairfilter_L(const Dev& D, const Tables& T, int cur, int par, int g0, int g1, hipStream_t s) {
+static void launch_pairfilter_L(const Dev& D, const Tables* T, int cur, int par, int g0, int g1, hipStream_t s) {
   hipLaunchKernelGGL(k_pairfilter<L>, dim3(blocks_for((long)(g1 - g0) * L)), dim3(256), 0, s, D, T, cur, par, g0,
                      g1);
 }
 template <int L>
-static void launch_force_L(const Dev& D, const Tables& T, int cur, int pr, int rel, int flags, bool next,
+static void launch_force_L(const Dev& D, const Tables* T, int cur, int pr, int rel, int flags, bool next,
                            const double* noise, int g0, int g1, hipStream_t s) {
   const dim3 grid(blocks_for((long)(g1 - g0) * L)), block(256);
   if (next)
@@ -619,18 +628,18 @@ static void launch_force_L(const Dev& D, const Tables& T, int cur, int pr, int r
 void launch_cellcoords(const Dev& D, int cur, int g0, int g1, hipStream_t s) {
   hipLaunchKernelGGL(k_cellcoords, dim3(blocks_for(g1 - g0)), dim3(256), 0, s, D, cur, g0, g1);
 }
-void launch_rebuild(int L, const Dev& D, const Tables& T, int cur, int par, int rel, int mode, int g0, int g1,
+void launch_rebuild(int L, const Dev& D, const Tables* T, int cur, int par, int rel, int mode, int g0, int g1,
                     hipStream_t s) {
 #define C_(X) launch_rebuild_L<X>(D, T, cur, par, rel, mode, g0, g1, s)
   SOPG_DISPATCH_L(L, C_)
 #undef C_
 }
-void launch_pairfilter(int L, const Dev& D, const Tables& T, int cur, int par, int g0, int g1, hipStream_t s) {
+void launch_pairfilter(int L, const Dev& D, const Tables* T, int cur, int par, int g0, int g1, hipStream_t s) {
 #define C_(X) launch_pairfilter_L<X>(D, T, cur, par, g0, g1, s)
   SOPG_DISPATCH_L(L, C_)
 #undef C_
 }
-void launch_force(int L, const Dev& D, const Tables& T, int cur, int pr, int rel, int flags, bool next,
+void launch_force(int L, const Dev& D, const Tables* T, int cur, int pr, int rel, int flags, bool next,
                   const double* noise, int g0, int g1, hipStream_t s) {
 #define C_(X) launch_force_L<X>(D, T, cur, pr, rel, flags, next, noise, g0, g1, s)
   SOPG_DISPATCH_L(L, C_)

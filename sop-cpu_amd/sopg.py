@@ -71,7 +71,7 @@ EXPORTS = [
     "sopg_get_state", "sopg_nchains", "sopg_chain_nbead", "sopg_update_neighbor_list",
     "sopg_update_cell_list", "sopg_update_pair_list", "sopg_force_eval", "sopg_energy_eval",
     "sopg_underdamped_iteration", "sopg_calculate_observables", "sopg_get_list", "sopg_run",
-    "sopg_set_step", "sopg_sync",
+    "sopg_set_step", "sopg_sync", "sopg_list_counts", "sopg_profile", "sopg_profile_read",
     "sopgen_saw", "sopgen_native", "sopgen_maxwell", "sopgen_check", "sopgen_write_topology",
     "sopgen_write_xyz",
 ]
@@ -93,7 +93,7 @@ def lib():
         L.sopg_last_error.restype = C.c_char_p
         L.sopg_default_tables.argtypes = [C.POINTER(Tables)]
         L.sopg_set_tables.argtypes = [vp, C.POINTER(Tables)]
-        L.sopg_set_lanes.argtypes = [vp, C.c_int]
+        L.sopg_set_lanes.argtypes = [vp, C.c_int, C.c_int]
         L.sopg_set_capacity.argtypes = [vp, C.c_int]
         L.sopg_add_chain.argtypes = [vp, C.POINTER(Topology), C.POINTER(ChainParams), _IP]
         L.sopg_set_chain_params.argtypes = [vp, C.c_int, C.POINTER(ChainParams)]
@@ -111,6 +111,9 @@ def lib():
         L.sopg_run.argtypes = [vp, C.c_long, vp, C.POINTER(Record), C.c_long, C.POINTER(C.c_long)]
         L.sopg_set_step.argtypes = [vp, C.c_int, C.c_double, C.c_int, C.c_int]
         L.sopg_sync.argtypes = [vp]
+        L.sopg_list_counts.argtypes = [vp, C.c_int, _IP, _IP, _IP, _IP]
+        L.sopg_profile.argtypes = [vp, C.c_int]
+        L.sopg_profile_read.argtypes = [vp, C.c_int, _DP, C.POINTER(C.c_long)]
         L.sopgen_saw.argtypes = [C.c_int, C.c_uint64, vp]
         L.sopgen_native.argtypes = [C.c_int, C.c_uint64, vp]
         L.sopgen_maxwell.argtypes = [C.c_int, C.c_double, C.c_uint64, vp]
@@ -236,7 +239,7 @@ def chain_params(T=0.6, zeta=50.0, h=0.05, boxl=500.0, ncell=55.0, lcell=None, c
 class Context:
     """One device context (a ragged batch of chains) on one GPU."""
 
-    def __init__(self, device=0, lanes=0, capacity=0):
+    def __init__(self, device=0, lanes=0, capacity=0, list_lanes=0):
         self.L = lib()
         h = C.c_void_p()
         rc = self.L.sopg_create(device, C.byref(h))
@@ -244,8 +247,8 @@ class Context:
             raise SopgError(f"sopg_create failed ({rc}): no usable GPU")
         self.h = h
         self._keep = []
-        if lanes:
-            self._chk(self.L.sopg_set_lanes(self.h, lanes))
+        if lanes or list_lanes:
+            self._chk(self.L.sopg_set_lanes(self.h, lanes, list_lanes))
         if capacity:
             self._chk(self.L.sopg_set_capacity(self.h, capacity))
 
@@ -365,3 +368,16 @@ class Context:
 
     def sync(self):
         self._chk(self.L.sopg_sync(self.h))
+
+    def list_counts(self, chain=0):
+        v = [C.c_int() for _ in range(4)]
+        self._chk(self.L.sopg_list_counts(self.h, chain, *[C.byref(x) for x in v]))
+        return dict(zip(("nnl_att", "nnl_rep", "nil_att", "nil_rep"), (x.value for x in v)))
+
+    def profile(self, enable=True):
+        self._chk(self.L.sopg_profile(self.h, 1 if enable else 0))
+
+    def profile_read(self, kernel):
+        ms, n = C.c_double(), C.c_long()
+        self._chk(self.L.sopg_profile_read(self.h, kernel, C.byref(ms), C.byref(n)))
+        return ms.value, n.value

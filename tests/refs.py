@@ -49,8 +49,8 @@ class Oracle:
             L.orc_rnd.restype = C.c_double
             L.orc_pow_cr.argtypes = [C.c_double, C.c_int]
             L.orc_pow_cr.restype = C.c_double
-            L.orc_ran2.argtypes = [_vp]
-            L.orc_ran2.restype = C.c_double
+            L.orc_sys_ran2.argtypes = [_vp]
+            L.orc_sys_ran2.restype = C.c_double
             cls._lib = L
         return cls._lib
 
@@ -135,6 +135,7 @@ class Ref:
             L.ref_get_list.argtypes = [C.c_int, _vp, _vp]
             L.ref_gasdev_n.argtypes = [_vp, C.c_long]
             L.ref_set_seed.argtypes = [C.c_int]
+            L.ref_rng_restart.argtypes = [C.c_char_p]
             L.ref_ran2.restype = C.c_double
             L.ref_gasdev.restype = C.c_double
             cls._lib = L
@@ -143,6 +144,13 @@ class Ref:
     def __init__(self):
         self.L = self.lib()
         self.L.ref_init()
+
+    def reset_rng(self, seed, tmpdir):
+        """Fresh generator state (mseed = seed, no cached deviate) via Ran_Gen::restart."""
+        path = os.path.join(str(tmpdir), "ran_gen_state.dat")
+        with open(path, "w") as f:
+            f.write(f"{seed}\n0\n0.0\n123456789\n0\n" + "0\n" * 32)
+        self.L.ref_rng_restart(path.encode())
 
     def set(self, key, val):
         self.L.ref_set(key.encode(), str(val).encode())

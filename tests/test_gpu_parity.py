@@ -47,7 +47,7 @@ def make_pair(root, n, seed, cmds, params, coords_vel=True, lanes=0, topo=None):
     x, v, d = chain_inputs(root, n, seed)
     orc = Oracle()
     setup_system(orc, d, cmds)
-    ctx = sopg.Context(lanes=lanes)
+    ctx = sopg.Context(lanes=lanes, list_lanes=lanes)
     ctx.add_chain(topo or sopg.chain_topology(x), params)
     if coords_vel:
         orc.set_coords(2, v)
@@ -255,7 +255,7 @@ def test_explicit_master_lists(inputs_root, pow_cr, tmp_path):
 
 
 def _philox_ctx(chains, lanes=0):
-    ctx = sopg.Context(lanes=lanes)
+    ctx = sopg.Context(lanes=lanes, list_lanes=lanes)
     for (x, v, seed) in chains:
         ctx.add_chain(sopg.chain_topology(x), sopg.chain_params(cutoff=sopg.CUTOFF_NEIGHBOR, nnlup=20, nup=100,
                                                                  noise_mode=sopg.NOISE_PHILOX, seed=seed))
