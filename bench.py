@@ -1,22 +1,29 @@
 #!/usr/bin/env python3
 """Throughput benchmark of the SOP Langevin hot path on one MI355X.
 
-Workload: plain SOP single chain, N=3000 beads (seeded SAW + MC collapse into a
-sphere of 3.0*N^(1/3) A), FP64, cell list with 12 A cells (boxl 504, ncell 42)
-rebuilt every 20 steps, pair list every step, underdamped Langevin h=0.05,
-zeta=50, kT=0.6, records (energy + observables) every 1000 steps, device
-(Philox) noise.  Inputs are resident in HBM before the timed region.
+Default workload (mixed256): 256 independent chains, N drawn log-uniform in
+[50, 5000] from seed 4, each its own synthetic native structure (seeded SAW +
+MC collapse into 3.0*N^(1/3) A), all advanced together in one ragged batch:
+FP64, distance neighbor list rebuilt every 20 steps, pair list every step,
+underdamped Langevin h=0.05, zeta=50, kT=0.6, records (energy + observables)
+every 1000 steps, device Philox noise.  Other workloads: --workload plain3000
+(one N=3000 chain, 12 A cell list, rebuild every 20 steps) and replicas1024
+(1024 trajectories of one N=76 chain, rebuild every 50 steps).
 
-One step = the reference's underdamped_ctrl loop body (sop.cpp:153-189).
-value = bead-steps/s = N * K / wall seconds of exactly K timed steps.
+One step = the reference's underdamped_ctrl loop body (sop.cpp:153-189) for
+every chain.  value = bead-steps/s = sum(N) * K / wall seconds of exactly K
+timed steps of the batched form, inputs resident in HBM.  plain = the same
+chains advanced one chain per context (one chain per launch), K_plain steps
+each; speedup_vs_plain = value / plain.
 
 roofline: the fused step kernel (SOPG_KERNEL_STEP), timed with HIP events on
 the library's stream over the timed region; algorithmic bytes per launch =
-192 N + 16 nbnd + 8 nang + 40 nnl_att + 8 nnl_rep (bead state read+write, bond
-and angle indices, neighbor list streamed once), against 8 TB/s HBM.
+sum over chains of 192 N + 16 nbnd + 8 nang + 40 nnl_att + 8 nnl_rep, against
+8 TB/s HBM.
 
-cpu_baseline: the reference itself (oracle/_ref, compiled from its sources with
-its own flags) on a bounded sample of the same workload, one host core.
+cpu_baseline: the reference itself (oracle/_ref, compiled from SOP-CPU's
+sources with its own flags), one host core, on a bounded sample of the same
+workload (named in `sample`).
 """
 import argparse
 import json
@@ -24,6 +31,7 @@ import os
 import sys
 import tempfile
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "sop-cpu_amd"))
@@ -34,101 +42,197 @@ import numpy as np  # noqa: E402
 import sopg  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-N_BEADS = 3000
-SEED = 2024
 KT = 0.6
-NNLUP = 20
 NUP = 1000
-BOXL, NCELL = 504.0, 42.0
+
+WORKLOADS = {
+    "mixed256": dict(desc="256 chains, N log-uniform in [50, 5000] (seed 4), each its own native structure; "
+                          "neighbor list every 20 steps; kT 0.6, h 0.05, zeta 50; Philox noise",
+                     cutoff=sopg.CUTOFF_NEIGHBOR, nnlup=20, boxl=500.0, ncell=55.0),
+    "plain3000": dict(desc="one chain N=3000; cell list 12 A (boxl 504, ncell 42) rebuilt every 20 steps; "
+                           "kT 0.6, h 0.05, zeta 50; Philox noise",
+                      cutoff=sopg.CUTOFF_CELL, nnlup=20, boxl=504.0, ncell=42.0),
+    "replicas1024": dict(desc="1024 replicas of one N=76 chain, own seeds and Maxwell velocities; "
+                              "neighbor list every 50 steps; kT 0.6, h 0.05, zeta 50; Philox noise",
+                         cutoff=sopg.CUTOFF_NEIGHBOR, nnlup=50, boxl=500.0, ncell=55.0),
+}
 
 
-def gpu_run(x, v, steps, warmup, lanes, list_lanes=0):
-    topo = sopg.chain_topology(x)
-    p = sopg.chain_params(T=KT, boxl=BOXL, ncell=NCELL, cutoff=sopg.CUTOFF_CELL, nnlup=NNLUP, nup=NUP,
-                          noise_mode=sopg.NOISE_PHILOX, seed=SEED)
+def chain_sizes(workload):
+    if workload == "mixed256":
+        rng = np.random.default_rng(4)
+        return [int(v) for v in np.round(np.exp(rng.uniform(np.log(50), np.log(5000), 256)))]
+    if workload == "plain3000":
+        return [3000]
+    return [76] * 1024
+
+
+def make_chains(workload, threads):
+    sizes = chain_sizes(workload)
+    if workload == "replicas1024":
+        x = sopg.gen_native(76, 76)
+        return [(x, sopg.gen_maxwell(76, KT, 10_000 + r), 10_000 + r) for r in range(len(sizes))]
+
+    def one(c):
+        n = sizes[c]
+        return (sopg.gen_native(n, 1000 + c), sopg.gen_maxwell(n, KT, 1000 + c), 1000 + c)
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        return list(ex.map(one, range(len(sizes))))
+
+
+def build_ctx(chains, w, lanes=0, list_lanes=0, topo_cache=None):
     ctx = sopg.Context(device=0, lanes=lanes, list_lanes=list_lanes)
-    ctx.add_chain(topo, p)
-    ctx.put_state(0, pos=x, unc=x, vel=v, force=np.zeros_like(x))
-    # underdamped_ctrl preamble: lists, energy, first force (untimed)
-    ctx.update_cell_list()
-    ctx.update_pair_list()
-    ctx.energy_eval()
-    ctx.force_eval()
+    topos = []
+    for (x, v, seed) in chains:
+        key = id(x)
+        t = topo_cache.get(key) if topo_cache is not None else None
+        if t is None:
+            t = sopg.chain_topology(x)
+            if topo_cache is not None:
+                topo_cache[key] = t
+        topos.append(t)
+        ctx.add_chain(t, sopg.chain_params(T=KT, boxl=w["boxl"], ncell=w["ncell"], cutoff=w["cutoff"],
+                                           nnlup=w["nnlup"], nup=NUP, noise_mode=sopg.NOISE_PHILOX, seed=seed))
+    for c, (x, v, seed) in enumerate(chains):
+        ctx.put_state(c, pos=x, unc=x, vel=v, force=np.zeros_like(x))
+    # underdamped_ctrl preamble (sop.cpp:119-147): lists, energy, first force (untimed)
+    for c in range(len(chains)):
+        if w["cutoff"] == sopg.CUTOFF_CELL:
+            ctx.update_cell_list(c)
+        else:
+            ctx.update_neighbor_list(c)
+        ctx.update_pair_list(c)
+        ctx.force_eval(c)
+    return ctx, topos
+
+
+def timed_run(ctx, steps, warmup, profile):
     if warmup:
         ctx.run(warmup)
-    counts = ctx.list_counts()
     ctx.sync()
-    ctx.profile(True)
+    if profile:
+        ctx.profile(True)
     t0 = time.perf_counter()
     recs = ctx.run(steps)
     ctx.sync()
     t1 = time.perf_counter()
-    step_ms, step_n = ctx.profile_read(0)
-    reb_ms, reb_n = ctx.profile_read(1)
-    counts_end = ctx.list_counts()
-    ctx.profile(False)
+    prof = {k: ctx.profile_read(k) for k in range(4)} if profile else None
+    if profile:
+        ctx.profile(False)
+    return t1 - t0, prof, recs
+
+
+def gpu_batched(chains, w, steps, warmup, lanes, list_lanes, topo_cache):
+    ctx, topos = build_ctx(chains, w, lanes, list_lanes, topo_cache)
+    seconds, prof, recs = timed_run(ctx, steps, warmup, True)
+    bytes_step = 0
+    counts = []
+    for c, t in enumerate(topos):
+        k = ctx.list_counts(c)
+        counts.append(k)
+        bytes_step += (192 * t["n"] + 16 * len(t["ib"]) + 8 * len(t["ia"]) + 40 * k["nnl_att"] + 8 * k["nnl_rep"])
     ctx.close()
-    return dict(seconds=t1 - t0, step_ms=step_ms, step_n=step_n, reb_ms=reb_ms, reb_n=reb_n,
-                counts=counts, counts_end=counts_end, records=len(recs), topo=topo)
+    return dict(seconds=seconds, prof=prof, bytes_step=bytes_step, counts=counts, records=len(recs))
 
 
-def cpu_baseline(x, v, steps):
-    """The reference (oracle/_ref) on `steps` steps of the same workload, one core."""
+def gpu_plain(chains, w, steps, warmup, topo_cache, lanes=0, list_lanes=0):
+    """One chain per context: every launch advances a single chain."""
+    total_bs, total_t = 0.0, 0.0
+    for ch in chains:
+        ctx, topos = build_ctx([ch], w, lanes, list_lanes, topo_cache)
+        seconds, _, _ = timed_run(ctx, steps, warmup, False)
+        total_bs += topos[0]["n"] * steps
+        total_t += seconds
+        ctx.close()
+    return total_bs / total_t, total_t
+
+
+def cpu_baseline(chains, w, budget_s=20.0, max_n=3200):
+    """The reference on a size-stratified sample of the workload's chains, one core."""
     from refs import Ref, setup_system, std_cmds
     if not Ref.available():
         return None
-    n = len(x) // 3
-    with tempfile.TemporaryDirectory() as d:
-        sopg.write_inputs(x, d, vel=v)
-        r = Ref()
-        setup_system(r, d, std_cmds(temp=KT, cutoff="celllist", nnlup=NNLUP, nup=NUP,
-                                    extra=[("boxl", str(int(BOXL))), ("ncell", str(int(NCELL)))]))
-    r.set_coords(2, v)
-    r.call("update_cell_list")
-    r.call("update_pair_list")
-    r.call("set_potential_forces")
-    r.call("energy_eval")
-    r.call("force_eval")
-    inlup = 1
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        if inlup % NNLUP == 0:
-            r.call("update_cell_list")
-            inlup = 0
-        inlup += 1
+    order = sorted(range(len(chains)), key=lambda c: len(chains[c][0]))
+    picks = []
+    for q in (0.1, 0.3, 0.5, 0.7, 0.9):
+        c = order[min(int(q * len(order)), len(order) - 1)]
+        if len(chains[c][0]) // 3 <= max_n and c not in picks:
+            picks.append(c)
+    cut = "celllist" if w["cutoff"] == sopg.CUTOFF_CELL else "neighborlist"
+    extra = [("boxl", str(int(w["boxl"]))), ("ncell", str(int(w["ncell"])))] if cut == "celllist" else []
+    per = budget_s / max(len(picks), 1)
+    bs, secs, desc = 0.0, 0.0, []
+    for c in picks:
+        x, v, _ = chains[c]
+        n = len(x) // 3
+        with tempfile.TemporaryDirectory() as d:
+            sopg.write_inputs(x, d, vel=v)
+            r = Ref()
+            setup_system(r, d, std_cmds(temp=KT, cutoff=cut, nnlup=w["nnlup"], nup=NUP, extra=extra))
+        r.set_coords(2, v)
+        r.call("update_cell_list" if cut == "celllist" else "update_neighbor_list")
         r.call("update_pair_list")
-        r.call("underdamped_iteration")
-    dt = time.perf_counter() - t0
-    return dict(value=n * steps / dt, unit="bead-steps/s", cores=1, kind="reference",
-                sample=f"{steps} steps ({steps // NNLUP} cell-list periods) of the N={n} workload, "
-                       f"SOP-CPU compiled from its sources with g++ -O3, {dt:.1f} s")
+        r.call("set_potential_forces")
+        r.call("energy_eval")
+        r.call("force_eval")
+        inlup, steps = 1, 0
+        t0 = time.perf_counter()
+        while steps < 20 * w["nnlup"] and (steps < w["nnlup"] or time.perf_counter() - t0 < per):
+            if inlup % w["nnlup"] == 0:
+                r.call("update_cell_list" if cut == "celllist" else "update_neighbor_list")
+                inlup = 0
+            inlup += 1
+            r.call("update_pair_list")
+            r.call("underdamped_iteration")
+            steps += 1
+        dt = time.perf_counter() - t0
+        bs += n * steps
+        secs += dt
+        desc.append(f"N={n}:{steps}")
+    return dict(value=bs / secs, unit="bead-steps/s", cores=1, kind="reference",
+                sample=f"SOP-CPU compiled from its sources (g++ -O3), one core, chains at the 10/30/50/70/90% "
+                       f"size quantiles of the workload (steps run: {', '.join(desc)}), {secs:.1f} s; "
+                       f"rate = sum(N*steps)/sum(seconds)")
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5000)
-    ap.add_argument("--warmup", type=int, default=500)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--workload", default="mixed256", choices=sorted(WORKLOADS))
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--list-lanes", type=int, default=0)
-    ap.add_argument("--cpu-steps", type=int, default=800)
+    ap.add_argument("--plain-steps", type=int, default=100)
+    ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (independent chains shard across GPUs with no "
               f"per-step exchange); --gpus {a.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
-    x = sopg.gen_native(N_BEADS, SEED)
-    v = sopg.gen_maxwell(N_BEADS, KT, SEED)
-    g = gpu_run(x, v, a.steps, a.warmup, a.lanes, a.list_lanes)
-    value = N_BEADS * a.steps / g["seconds"]
-    c = g["counts"]
-    topo = g["topo"]
-    nbnd, nang = len(topo["ib"]), len(topo["ia"])
-    bytes_per_launch = 192 * N_BEADS + 16 * nbnd + 8 * nang + 40 * c["nnl_att"] + 8 * c["nnl_rep"]
-    avg_s = g["step_ms"] / max(g["step_n"], 1) / 1e3
-    achieved = bytes_per_launch / avg_s / 1e9
-    cpu = None if a.no_cpu else cpu_baseline(x, v, a.cpu_steps)
+    w = WORKLOADS[a.workload]
+    t0 = time.perf_counter()
+    chains = make_chains(a.workload, a.threads)
+    print(f"[bench] generated {len(chains)} chains in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    sizes = [len(x) // 3 for x, _, _ in chains]
+    nb = sum(sizes)
+    topo_cache = {}
+    g = gpu_batched(chains, w, a.steps, a.warmup, a.lanes, a.list_lanes, topo_cache)
+    value = nb * a.steps / g["seconds"]
+    print(f"[bench] batched {value:.4g} bead-steps/s", file=sys.stderr, flush=True)
+    plain, plain_t = (None, None) if a.no_plain else gpu_plain(chains, w, a.plain_steps, 10, topo_cache, a.lanes,
+                                                              a.list_lanes)
+    if plain:
+        print(f"[bench] plain {plain:.4g} bead-steps/s ({plain_t:.1f} s)", file=sys.stderr, flush=True)
+    step_ms, step_n = g["prof"][sopg.KERNEL_STEP]
+    reb_ms, reb_n = g["prof"][sopg.KERNEL_REBUILD]
+    obs_ms, obs_n = g["prof"][sopg.KERNEL_OBSERVE]
+    avg_s = step_ms / max(step_n, 1) / 1e3
+    achieved = g["bytes_step"] / avg_s / 1e9
+    cpu = None if a.no_cpu else cpu_baseline(chains, w)
+    tot = lambda k: sum(c[k] for c in g["counts"])
     line = {
         "metric": "bead-steps/s (batched vs plain) at FP64 force parity; p50/p99 job latency",
         "value": value,
@@ -142,16 +246,18 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "plain SOP single chain N=3000, cell list 12 A (boxl 504, ncell 42), "
-                               "list rebuild every 20 steps, kT 0.6, h 0.05, zeta 50, Philox noise",
-                   "n_beads": N_BEADS, "form": "plain (one chain per launch)",
-                   "nnl_att": c["nnl_att"], "nnl_rep": c["nnl_rep"], "nil_att": c["nil_att"],
-                   "nil_rep": c["nil_rep"], "records": g["records"]},
+        "config": {"workload": f"{a.workload}: {w['desc']}", "chains": len(chains), "beads": nb,
+                   "n_min": min(sizes), "n_max": max(sizes), "form": "batched (all chains per launch)",
+                   "nnl_att": tot("nnl_att"), "nnl_rep": tot("nnl_rep"), "nil_att": tot("nil_att"),
+                   "nil_rep": tot("nil_rep"), "records": g["records"]},
+        "plain": None if plain is None else {"value": plain, "unit": "bead-steps/s",
+                                             "form": "plain (one chain per launch)", "steps_per_chain": a.plain_steps},
+        "speedup_vs_plain": None if plain is None else value / plain,
         "roofline": {"bound": "hbm", "kernel": "k_force (fused step)", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "bytes_per_launch": bytes_per_launch,
-                     "avg_launch_us": avg_s * 1e6, "launches": g["step_n"],
-                     "rebuild_avg_us": g["reb_ms"] / max(g["reb_n"], 1) * 1e3},
+                     "traffic": None, "bytes_per_launch": g["bytes_step"], "avg_launch_us": avg_s * 1e6,
+                     "launches": step_n, "rebuild_avg_us": reb_ms / max(reb_n, 1) * 1e3,
+                     "rebuilds": reb_n, "observe_avg_us": obs_ms / max(obs_n, 1) * 1e3},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -71,6 +71,7 @@ typedef struct {
   int nup;                             /* observable/record period (steps) */
   int noise_mode;                      /* SOPG_NOISE_* */
   uint64_t seed;                       /* Philox key (SOPG_NOISE_PHILOX) */
+  int sim_type;                        /* 1 underdamped (sop.cpp:282-329), 2 overdamped (sop.cpp:331-363) */
 } sopg_chain_params;
 
 /* One chain's topology.  Arrays are 0-based with 1-based bead indices.
@@ -131,10 +132,22 @@ int sopg_update_pair_list(sopg_ctx* ctx, int chain, int* nil_att, int* nil_rep);
  * noise: the 3N gaussians this call consumes (bead-major x,y,z), or NULL for
  * the chain's device generator. */
 int sopg_force_eval(sopg_ctx* ctx, int chain, const double* noise);
+/* The individual force terms of the reference's force_term[] table, ADDED to
+ * the chain's current force (each reference term accumulates into force[]):
+ * random_force (energy.cpp:449-465), fene_forces (:402-447),
+ * soft_sphere_angular_forces (:354-400), vdw_forces (:259-352).  Terms run in
+ * the table's order (random, FENE, angular, vdw) within one call. */
+#define SOPG_TERM_RANDOM 1
+#define SOPG_TERM_FENE 2
+#define SOPG_TERM_ANGULAR 4
+#define SOPG_TERM_VDW 8
+int sopg_force_terms(sopg_ctx* ctx, int chain, int terms, const double* noise);
 /* energy_eval()           energy.cpp:6-19 */
 int sopg_energy_eval(sopg_ctx* ctx, int chain, sopg_energies* out);
 /* underdamped_iteration() sop.cpp:282-329 */
 int sopg_underdamped_iteration(sopg_ctx* ctx, int chain, const double* noise);
+/* overdamped_iteration()  sop.cpp:331-363 (chain with sim_type 2) */
+int sopg_overdamped_iteration(sopg_ctx* ctx, int chain, const double* noise);
 /* calculate_observables() sop.cpp:200-280 */
 int sopg_calculate_observables(sopg_ctx* ctx, int chain, sopg_observables* out);
 /* The current NL / PL of a chain in the reference's order (1-based bead ids).

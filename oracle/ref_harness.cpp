@@ -66,6 +66,15 @@ void ref_load(const char* kind, const char* path) {
 void ref_read_input(const char* path) { read_input(path); }
 void ref_ex_cmds() { ex_cmds(); }
 
+// ex_cmds (sop.cpp:55-74) without `run`: the load / set commands of a script.
+void ref_ex_cmds_norun() {
+  for (int i = 1; i <= ncmd; i++) {
+    if (!std::strcmp(cmd[i], "load")) load(i);
+    else if (!std::strcmp(cmd[i], "set")) set_params(i);
+  }
+}
+int ref_ncmd() { return ncmd; }
+
 void ref_set_potential_forces() { set_potential(); set_forces(); }
 void ref_update_neighbor_list() { update_neighbor_list(); }
 void ref_update_cell_list() { update_cell_list(); }

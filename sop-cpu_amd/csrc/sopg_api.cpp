@@ -27,7 +27,8 @@ void launch_posupdate(const Dev& D, int src, int dst, int g0, int g1, hipStream_
 void launch_velupdate(const Dev& D, int g0, int g1, hipStream_t s);
 void launch_observe(const Dev& D, int cur, int par, int rel, int all, int what, int g0, int g1, hipStream_t s);
 void launch_chain_reduce(const Dev& D, int c0, int nc, double* out, hipStream_t s);
-enum { F_STORE = 1, F_VEL = 2, F_ENERGY = 4, F_ENERGY_ALL = 8 };
+enum { F_STORE = 1, F_VEL = 2, F_ENERGY = 4, F_ENERGY_ALL = 8, F_NOISE = 16, F_BOND = 32, F_ANGLE = 64,
+       F_VDW = 128, F_ACCUM = 256, F_ALLTERMS = 16 | 32 | 64 | 128 };
 enum { OBS_PHYS = 1, OBS_COUNTS = 2 };
 }  // namespace sopg
 
@@ -205,6 +206,7 @@ void fill_chainp(const HostChain& h, ChainP& P) {
   P.var = std::sqrt(2.0 * p.T * p.zeta / p.h);  // energy.cpp:456
   P.T = p.T;
   P.seed = p.seed;
+  P.sim_type = p.sim_type == 2 ? 2 : 1;
   P.noise_base = h.noise_base;
 }
 
@@ -474,6 +476,13 @@ int rebuild_one(sopg_ctx* c, int chain, int mode, int* nnl_att, int* nnl_rep) {
   if (nnl_att) *nnl_att = (int)out[R_NLA];
   if (nnl_rep) *nnl_rep = (int)out[R_NLR];
   return SOPG_OK;
+}
+
+// kinT (sop.cpp:262-279): sum v^2 / 3N underdamped; overdamped uses the
+// position increments scaled by zeta / (2h).
+double kin_temp(const sopg_chain_params& p, double sumsq, int n) {
+  if (p.sim_type == 2) sumsq *= p.zeta / (2.0 * p.h);
+  return sumsq / (3.0 * (double)n);
 }
 
 const double* chain_noise(sopg_ctx* c, const ChainP& P, const HostChain& h, const double* noise, int* rc) {
@@ -761,10 +770,36 @@ int sopg_force_eval(sopg_ctx* c, int chain, const double* noise) {
   if ((rc = upload_chainp(c))) return rc;
   const double* dn = chain_noise(c, *P, *h, noise, &rc);
   if (rc) return rc;
-  launch_force(c->lanes, c->D, c->d_T, c->cur, c->par, 0, F_STORE, false, dn, P->off, P->off + P->n, c->stream);
+  launch_force(c->lanes, c->D, c->d_T, c->cur, c->par, 0, F_STORE | F_ALLTERMS, false, dn, P->off, P->off + P->n,
+               c->stream);
   HIP_OK(c, hipGetLastError());
   HIP_OK(c, hipStreamSynchronize(c->stream));
   h->noise_base++;
+  return SOPG_OK;
+}
+
+int sopg_force_terms(sopg_ctx* c, int chain, int terms, const double* noise) {
+  HostChain* h;
+  ChainP* P;
+  int rc = get_chain(c, chain, &h, &P);
+  if (rc) return rc;
+  if (terms & ~(SOPG_TERM_RANDOM | SOPG_TERM_FENE | SOPG_TERM_ANGULAR | SOPG_TERM_VDW))
+    return fail(c, SOPG_ERR_ARG, "unknown force term");
+  if ((rc = upload_chainp(c))) return rc;
+  const double* dn = nullptr;
+  if (terms & SOPG_TERM_RANDOM) {
+    dn = chain_noise(c, *P, *h, noise, &rc);
+    if (rc) return rc;
+  }
+  int flags = F_STORE | F_ACCUM;
+  if (terms & SOPG_TERM_RANDOM) flags |= F_NOISE;
+  if (terms & SOPG_TERM_FENE) flags |= F_BOND;
+  if (terms & SOPG_TERM_ANGULAR) flags |= F_ANGLE;
+  if (terms & SOPG_TERM_VDW) flags |= F_VDW;
+  launch_force(c->lanes, c->D, c->d_T, c->cur, c->par, 0, flags, false, dn, P->off, P->off + P->n, c->stream);
+  HIP_OK(c, hipGetLastError());
+  HIP_OK(c, hipStreamSynchronize(c->stream));
+  if (terms & SOPG_TERM_RANDOM) h->noise_base++;
   return SOPG_OK;
 }
 
@@ -774,7 +809,8 @@ int sopg_energy_eval(sopg_ctx* c, int chain, sopg_energies* out) {
   int rc = get_chain(c, chain, &h, &P);
   if (rc) return rc;
   if ((rc = upload_chainp(c))) return rc;
-  launch_force(c->lanes, c->D, c->d_T, c->cur, c->par, 0, F_ENERGY_ALL, false, nullptr, P->off, P->off + P->n,
+  launch_force(c->lanes, c->D, c->d_T, c->cur, c->par, 0, F_ENERGY_ALL | F_ALLTERMS, false, nullptr, P->off,
+               P->off + P->n,
                c->stream);
   HIP_OK(c, hipGetLastError());
   double r[kRed];
@@ -799,13 +835,23 @@ int sopg_underdamped_iteration(sopg_ctx* c, int chain, const double* noise) {
   const double* dn = chain_noise(c, *P, *h, noise, &rc);
   if (rc) return rc;
   launch_posupdate(c->D, c->cur, c->cur, P->off, P->off + P->n, c->stream);
-  launch_force(c->lanes, c->D, c->d_T, c->cur, c->par, 0, F_STORE | F_VEL, false, dn, P->off, P->off + P->n,
+  launch_force(c->lanes, c->D, c->d_T, c->cur, c->par, 0, F_STORE | F_VEL | F_ALLTERMS, false, dn, P->off,
+               P->off + P->n,
                c->stream);
   HIP_OK(c, hipGetLastError());
   HIP_OK(c, hipStreamSynchronize(c->stream));
   h->noise_base++;
   c->pl_fresh = false;
   return SOPG_OK;
+}
+
+int sopg_overdamped_iteration(sopg_ctx* c, int chain, const double* noise) {
+  HostChain* h;
+  ChainP* P;
+  int rc = get_chain(c, chain, &h, &P);
+  if (rc) return rc;
+  if (h->p.sim_type != 2) return fail(c, SOPG_ERR_ARG, "chain is not overdamped (sim_type 2)");
+  return sopg_underdamped_iteration(c, chain, noise);
 }
 
 int sopg_calculate_observables(sopg_ctx* c, int chain, sopg_observables* out) {
@@ -821,7 +867,7 @@ int sopg_calculate_observables(sopg_ctx* c, int chain, sopg_observables* out) {
     out->contct_nat = (int)r[R_CONT];
     out->Q = (double)out->contct_nat / (double)P->natt;                // sop.cpp:240
     out->rgsq = r[R_RG] / ((double)P->n * (double)P->n);              // sop.cpp:258
-    out->kinT = r[R_V2] / (3.0 * (double)P->n);                       // sop.cpp:269
+    out->kinT = kin_temp(h->p, r[R_V2], P->n);
   }
   return SOPG_OK;
 }
@@ -883,8 +929,11 @@ int sopg_set_step(sopg_ctx* c, int chain, double istep, int inlup, int iup) {
   if (!c) return SOPG_ERR_ARG;
   if (chain < 0 || chain >= (int)c->chains.size()) return fail(c, SOPG_ERR_ARG, "bad chain id");
   HostChain& h = c->chains[chain];
+  // inlup = 0 (overdamped_ctrl starts from the global inlup = 0, sop.cpp:428)
+  // rebuilds on the first step, as inlup = nnlup does
+  if (inlup == 0) inlup = h.p.nnlup;
   if (inlup < 1 || inlup > h.p.nnlup || iup < 1 || iup > h.p.nup)
-    return fail(c, SOPG_ERR_ARG, "inlup must be in [1, nnlup] and iup in [1, nup]");
+    return fail(c, SOPG_ERR_ARG, "inlup must be in [0, nnlup] and iup in [1, nup]");
   h.istep = istep;
   h.inlup = inlup;
   h.iup = iup;
@@ -989,7 +1038,7 @@ int sopg_run(sopg_ctx* c, long nsteps, const double* noise, sopg_record* rec, lo
     const bool last = (r == nsteps - 1);
     const double* nz = noise ? c->d_noise + (size_t)r * 3 * nb : nullptr;
     pe = prof_begin(c);
-    launch_force(L, D, c->d_T, c->cur, c->par, (int)r, F_STORE | F_VEL | F_ENERGY, !last, nz, 0, nb, s);
+    launch_force(L, D, c->d_T, c->cur, c->par, (int)r, F_STORE | F_VEL | F_ENERGY | F_ALLTERMS, !last, nz, 0, nb, s);
     prof_end(c, SOPG_KERNEL_STEP, pe);
     if (records(r)) {
       pe = prof_begin(c);
@@ -1031,7 +1080,7 @@ int sopg_run(sopg_ctx* c, long nsteps, const double* noise, sopg_record* rec, lo
         o.chain = ic;
         o.istep = h.istep + (double)r;
         o.T = h.p.T;
-        o.kinT = v[R_V2] / (3.0 * (double)P.n);
+        o.kinT = kin_temp(h.p, v[R_V2], P.n);
         o.e_bnd = v[R_EBND] * -c->T.e_bnd_coeff;
         o.e_ang_ss = v[R_EANG];
         o.e_vdw_rr = v[R_EATT] + v[R_EREP];

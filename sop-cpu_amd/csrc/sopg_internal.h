@@ -39,6 +39,7 @@ struct ChainP {
   double a1, a2, a3, a4, a5, var, T;
   double thr_att;      // tol * max lj_nat_pdb_dist2 of the chain
   uint64_t seed, noise_base;  // Philox key, force evaluations done so far
+  int sim_type;        // 1 underdamped, 2 overdamped
 };
 
 // Per-bead reduction fields (red[g * kRed + field]).

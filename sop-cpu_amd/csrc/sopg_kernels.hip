@@ -271,9 +271,11 @@ __global__ void k_posupdate(Dev D, int src, int dst, int g0, int g1) {
   const int g = g0 + blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= g1) return;
   const ChainP& P = D.cp[D.chain[g]];
-  const double ix = P.a1 * D.vx[g] + P.a2 * D.fx[g];
-  const double iy = P.a1 * D.vy[g] + P.a2 * D.fy[g];
-  const double iz = P.a1 * D.vz[g] + P.a2 * D.fz[g];
+  // underdamped: incr = a1 v + a2 F (sop.cpp:292-294); overdamped: incr = a5 F (sop.cpp:339-341)
+  const bool od = P.sim_type == 2;
+  const double ix = od ? P.a5 * D.fx[g] : P.a1 * D.vx[g] + P.a2 * D.fx[g];
+  const double iy = od ? P.a5 * D.fy[g] : P.a1 * D.vy[g] + P.a2 * D.fy[g];
+  const double iz = od ? P.a5 * D.fz[g] : P.a1 * D.vz[g] + P.a2 * D.fz[g];
   D.ix[g] = ix; D.iy[g] = iy; D.iz[g] = iz;
   (dst ? D.px[1] : D.px[0])[g] = mimg((src ? D.px[1] : D.px[0])[g] + ix, P.boxl, P.qbox);
   (dst ? D.py[1] : D.py[0])[g] = mimg((src ? D.py[1] : D.py[0])[g] + iy, P.boxl, P.qbox);
@@ -288,7 +290,7 @@ __global__ void k_velupdate(Dev D, int g0, int g1) {
   const int g = g0 + blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= g1) return;
   const ChainP& P = D.cp[D.chain[g]];
-  if (!P.vel_on) return;
+  if (!P.vel_on || P.sim_type == 2) return;
   D.vx[g] = P.a3 * D.ix[g] + P.a4 * D.fx[g];
   D.vy[g] = P.a3 * D.iy[g] + P.a4 * D.fy[g];
   D.vz[g] = P.a3 * D.iz[g] + P.a4 * D.fz[g];
@@ -302,7 +304,11 @@ __global__ void k_velupdate(Dev D, int g0, int g1) {
 //                  falls on this step (all chains if F_ENERGY_ALL)
 // NEXT:  also the next step's pair list (parity pr^1) from these positions
 //        and the next position update into buffer cur^1.
-enum { F_STORE = 1, F_VEL = 2, F_ENERGY = 4, F_ENERGY_ALL = 8 };
+//        F_NOISE / F_BOND / F_ANGLE / F_VDW  which force terms to add (the
+//                  reference's force_term[] table, energy.cpp:78-113)
+//        F_ACCUM   add to the stored force instead of starting from zero
+enum { F_STORE = 1, F_VEL = 2, F_ENERGY = 4, F_ENERGY_ALL = 8, F_NOISE = 16, F_BOND = 32, F_ANGLE = 64,
+       F_VDW = 128, F_ACCUM = 256 };
 
 template <int L, bool NEXT>
 __global__ void __launch_bounds__(256) k_force(Dev D, const Tables* __restrict__ T, int cur, int pr, int rel, int flags,
@@ -320,10 +326,12 @@ __global__ void __launch_bounds__(256) k_force(Dev D, const Tables* __restrict__
   const bool energy = (flags & F_ENERGY_ALL) || ((flags & F_ENERGY) && ((P.iup0 + rel) % P.nup == 0));
   double eb = 0.0, ea = 0.0, eatt = 0.0, erep = 0.0;
 
-  // random force (energy.cpp:449-465): force = 0 + var * gasdev(), x, y, z
+  // clear_forces, then random_force (energy.cpp:35-45, 449-465):
+  // force = 0 + var * gasdev() for x, y, z
   double gx = 0.0, gy = 0.0, gz = 0.0;
-  if (!(flags & F_STORE)) {
-    // energy only: no random force
+  const bool with_noise = (flags & F_STORE) && (flags & F_NOISE);
+  if (!with_noise) {
+    // energy only, or the random term not requested
   } else if (P.noise_mode == SOPG_NOISE_REPLAY) {
     gx = noise[3 * (size_t)g];
     gy = noise[3 * (size_t)g + 1];
@@ -343,12 +351,17 @@ __global__ void __launch_bounds__(256) k_force(Dev D, const Tables* __restrict__
       gz = __shfl(a, 1, L);
     }
   }
-  double Fx = 0.0 + P.var * gx;
-  double Fy = 0.0 + P.var * gy;
-  double Fz = 0.0 + P.var * gz;
+  double Fx = (flags & F_ACCUM) ? D.fx[g] : 0.0;
+  double Fy = (flags & F_ACCUM) ? D.fy[g] : 0.0;
+  double Fz = (flags & F_ACCUM) ? D.fz[g] : 0.0;
+  if (with_noise) {
+    Fx += P.var * gx;
+    Fy += P.var * gy;
+    Fz += P.var * gz;
+  }
 
   // FENE bonds (energy.cpp:402-447): F_i -= f, F_j += f
-  {
+  if (flags & F_BOND) {
     const int p0 = D.bond_ptr[g], p1 = D.bond_ptr[g + 1];
     for (int base = p0; base < p1; base += L) {
       const int q = base + l;
@@ -372,7 +385,7 @@ __global__ void __launch_bounds__(256) k_force(Dev D, const Tables* __restrict__
     }
   }
   // soft-sphere (i, i+2) repulsion (energy.cpp:354-400): F_i -= f, F_k += f
-  {
+  if (flags & F_ANGLE) {
     const int p0 = D.ang_ptr[g], p1 = D.ang_ptr[g + 1];
     for (int base = p0; base < p1; base += L) {
       const int q = base + l;
@@ -395,7 +408,7 @@ __global__ void __launch_bounds__(256) k_force(Dev D, const Tables* __restrict__
     }
   }
   // native LJ 12-6 over the pair list (energy.cpp:274-311): F_i += f, F_j -= f
-  {
+  if (flags & F_VDW) {
     const int p0 = D.att_ptr[g], p1 = D.att_ptr[g + 1];
     uint8_t* aplw = pw ? D.am_pl1 : D.am_pl0;
     for (int base = p0; base < p1; base += L) {
@@ -440,7 +453,7 @@ __global__ void __launch_bounds__(256) k_force(Dev D, const Tables* __restrict__
     }
   }
   // non-native repulsion over the pair list (energy.cpp:313-350): F_i += f, F_j -= f
-  {
+  if (flags & F_VDW) {
     const int cnt = D.cnt[g];
     uint32_t* S = D.slot + (size_t)g * D.cap;
     const uint32_t rbit = 1u << (30 + pr), wbit = 1u << (30 + pw);
@@ -489,7 +502,7 @@ __global__ void __launch_bounds__(256) k_force(Dev D, const Tables* __restrict__
   if (!(flags & F_STORE)) return;
   D.fx[g] = Fx; D.fy[g] = Fy; D.fz[g] = Fz;
   double vx = D.vx[g], vy = D.vy[g], vz = D.vz[g];
-  if ((flags & F_VEL) && P.vel_on) {
+  if ((flags & F_VEL) && P.vel_on && P.sim_type != 2) {
     vx = P.a3 * D.ix[g] + P.a4 * Fx;
     vy = P.a3 * D.iy[g] + P.a4 * Fy;
     vz = P.a3 * D.iz[g] + P.a4 * Fz;
@@ -497,9 +510,10 @@ __global__ void __launch_bounds__(256) k_force(Dev D, const Tables* __restrict__
   }
   if constexpr (NEXT) {
     // the next step's position half (sop.cpp:288-310) into buffer cur^1
-    const double ix = P.a1 * vx + P.a2 * Fx;
-    const double iy = P.a1 * vy + P.a2 * Fy;
-    const double iz = P.a1 * vz + P.a2 * Fz;
+    const bool od = P.sim_type == 2;
+    const double ix = od ? P.a5 * Fx : P.a1 * vx + P.a2 * Fx;
+    const double iy = od ? P.a5 * Fy : P.a1 * vy + P.a2 * Fy;
+    const double iz = od ? P.a5 * Fz : P.a1 * vz + P.a2 * Fz;
     D.ix[g] = ix; D.iy[g] = iy; D.iz[g] = iz;
     double* pxs = cur ? D.px[1] : D.px[0];
     double* pys = cur ? D.py[1] : D.py[0];
@@ -566,7 +580,11 @@ __global__ void k_observe(Dev D, int cur, int par, int rel, int all, int what, i
     const double dz = mimg(uz[g] - uz[h], boxl, qbox);
     rg += (dx * dx + dy * dy + dz * dz);
   }
-  const double v2 = D.vx[g] * D.vx[g] + D.vy[g] * D.vy[g] + D.vz[g] * D.vz[g];
+  // kinT source: velocities (underdamped, sop.cpp:263-269) or position
+  // increments (overdamped, sop.cpp:271-278; scaled by zeta/(2h) on the host)
+  const bool od = P.sim_type == 2;
+  const double qx = od ? D.ix[g] : D.vx[g], qy = od ? D.iy[g] : D.vy[g], qz = od ? D.iz[g] : D.vz[g];
+  const double v2 = qx * qx + qy * qy + qz * qz;
   r[R_CONT] = cont; r[R_RG] = rg; r[R_V2] = v2;
 }
 

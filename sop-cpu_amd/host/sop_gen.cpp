@@ -136,14 +136,28 @@ bool collapse(int n, Rng& rng, double R, std::vector<double>& X, long max_sweeps
   for (long sweep = 0; sweep < max_sweeps && E > 0.0; sweep++) {
     for (int m = 0; m < n; m++) {
       double mv = rng.uni();
-      if (mv < 0.02 && n > 3) {
-        // pivot: rotate the shorter side of bead k about k by a random rotation
-        int k = 1 + (int)(rng.uni() * (n - 2));
-        bool left = k < n / 2;
-        int lo = left ? 0 : k + 1, hi = left ? k - 1 : n - 1;
-        double ax[3];
-        rng.unit(ax);
-        double ang = (rng.uni() - 0.5) * 1.2;
+      if (mv < 0.1 && n > 3) {
+        // segment move (bond lengths kept): rotate beads k+1..m-1 about the
+        // k--m axis (a long crankshaft), or a short end segment about bead k
+        const int len = 2 + (int)(rng.uni() * 31);
+        int k = (int)(rng.uni() * n), m = k + len + 1, lo, hi;
+        double ax[3], ang;
+        bool end = false;
+        if (m < n) {
+          lo = k + 1; hi = m - 1;
+          for (int a = 0; a < 3; a++) ax[a] = X[3 * m + a] - X[3 * k + a];
+          double l = norm3(ax);
+          if (l < 1e-9) continue;
+          for (int a = 0; a < 3; a++) ax[a] /= l;
+          ang = (rng.uni() - 0.5) * 6.283185307179586;
+        } else {
+          end = true;
+          if (rng.uni() < 0.5) { k = n - 1 - len; lo = k + 1; hi = n - 1; }
+          else { k = len; lo = 0; hi = k - 1; }
+          if (k < 1 || k > n - 2) continue;
+          rng.unit(ax);
+          ang = (rng.uni() - 0.5) * 1.2;
+        }
         Y.resize(3 * (size_t)(hi - lo + 1));
         double dE = 0.0;
         for (int i = lo; i <= hi; i++) {
@@ -154,9 +168,10 @@ bool collapse(int n, Rng& rng, double R, std::vector<double>& X, long max_sweeps
         bool bad = false;
         for (int i = lo; i <= hi && !bad; i++)
           bad = g.clash(&Y[3 * (i - lo)], X, [&](int j) {
-            bool fixed = left ? (j >= k) : (j <= k);
+            bool fixed = j < lo || j > hi;
             return fixed && std::abs(j - i) >= 2;
           });
+        (void)end;
         if (bad) continue;
         for (int i = lo; i <= hi; i++) {
           g.remove(i, &X[3 * i]);

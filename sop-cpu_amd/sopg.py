@@ -16,6 +16,7 @@ OK = 0
 CUTOFF_NONE, CUTOFF_NEIGHBOR, CUTOFF_CELL = 0, 1, 2
 NOISE_REPLAY, NOISE_PHILOX = 0, 1
 LIST_NL_ATT, LIST_NL_REP, LIST_PL_ATT, LIST_PL_REP = 0, 1, 2, 3
+KERNEL_STEP, KERNEL_REBUILD, KERNEL_PAIRLIST, KERNEL_OBSERVE = 0, 1, 2, 3
 
 _D33 = (C.c_double * 3) * 3
 
@@ -32,7 +33,7 @@ class ChainParams(C.Structure):
                 ("a1", C.c_double), ("a2", C.c_double), ("a3", C.c_double), ("a4", C.c_double),
                 ("a5", C.c_double), ("boxl", C.c_double), ("ncell", C.c_double), ("lcell", C.c_double),
                 ("cutoff", C.c_int), ("nnlup", C.c_int), ("nup", C.c_int), ("noise_mode", C.c_int),
-                ("seed", C.c_uint64)]
+                ("seed", C.c_uint64), ("sim_type", C.c_int)]
 
 
 _IP = C.POINTER(C.c_int)
@@ -70,7 +71,7 @@ EXPORTS = [
     "sopg_set_lanes", "sopg_set_capacity", "sopg_add_chain", "sopg_set_chain_params", "sopg_put_state",
     "sopg_get_state", "sopg_nchains", "sopg_chain_nbead", "sopg_update_neighbor_list",
     "sopg_update_cell_list", "sopg_update_pair_list", "sopg_force_eval", "sopg_energy_eval",
-    "sopg_underdamped_iteration", "sopg_calculate_observables", "sopg_get_list", "sopg_run",
+    "sopg_underdamped_iteration", "sopg_overdamped_iteration", "sopg_calculate_observables", "sopg_get_list", "sopg_run",
     "sopg_set_step", "sopg_sync", "sopg_list_counts", "sopg_profile", "sopg_profile_read",
     "sopgen_saw", "sopgen_native", "sopgen_maxwell", "sopgen_check", "sopgen_write_topology",
     "sopgen_write_xyz",
@@ -225,7 +226,7 @@ def underdamped_coeffs(h=0.05, zeta=50.0):
 
 
 def chain_params(T=0.6, zeta=50.0, h=0.05, boxl=500.0, ncell=55.0, lcell=None, cutoff=CUTOFF_NEIGHBOR,
-                 nnlup=50, nup=1000, noise_mode=NOISE_PHILOX, seed=1, coeffs=None):
+                 nnlup=50, nup=1000, noise_mode=NOISE_PHILOX, seed=1, coeffs=None, sim_type=1):
     a = coeffs if coeffs is not None else underdamped_coeffs(h, zeta)
     p = ChainParams()
     p.T, p.zeta, p.h = T, zeta, h
@@ -233,6 +234,7 @@ def chain_params(T=0.6, zeta=50.0, h=0.05, boxl=500.0, ncell=55.0, lcell=None, c
     p.boxl, p.ncell = boxl, ncell
     p.lcell = boxl / ncell if lcell is None else lcell
     p.cutoff, p.nnlup, p.nup, p.noise_mode, p.seed = cutoff, nnlup, nup, noise_mode, seed
+    p.sim_type = sim_type
     return p
 
 
