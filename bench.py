@@ -10,6 +10,12 @@ every 1000 steps, device Philox noise.  Other workloads: --workload plain3000
 (one N=3000 chain, 12 A cell list, rebuild every 20 steps) and replicas1024
 (1024 trajectories of one N=76 chain, rebuild every 50 steps).
 
+serve: a Poisson stream of jobs (20 jobs/s; N log-uniform in [50, 10000];
+steps log-uniform in [1e4, 1e6] scaled by --serve-step-scale so the stream
+finishes in seconds; kT uniform in [0.5, 1.3]) replayed in real time against
+the continuous-batching server (include/sopserve.h); p50/p99 job latency =
+finish - arrival, throughput = job bead-steps / (last finish - first arrival).
+
 One step = the reference's underdamped_ctrl loop body (sop.cpp:153-189) for
 every chain.  value = bead-steps/s = sum(N) * K / wall seconds of exactly K
 timed steps of the batched form, inputs resident in HBM.  plain = the same
@@ -147,6 +153,65 @@ def gpu_plain(chains, w, steps, warmup, topo_cache, lanes=0, list_lanes=0):
     return total_bs / total_t, total_t
 
 
+def serve_jobs(njobs, step_scale, threads, seed=5, rate=20.0):
+    """The serving stream: Poisson arrivals (rate jobs/s), N log-uniform in
+    [50, 10000], steps log-uniform in [1e4, 1e6] times step_scale, kT uniform
+    in [0.5, 1.3]; each job its own native structure and Maxwell velocities."""
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, njobs))
+    sizes = np.round(np.exp(rng.uniform(np.log(50), np.log(10000), njobs))).astype(int)
+    steps = np.maximum(1, np.round(np.exp(rng.uniform(np.log(1e4), np.log(1e6), njobs)) * step_scale)).astype(int)
+    kts = rng.uniform(0.5, 1.3, njobs)
+
+    def one(j):
+        n = int(sizes[j])
+        x = sopg.gen_native(n, 20_000 + j)
+        return x, sopg.gen_maxwell(n, float(kts[j]), 20_000 + j), sopg.chain_topology(x)
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        built = list(ex.map(one, range(njobs)))
+    return [dict(arrival=float(arrivals[j]), n=int(sizes[j]), steps=int(steps[j]), kT=float(kts[j]), x=built[j][0],
+                 v=built[j][1], topo=built[j][2], seed=20_000 + j) for j in range(njobs)]
+
+
+def gpu_serve(jobs, w, hbm_budget, max_chains, chunk, lanes):
+    """Replay the arrival stream against the continuous-batching server in real
+    time; latency = finish - arrival of each job."""
+    srv = sopg.Server(hbm_budget=hbm_budget, max_chains=max_chains, chunk=chunk, lanes=lanes)
+    t0 = srv.now()
+    i, active, queued = 0, 0, 0
+    ids = []
+    peak_active = 0
+    while i < len(jobs) or active or queued:
+        now = srv.now() - t0
+        while i < len(jobs) and jobs[i]["arrival"] <= now:
+            jb = jobs[i]
+            p = sopg.chain_params(T=jb["kT"], boxl=w["boxl"], ncell=w["ncell"], cutoff=w["cutoff"],
+                                  nnlup=w["nnlup"], nup=NUP, noise_mode=sopg.NOISE_PHILOX, seed=jb["seed"])
+            ids.append(srv.submit(jb["topo"], p, jb["x"], jb["v"], jb["steps"], arrival=t0 + jb["arrival"]))
+            queued += 1
+            i += 1
+        if active == 0 and queued == 0:
+            time.sleep(max(0.0, min(0.01, jobs[i]["arrival"] - (srv.now() - t0))))
+            continue
+        active, queued = srv.poll()
+        peak_active = max(peak_active, active)
+    info = [srv.job(j) for j in ids]
+    st = srv.stats()
+    nrec = sum(len(srv.records(j)) for j in ids)
+    srv.close()
+    done = [k for k in info if k["state"] == sopg.JOB_DONE]
+    lat = np.array([k["t_finish"] - k["t_arrival"] for k in done])
+    wait = np.array([k["t_admit"] - k["t_arrival"] for k in done])
+    span = max(k["t_finish"] for k in done) - (t0 + jobs[0]["arrival"])
+    work = sum(k["nbead"] * k["nsteps"] for k in done)
+    return dict(value=work / span, unit="bead-steps/s", jobs=len(jobs), done=len(done),
+                rejected=sum(k["state"] == sopg.JOB_REJECTED for k in info),
+                p50_latency_s=float(np.percentile(lat, 50)), p99_latency_s=float(np.percentile(lat, 99)),
+                p50_wait_s=float(np.percentile(wait, 50)), p99_wait_s=float(np.percentile(wait, 99)),
+                makespan_s=span, rounds=st["rounds"], busy_s=st["busy_s"], peak_resident=peak_active,
+                records=nrec)
+
+
 def cpu_baseline(chains, w, budget_s=20.0, max_n=3200):
     """The reference on a size-stratified sample of the workload's chains, one core."""
     from refs import Ref, setup_system, std_cmds
@@ -207,6 +272,15 @@ def main():
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=100)
+    ap.add_argument("--serve-step-scale", type=float, default=0.01,
+                    help="job step counts are log-uniform in [1e4, 1e6] times this factor")
+    ap.add_argument("--serve-budget-gib", type=float, default=8.0)
+    ap.add_argument("--serve-chunk", type=int, default=1000)
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="after the run, copy this many MiB of FP64 device memory three times with a plain "
+                         "elementwise kernel (the known-size reference for HBM counter scaling)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (independent chains shard across GPUs with no "
@@ -231,6 +305,18 @@ def main():
     obs_ms, obs_n = g["prof"][sopg.KERNEL_OBSERVE]
     avg_s = step_ms / max(step_n, 1) / 1e3
     achieved = g["bytes_step"] / avg_s / 1e9
+    serve = None
+    if not a.no_serve:
+        t1 = time.perf_counter()
+        jobs = serve_jobs(a.serve_jobs, a.serve_step_scale, a.threads)
+        print(f"[bench] generated {len(jobs)} jobs in {time.perf_counter() - t1:.1f} s", file=sys.stderr, flush=True)
+        serve = gpu_serve(jobs, WORKLOADS["mixed256"], a.serve_budget_gib * 2**30, 1024, a.serve_chunk, a.lanes)
+        serve.update(stream=f"Poisson arrivals 20 jobs/s, {a.serve_jobs} jobs (seed 5); N log-uniform [50, 10000]; "
+                            f"steps log-uniform [1e4, 1e6] x {a.serve_step_scale:g}; kT uniform [0.5, 1.3]; "
+                            f"neighbor list every 20 steps; Q/Rg/energy record every {NUP} steps",
+                     hbm_budget_gib=a.serve_budget_gib, chunk=a.serve_chunk)
+        print(f"[bench] serve {serve['value']:.4g} bead-steps/s p50 {serve['p50_latency_s']:.3f} s "
+              f"p99 {serve['p99_latency_s']:.3f} s", file=sys.stderr, flush=True)
     cpu = None if a.no_cpu else cpu_baseline(chains, w)
     tot = lambda k: sum(c[k] for c in g["counts"])
     line = {
@@ -253,6 +339,9 @@ def main():
         "plain": None if plain is None else {"value": plain, "unit": "bead-steps/s",
                                              "form": "plain (one chain per launch)", "steps_per_chain": a.plain_steps},
         "speedup_vs_plain": None if plain is None else value / plain,
+        "serve": serve,
+        "p50_latency_s": None if serve is None else serve["p50_latency_s"],
+        "p99_latency_s": None if serve is None else serve["p99_latency_s"],
         "roofline": {"bound": "hbm", "kernel": "k_force (fused step)", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "bytes_per_launch": g["bytes_step"], "avg_launch_us": avg_s * 1e6,
@@ -260,7 +349,16 @@ def main():
                      "rebuilds": reb_n, "observe_avg_us": obs_ms / max(obs_n, 1) * 1e3},
         "cpu_baseline": cpu,
     }
-    print(json.dumps(line))
+    print(json.dumps(line), flush=True)
+    if a.calib_copy_mib:
+        import torch
+        n = a.calib_copy_mib * 2**20 // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)  # reads and writes calib_copy_mib MiB each
+        torch.cuda.synchronize()
+        print(f"[bench] calibration copy: 3 x {a.calib_copy_mib} MiB read + written", file=sys.stderr, flush=True)
 
 
 if __name__ == "__main__":

@@ -107,12 +107,29 @@ int sopg_set_tables(sopg_ctx* ctx, const sopg_tables* t);
  * step (force) kernels and in the list-rebuild kernels.  Results do not depend
  * on either. */
 int sopg_set_lanes(sopg_ctx* ctx, int step_lanes, int list_lanes);
-/* Per-bead capacity of the non-native neighbor list (0 = automatic). */
+int sopg_get_lanes(const sopg_ctx* ctx, int* step_lanes, int* list_lanes);
+/* Time the step kernel at every lane count on the context's current chains
+ * and keep the fastest (no counterpart in the reference: the CPU loop has no
+ * launch shape).  *step_lanes = the choice. */
+int sopg_autotune(sopg_ctx* ctx, int* step_lanes);
+/* enable = 0 disables the binned (cell-grid) list rebuild and always scans
+ * every candidate; the lists are identical either way (default: enabled). */
+int sopg_set_binned(sopg_ctx* ctx, int enable);
+/* Per-bead capacity of the non-native neighbor list (0 = automatic: 1024, or
+ * the largest no-cutoff chain).  Fixed at the first device operation. */
 int sopg_set_capacity(sopg_ctx* ctx, int cap);
+/* Pre-size the device pools for `beads` beads in `chains` chains (optional:
+ * pools grow on demand, but growing copies every array once). */
+int sopg_reserve(sopg_ctx* ctx, long beads, int chains);
 
 /* ---------------------------------------------------------------- chains */
-/* Topology copy of the loaders (params.cpp:110-260); returns chain id in *id. */
+/* Topology copy of the loaders (params.cpp:110-260); returns chain id in *id
+ * (the lowest free id).  A chain may join a context at any time, also between
+ * sopg_run calls: its beads start at zero (set them with sopg_put_state). */
 int sopg_add_chain(sopg_ctx* ctx, const sopg_topology* topo, const sopg_chain_params* p, int* id);
+/* Release a chain's device memory and id (continuous batching: a finished
+ * chain leaves, a queued one joins; the other chains are not touched). */
+int sopg_remove_chain(sopg_ctx* ctx, int chain);
 int sopg_set_chain_params(sopg_ctx* ctx, int chain, const sopg_chain_params* p);
 /* State of one chain: pos, unc_pos, vel, force as AoS 3N doubles (any may be NULL). */
 int sopg_put_state(sopg_ctx* ctx, int chain, const double* pos, const double* unc_pos,
@@ -159,12 +176,14 @@ int sopg_get_list(sopg_ctx* ctx, int chain, int which, int* ibead, int* jbead, l
  * loop body (sop.cpp:153-189): list rebuild every nnlup steps, pair list every
  * step, underdamped_iteration, and an energy + observables record every nup
  * steps.  Step counters continue across calls (sopg_set_step).  noise: NULL
- * (device generator) or nsteps*3*Ntot gaussians, step-major, beads in chain
- * order.  Records go to rec (up to rec_cap); *nrec = count produced. */
+ * (device generator) or nsteps*3*Ntot gaussians, step-major; within a step the
+ * chains in ascending id order, 3N each, bead-major.  Records go to rec (up to
+ * rec_cap), in step order and ascending chain id; *nrec = count produced. */
 int sopg_run(sopg_ctx* ctx, long nsteps, const double* noise, sopg_record* rec, long rec_cap, long* nrec);
 /* Set the step counters of a chain: istep of the next step, and the
  * reference's inlup / iup counters as they stand before that step. */
 int sopg_set_step(sopg_ctx* ctx, int chain, double istep, int inlup, int iup);
+int sopg_get_step(const sopg_ctx* ctx, int chain, double* istep, int* inlup, int* iup);
 /* Block until all work queued on the context's stream has finished. */
 int sopg_sync(sopg_ctx* ctx);
 

@@ -27,6 +27,9 @@ int sopgen_check(int n, const double* xyz, double* min_nonbonded, double* max_bo
  * any path may be NULL.  Returns the number of native pairs (r < rcut_nat). */
 int sopgen_write_topology(int n, const double* xyz, const char* bonds_path, const char* angles_path,
                           const char* vdw_path, const char* init_path, double rcut_nat);
+/* Native contacts (pairs j-i>2 closer than rcut) in (i,j) order, 1-based, with
+ * the distances the vdw file carries.  Returns the count (writes at most cap). */
+long sopgen_natives(int n, const double* xyz, double rcut, int* out_i, int* out_j, double* out_r, long cap);
 /* "idx x y z" lines (coordinates or velocities) at %.17g. */
 int sopgen_write_xyz(int n, const double* xyz, const char* path);
 

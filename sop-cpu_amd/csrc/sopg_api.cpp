@@ -1,11 +1,18 @@
 // sopg_api.cpp -- C ABI of the gfx950 SOP hot path (declared in include/sopg.h).
 //
 // Host side of a context: validates and copies topologies, builds the per-bead
-// incidence lists (CSR, in the reference's list order), owns device memory,
-// and sequences the kernels of sopg_kernels.hip on one HIP stream.
+// incidence lists (in the reference's list order), owns device memory, and
+// sequences the kernels of sopg_kernels.hip on one HIP stream.
+//
+// Device memory is a set of pools (beads; bond, angle, native, non-native
+// master entries and their per-bead incidences; chain slots).  Each pool is a
+// first-fit range allocator whose device arrays grow on demand, so chains can
+// join and leave a running context (continuous batching) without touching the
+// other chains' memory; a free bead carries chain id -1 and is skipped.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -19,7 +26,7 @@
 namespace sopg {
 void launch_cellcoords(const Dev& D, int cur, int g0, int g1, hipStream_t s);
 void launch_rebuild(int L, const Dev& D, const Tables* T, int cur, int par, int rel, int mode, int g0, int g1,
-                    hipStream_t s);
+                    int c0, int nc, int binned, hipStream_t s);
 void launch_pairfilter(int L, const Dev& D, const Tables* T, int cur, int par, int g0, int g1, hipStream_t s);
 void launch_force(int L, const Dev& D, const Tables* T, int cur, int pr, int rel, int flags, bool next,
                   const double* noise, int g0, int g1, hipStream_t s);
@@ -36,10 +43,56 @@ using namespace sopg;
 
 namespace {
 
+// First-fit allocator of index ranges in [0, cap).
+struct Pool {
+  size_t cap = 0, hw = 0;
+  std::vector<std::pair<size_t, size_t>> fr;  // free [begin, end), sorted, merged
+  bool take(size_t n, size_t& at) {
+    if (n == 0) { at = 0; return true; }
+    for (size_t k = 0; k < fr.size(); k++)
+      if (fr[k].second - fr[k].first >= n) {
+        at = fr[k].first;
+        fr[k].first += n;
+        if (fr[k].first == fr[k].second) fr.erase(fr.begin() + (long)k);
+        hw = std::max(hw, at + n);
+        return true;
+      }
+    return false;
+  }
+  void give(size_t at, size_t n) {
+    if (n == 0) return;
+    fr.push_back({at, at + n});
+    merge();
+  }
+  void extend(size_t ncap) {
+    if (ncap > cap) fr.push_back({cap, ncap});
+    cap = ncap;
+    merge();
+  }
+  void merge() {
+    std::sort(fr.begin(), fr.end());
+    std::vector<std::pair<size_t, size_t>> m;
+    for (auto& r : fr) {
+      if (!m.empty() && m.back().second >= r.first) m.back().second = std::max(m.back().second, r.second);
+      else m.push_back(r);
+    }
+    fr.swap(m);
+  }
+};
+
+// A device array living in a pool: per pool unit it holds `per` elements.
+struct Arr {
+  void** p;
+  size_t esz, per;
+};
+
+enum { P_BEAD = 0, P_BOND, P_BINC, P_ANG, P_AINC, P_ATT, P_TINC, P_REP, P_RINC, P_CHAIN, NPOOL };
+
 struct HostChain {
+  bool active = false;
   sopg_chain_params p{};
   int n = 0;
-  std::vector<int> bi, bj;  // 0-based
+  std::vector<int> bi, bj;  // 0-based, chain-local
   std::vector<double> br0;
   std::vector<int> gi, gk;
   std::vector<int> ai, aj, ait, ajt;
@@ -50,6 +103,12 @@ struct HostChain {
   double istep = 1.0;
   int inlup = 1, iup = 1;
   uint64_t noise_base = 0;
+  bool on_device = false;
+  // the pair list of parity `par` was built at the current positions (an
+  // explicit list update since the last position change): sopg_run may skip
+  // its first pair-list filter for this chain
+  bool pl_fresh = false;
+  size_t at[NPOOL] = {0};  // range starts in each pool
 };
 
 }  // namespace
@@ -61,18 +120,20 @@ struct sopg_ctx {
   sopg_tables tab{};
   Tables T{};
   int lanes = 16;       // lanes per bead, step / force kernels
-  int list_lanes = 64;  // lanes per bead, list rebuild
+  int list_lanes = 64;  // lanes per bead, full-scan list rebuild
+  int binned = 1;       // binned list rebuild where a chain's extent allows it
   int cap_user = 0;
+  size_t reserve_beads = 0, reserve_chains = 0;
   Tables* d_T = nullptr;
   std::vector<HostChain> chains;
-  bool committed = false;
+  bool device_ready = false;
   Dev D{};
   std::vector<ChainP> cp;
-  std::vector<void*> allocs;
-  int nb = 0;
+  Pool pool[NPOOL];
+  std::vector<Arr> arrs[NPOOL];
+  std::vector<void*> allocs;  // fixed allocations (not in pools)
   int cur = 0, par = 0;
-  bool pl_fresh = false;
-  double* d_out = nullptr;      // chain reduction output (nchains * kRed)
+  double* d_out = nullptr;      // reduction output (chain slots * kRed)
   double* d_noise = nullptr;    // replay noise
   size_t noise_cap = 0;         // doubles
   double* d_rec = nullptr;      // per-record-step reductions
@@ -100,7 +161,8 @@ int fail(sopg_ctx* c, int code, const std::string& m) {
       return fail(ctx, SOPG_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-// Event bracket around one launch when profiling; returns the pool index pair.
+// ------------------------------------------------------------ timing
+
 int prof_begin(sopg_ctx* c) {
   if (!c->prof) return -1;
   while (c->ev_pool.size() < c->ev_next + 2) {
@@ -118,7 +180,6 @@ void prof_end(sopg_ctx* c, int kernel, int i) {
   (void)hipEventRecord(c->ev_pool[i + 1], c->stream);
   c->ev_pending.push_back({kernel, {i, i + 1}});
 }
-// After a stream synchronisation: fold the recorded brackets into the totals.
 void prof_collect(sopg_ctx* c) {
   for (auto& p : c->ev_pending) {
     float ms = 0.0f;
@@ -131,6 +192,8 @@ void prof_collect(sopg_ctx* c) {
   c->ev_next = 0;
 }
 
+// ------------------------------------------------------------ memory
+
 template <class T>
 int dalloc(sopg_ctx* c, T** p, size_t n) {
   void* q = nullptr;
@@ -141,12 +204,97 @@ int dalloc(sopg_ctx* c, T** p, size_t n) {
   return SOPG_OK;
 }
 
+void dfree(sopg_ctx* c, void* p) {
+  if (!p) return;
+  auto it = std::find(c->allocs.begin(), c->allocs.end(), p);
+  if (it != c->allocs.end()) c->allocs.erase(it);
+  (void)hipFree(p);
+}
+
+// Host-to-device copy that has finished when it returns (the sources are
+// often temporaries).
 template <class T>
-int upload(sopg_ctx* c, T* d, const std::vector<T>& h) {
-  if (h.empty()) return SOPG_OK;
-  HIP_OK(c, hipMemcpyAsync(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+int upload_at(sopg_ctx* c, T* d, size_t at, const T* h, size_t n) {
+  if (n == 0) return SOPG_OK;
+  HIP_OK(c, hipMemcpyAsync(d + at, h, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  HIP_OK(c, hipStreamSynchronize(c->stream));
   return SOPG_OK;
 }
+template <class T>
+int upload_at(sopg_ctx* c, T* d, size_t at, const std::vector<T>& h) {
+  return upload_at(c, d, at, h.data(), h.size());
+}
+
+// Grow a pool to at least `need` units: new arrays, old contents copied.
+int grow_pool(sopg_ctx* c, int pi, size_t need) {
+  Pool& P = c->pool[pi];
+  if (need <= P.cap) return SOPG_OK;
+  const size_t ncap = std::max(need, P.cap * 2 + 16);
+  HIP_OK(c, hipStreamSynchronize(c->stream));
+  for (Arr& a : c->arrs[pi]) {
+    void* q = nullptr;
+    const size_t nb = ncap * a.per * a.esz, ob = P.cap * a.per * a.esz;
+    hipError_t e = hipMalloc(&q, std::max<size_t>(nb, 1));
+    if (e != hipSuccess) return fail(c, SOPG_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    HIP_OK(c, hipMemsetAsync(q, 0, std::max<size_t>(nb, 1), c->stream));
+    if (*a.p && ob) HIP_OK(c, hipMemcpyAsync(q, *a.p, ob, hipMemcpyDeviceToDevice, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    if (*a.p) (void)hipFree(*a.p);
+    *a.p = q;
+  }
+  P.extend(ncap);
+  return SOPG_OK;
+}
+
+int take_range(sopg_ctx* c, int pi, size_t n, size_t& at) {
+  Pool& P = c->pool[pi];
+  if (P.take(n, at)) return SOPG_OK;
+  // grow past the current end (the last free range may be extended in place)
+  size_t tail = 0;
+  if (!P.fr.empty() && P.fr.back().second == P.cap) tail = P.fr.back().second - P.fr.back().first;
+  int rc = grow_pool(c, pi, P.cap + n - tail);
+  if (rc) return rc;
+  if (!P.take(n, at)) return fail(c, SOPG_ERR_ARG, "pool allocation failed");
+  return SOPG_OK;
+}
+
+void register_arrays(sopg_ctx* c) {
+  Dev& D = c->D;
+  auto add = [&](int pi, void* pp, size_t esz, size_t per) { c->arrs[pi].push_back({(void**)pp, esz, per}); };
+  const size_t cap = (size_t)D.cap;
+  for (int s = 0; s < 2; s++) {
+    add(P_BEAD, &D.ux[s], 8, 1); add(P_BEAD, &D.uy[s], 8, 1); add(P_BEAD, &D.uz[s], 8, 1);
+    add(P_BEAD, &D.px[s], 8, 1); add(P_BEAD, &D.py[s], 8, 1); add(P_BEAD, &D.pz[s], 8, 1);
+  }
+  for (double** p : {&D.vx, &D.vy, &D.vz, &D.fx, &D.fy, &D.fz, &D.ix, &D.iy, &D.iz, &D.cx, &D.cy, &D.cz})
+    add(P_BEAD, p, 8, 1);
+  for (int** p : {&D.chain, &D.bond_beg, &D.bond_end, &D.ang_beg, &D.ang_end, &D.att_beg, &D.att_end, &D.rep_beg,
+                  &D.rep_end, &D.cnt, &D.bead_cell, &D.cell_beads})
+    add(P_BEAD, p, 4, 1);
+  add(P_BEAD, &D.type, 1, 1);
+  add(P_BEAD, &D.slot, 4, cap);
+  add(P_BEAD, &D.slot_k, 4, cap);
+  add(P_BEAD, &D.red, 8, kRed);
+  add(P_BOND, &D.bi, 4, 1); add(P_BOND, &D.bj, 4, 1); add(P_BOND, &D.br0, 8, 1);
+  add(P_BINC, &D.bond_code, 4, 1);
+  add(P_ANG, &D.gi, 4, 1); add(P_ANG, &D.gk, 4, 1);
+  add(P_AINC, &D.ang_code, 4, 1);
+  add(P_ATT, &D.am_i, 4, 1); add(P_ATT, &D.am_j, 4, 1); add(P_ATT, &D.am_t, 1, 1);
+  for (double** p : {&D.am_r2, &D.am_r6, &D.am_r12, &D.am_nlrc2, &D.am_plrc2}) add(P_ATT, p, 8, 1);
+  add(P_ATT, &D.am_rf, 4, 1);
+  add(P_ATT, &D.am_nl, 1, 1); add(P_ATT, &D.am_pl0, 1, 1); add(P_ATT, &D.am_pl1, 1, 1);
+  add(P_TINC, &D.att_code, 4, 1); add(P_TINC, &D.att_partner, 4, 1); add(P_TINC, &D.att_flag, 1, 1);
+  add(P_REP, &D.rm_i, 4, 1); add(P_REP, &D.rm_j, 4, 1); add(P_REP, &D.rm_t, 1, 1);
+  add(P_RINC, &D.rep_code, 4, 1);
+  add(P_CHAIN, &D.cp, sizeof(ChainP), 1);
+  add(P_CHAIN, &D.grid, sizeof(GridP), 1);
+  add(P_CHAIN, &D.cell_cnt, 4, kGridCells);
+  add(P_CHAIN, &D.cell_start, 4, kGridCells);
+  add(P_CHAIN, &D.cell_fill, 4, kGridCells);
+  add(P_CHAIN, &c->d_out, 8, kRed);
+}
+
+// ------------------------------------------------------------ parameters
 
 void derive_tables(const sopg_tables& t, Tables& T) {
   double thr = 0.0;
@@ -188,8 +336,23 @@ int check_params(sopg_ctx* c, const sopg_chain_params* p) {
   return SOPG_OK;
 }
 
-void fill_chainp(const HostChain& h, ChainP& P) {
+void fill_chainp(const HostChain& h, const Tables& T, ChainP& P) {
+  if (!h.active || !h.on_device) {
+    std::memset(&P, 0, sizeof P);
+    P.cutoff = SOPG_CUTOFF_NONE;
+    P.nnlup = P.nup = 1;
+    P.phase = P.iup0 = 1;
+    return;
+  }
   const sopg_chain_params& p = h.p;
+  P.off = (int)h.at[P_BEAD];
+  P.n = h.n;
+  P.att_off = (int)h.at[P_ATT];
+  P.natt = (int)h.ai.size();
+  P.rep_explicit = h.rep_explicit ? 1 : 0;
+  double mx = 0.0;
+  for (double r : h.ar) mx = std::max(mx, r * r);
+  P.thr_att = kTol * mx;
   P.cutoff = p.cutoff;
   P.nnlup = p.nnlup;
   P.nup = p.nup;
@@ -206,228 +369,242 @@ void fill_chainp(const HostChain& h, ChainP& P) {
   P.var = std::sqrt(2.0 * p.T * p.zeta / p.h);  // energy.cpp:456
   P.T = p.T;
   P.seed = p.seed;
-  P.sim_type = p.sim_type == 2 ? 2 : 1;
   P.noise_base = h.noise_base;
+  P.sim_type = p.sim_type == 2 ? 2 : 1;
+  // grid edge of the binned rebuild: the largest neighbor-list cutoff among
+  // the chain's bead types, with margin for the rounding of the cell index
+  double rc2 = 0.0;
+  bool present[3] = {false, false, false};
+  for (int8_t t : h.type) present[t] = true;
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++)
+      if (present[a] && present[b]) rc2 = std::max(rc2, T.nlrc2[a][b]);
+  P.gcell = std::sqrt(rc2) * 1.001 + 1e-9;
 }
 
-// Counting sort of incidences by bead, stable in entry order: entry e adds
-// (e<<1|0) to bead a[e] and then (e<<1|1) to bead b[e].
-void build_csr(int nbtot, const std::vector<int>& a, const std::vector<int>& b, std::vector<int>& ptr,
-               std::vector<int>& code) {
-  ptr.assign((size_t)nbtot + 1, 0);
-  for (size_t e = 0; e < a.size(); e++) { ptr[a[e] + 1]++; ptr[b[e] + 1]++; }
-  for (int g = 0; g < nbtot; g++) ptr[g + 1] += ptr[g];
-  code.assign(ptr[nbtot], 0);
-  std::vector<int> fill(ptr.begin(), ptr.end() - 1);
-  for (size_t e = 0; e < a.size(); e++) {
-    code[fill[a[e]]++] = (int)(e << 1);
-    code[fill[b[e]]++] = (int)(e << 1) | 1;
+// Device copy of the per-chain parameters: chain `only`, or all (only < 0).
+int upload_chainp(sopg_ctx* c, int only = -1) {
+  if (!c->device_ready) return SOPG_OK;
+  const size_t cap = c->pool[P_CHAIN].cap;
+  if (c->cp.size() != cap) {
+    c->cp.resize(cap);
+    only = -1;
   }
+  const HostChain none;
+  for (size_t i = 0; i < cap; i++) {
+    if (only >= 0 && (int)i != only) continue;
+    fill_chainp(i < c->chains.size() ? c->chains[i] : none, c->T, c->cp[i]);
+  }
+  if (only >= 0) return upload_at(c, c->D.cp, (size_t)only, &c->cp[only], 1);
+  return upload_at(c, c->D.cp, 0, c->cp);
 }
 
-bool chain_has_replay(const sopg_ctx* c) {
+// Per-bead incidence lists of entries (a[e], b[e]), stable in entry order:
+// entry e adds (base+e)<<1 to bead a[e], then (base+e)<<1|1 to bead b[e].
+void build_csr(int n, const std::vector<int>& a, const std::vector<int>& b, size_t base, size_t inc_at, int boff,
+               std::vector<int>& beg, std::vector<int>& end, std::vector<int>& code) {
+  std::vector<int> cnt((size_t)n + 1, 0);
+  for (size_t e = 0; e < a.size(); e++) { cnt[a[e] + 1]++; cnt[b[e] + 1]++; }
+  for (int g = 0; g < n; g++) cnt[g + 1] += cnt[g];
+  beg.resize(n);
+  end.resize(n);
+  for (int g = 0; g < n; g++) { beg[g] = (int)inc_at + cnt[g]; end[g] = (int)inc_at + cnt[g + 1]; }
+  code.assign(cnt[n], 0);
+  std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+  for (size_t e = 0; e < a.size(); e++) {
+    code[fill[a[e]]++] = (int)((base + e) << 1);
+    code[fill[b[e]]++] = (int)((base + e) << 1) | 1;
+  }
+  (void)boff;
+}
+
+int ensure_device(sopg_ctx* c);
+int check_err_flag(sopg_ctx* c);
+
+// Put one chain's topology and zeroed state into the pools.
+int materialize(sopg_ctx* c, int id) {
+  HostChain& h = c->chains[id];
+  Dev& D = c->D;
+  int rc;
+  // a no-cutoff chain keeps every non-native partner of a bead (up to n - 4)
+  if (h.p.cutoff == SOPG_CUTOFF_NONE && h.n > D.cap)
+    return fail(c, SOPG_ERR_ARG, "a no-cutoff chain of " + std::to_string(h.n) +
+                                     " beads exceeds the per-bead list capacity " + std::to_string(D.cap) +
+                                     " (sopg_set_capacity before the first device operation)");
+  const size_t nb = h.bi.size(), na = h.gi.size(), nt = h.ai.size(), nr = h.rep_explicit ? h.ri.size() : 0;
+  if ((rc = take_range(c, P_BEAD, h.n, h.at[P_BEAD])) || (rc = take_range(c, P_BOND, nb, h.at[P_BOND])) ||
+      (rc = take_range(c, P_BINC, 2 * nb, h.at[P_BINC])) || (rc = take_range(c, P_ANG, na, h.at[P_ANG])) ||
+      (rc = take_range(c, P_AINC, 2 * na, h.at[P_AINC])) || (rc = take_range(c, P_ATT, nt, h.at[P_ATT])) ||
+      (rc = take_range(c, P_TINC, 2 * nt, h.at[P_TINC])) || (rc = take_range(c, P_REP, nr, h.at[P_REP])) ||
+      (rc = take_range(c, P_RINC, 2 * nr, h.at[P_RINC])))
+    return rc;
+  if ((size_t)id >= c->pool[P_CHAIN].cap && (rc = grow_pool(c, P_CHAIN, (size_t)id + 1))) return rc;
+  const int off = (int)h.at[P_BEAD];
+  const int n = h.n;
+  D.bitmap_words = std::max(D.bitmap_words, (n + 31) / 32);
+  // per-bead arrays
+  std::vector<int> chain(n, id);
+  std::vector<int8_t> type(h.type.begin(), h.type.end());
+  std::vector<int> beg, end, code;
+  auto glob = [&](const std::vector<int>& v) {
+    std::vector<int> g(v.size());
+    for (size_t k = 0; k < v.size(); k++) g[k] = off + v[k];
+    return g;
+  };
+  if ((rc = upload_at(c, D.chain, off, chain)) || (rc = upload_at(c, D.type, off, type))) return rc;
+  build_csr(n, h.bi, h.bj, h.at[P_BOND], h.at[P_BINC], off, beg, end, code);
+  if ((rc = upload_at(c, D.bond_beg, off, beg)) || (rc = upload_at(c, D.bond_end, off, end)) ||
+      (rc = upload_at(c, D.bond_code, h.at[P_BINC], code)) || (rc = upload_at(c, D.bi, h.at[P_BOND], glob(h.bi))) ||
+      (rc = upload_at(c, D.bj, h.at[P_BOND], glob(h.bj))) || (rc = upload_at(c, D.br0, h.at[P_BOND], h.br0)))
+    return rc;
+  build_csr(n, h.gi, h.gk, h.at[P_ANG], h.at[P_AINC], off, beg, end, code);
+  if ((rc = upload_at(c, D.ang_beg, off, beg)) || (rc = upload_at(c, D.ang_end, off, end)) ||
+      (rc = upload_at(c, D.ang_code, h.at[P_AINC], code)) || (rc = upload_at(c, D.gi, h.at[P_ANG], glob(h.gi))) ||
+      (rc = upload_at(c, D.gk, h.at[P_ANG], glob(h.gk))))
+    return rc;
+  // natives
+  build_csr(n, h.ai, h.aj, h.at[P_ATT], h.at[P_TINC], off, beg, end, code);
+  std::vector<int> partner(code.size());
+  for (size_t q = 0; q < code.size(); q++) {
+    const size_t k = (size_t)(code[q] >> 1) - h.at[P_ATT];
+    partner[q] = off + ((code[q] & 1) ? h.ai[k] : h.aj[k]);
+  }
+  std::vector<int8_t> at(nt);
+  std::vector<double> r2(nt), r6(nt), r12(nt), nlrc2(nt), plrc2(nt);
+  std::vector<float> rf(nt);
+  for (size_t k = 0; k < nt; k++) {
+    at[k] = (int8_t)(h.ait[k] | (h.ajt[k] << 2));
+    const double r = h.ar[k];
+    // params.cpp:203-208 and neighbor_list.cpp:44-47, sop.cpp:506-507
+    r2[k] = r * r;
+    r6[k] = r2[k] * r2[k] * r2[k];
+    r12[k] = r6[k] * r6[k];
+    const double rn = 3.2 * r, rp = 2.5 * r;
+    nlrc2[k] = rn * rn;
+    plrc2[k] = rp * rp;
+    rf[k] = (float)r;
+  }
+  const size_t ta = h.at[P_ATT];
+  if ((rc = upload_at(c, D.att_beg, off, beg)) || (rc = upload_at(c, D.att_end, off, end)) ||
+      (rc = upload_at(c, D.att_code, h.at[P_TINC], code)) || (rc = upload_at(c, D.att_partner, h.at[P_TINC], partner)) ||
+      (rc = upload_at(c, D.am_i, ta, glob(h.ai))) || (rc = upload_at(c, D.am_j, ta, glob(h.aj))) ||
+      (rc = upload_at(c, D.am_t, ta, at)) || (rc = upload_at(c, D.am_r2, ta, r2)) || (rc = upload_at(c, D.am_r6, ta, r6)) ||
+      (rc = upload_at(c, D.am_r12, ta, r12)) || (rc = upload_at(c, D.am_nlrc2, ta, nlrc2)) ||
+      (rc = upload_at(c, D.am_plrc2, ta, plrc2)) || (rc = upload_at(c, D.am_rf, ta, rf)))
+    return rc;
+  if (nt) {
+    HIP_OK(c, hipMemsetAsync(D.att_flag + h.at[P_TINC], 0, 2 * nt, c->stream));
+    HIP_OK(c, hipMemsetAsync(D.am_nl + ta, 0, nt, c->stream));
+    HIP_OK(c, hipMemsetAsync(D.am_pl0 + ta, 0, nt, c->stream));
+    HIP_OK(c, hipMemsetAsync(D.am_pl1 + ta, 0, nt, c->stream));
+  }
+  // explicit non-natives
+  if (h.rep_explicit) {
+    build_csr(n, h.ri, h.rj, h.at[P_REP], h.at[P_RINC], off, beg, end, code);
+    std::vector<int8_t> rt(nr);
+    for (size_t k = 0; k < nr; k++) rt[k] = (int8_t)(h.rit[k] | (h.rjt[k] << 2));
+    if ((rc = upload_at(c, D.rep_beg, off, beg)) || (rc = upload_at(c, D.rep_end, off, end)) ||
+        (rc = upload_at(c, D.rep_code, h.at[P_RINC], code)) || (rc = upload_at(c, D.rm_i, h.at[P_REP], glob(h.ri))) ||
+        (rc = upload_at(c, D.rm_j, h.at[P_REP], glob(h.rj))) || (rc = upload_at(c, D.rm_t, h.at[P_REP], rt)))
+      return rc;
+  } else {
+    std::vector<int> z(n, 0);
+    if ((rc = upload_at(c, D.rep_beg, off, z)) || (rc = upload_at(c, D.rep_end, off, z))) return rc;
+  }
+  // zero state
+  for (int s = 0; s < 2; s++)
+    for (double* p : {D.ux[s], D.uy[s], D.uz[s], D.px[s], D.py[s], D.pz[s]})
+      HIP_OK(c, hipMemsetAsync(p + off, 0, sizeof(double) * n, c->stream));
+  for (double* p : {D.vx, D.vy, D.vz, D.fx, D.fy, D.fz, D.ix, D.iy, D.iz})
+    HIP_OK(c, hipMemsetAsync(p + off, 0, sizeof(double) * n, c->stream));
+  HIP_OK(c, hipMemsetAsync(D.cnt + off, 0, sizeof(int) * n, c->stream));
+  HIP_OK(c, hipMemsetAsync(D.red + (size_t)off * kRed, 0, sizeof(double) * n * kRed, c->stream));
+  HIP_OK(c, hipMemsetAsync(D.grid + id, 0, sizeof(GridP), c->stream));
+  HIP_OK(c, hipMemsetAsync(D.cell_cnt + (size_t)id * kGridCells, 0, sizeof(int) * kGridCells, c->stream));
+  h.on_device = true;
+  D.nb = (int)c->pool[P_BEAD].hw;
+  h.pl_fresh = false;  // the new chain has no pair list yet
+  if ((rc = upload_chainp(c, id))) return rc;
+  HIP_OK(c, hipStreamSynchronize(c->stream));
+  // No-cutoff chains interact through their full master lists, as the loader
+  // leaves them (params.cpp:210-230): build that list once.
+  if (h.p.cutoff == SOPG_CUTOFF_NONE) {
+    launch_rebuild(c->list_lanes, D, c->d_T, c->cur, c->par, 0, SOPG_CUTOFF_NONE, off, off + n, id, 1, 0, c->stream);
+    HIP_OK(c, hipGetLastError());
+    return check_err_flag(c);
+  }
+  return SOPG_OK;
+}
+
+// First device operation: allocate the pools (sized for the chains added so
+// far, or the reservation) and materialise every pending chain.
+int ensure_device(sopg_ctx* c) {
+  if (c->device_ready) return SOPG_OK;
+  HIP_OK(c, hipSetDevice(c->device));
+  Dev& D = c->D;
+  size_t beads = c->reserve_beads, nch = std::max(c->reserve_chains, c->chains.size());
+  size_t nbnd = 0, nang = 0, natt = 0, nrep = 0;
+  int maxn = 0;
+  bool any_none = false;
+  size_t pend = 0;
+  for (auto& h : c->chains) {
+    if (!h.active) continue;
+    pend += h.n;
+    nbnd += h.bi.size();
+    nang += h.gi.size();
+    natt += h.ai.size();
+    if (h.rep_explicit) nrep += h.ri.size();
+    maxn = std::max(maxn, h.n);
+    any_none |= h.p.cutoff == SOPG_CUTOFF_NONE;
+  }
+  beads = std::max(beads, pend);
+  int cap = c->cap_user;
+  if (cap <= 0) {
+    cap = 1024;
+    if (any_none) cap = std::max(cap, maxn);
+  }
+  D.cap = std::max(cap, 1);
+  D.bitmap_words = 1;
+  register_arrays(c);
+  int rc;
+  const size_t init[NPOOL] = {beads, nbnd, 2 * nbnd, nang, 2 * nang, natt, 2 * natt, nrep, 2 * nrep, nch};
+  for (int pi = 0; pi < NPOOL; pi++)
+    if ((rc = grow_pool(c, pi, std::max<size_t>(init[pi], 1)))) return rc;
+  if ((rc = dalloc(c, &D.err, 1)) || (rc = dalloc(c, &c->d_T, 1))) return rc;
+  HIP_OK(c, hipMemsetAsync(D.err, 0, sizeof(int), c->stream));
+  HIP_OK(c, hipMemcpyAsync(c->d_T, &c->T, sizeof(Tables), hipMemcpyHostToDevice, c->stream));
+  c->device_ready = true;
+  for (size_t id = 0; id < c->chains.size(); id++)
+    if (c->chains[id].active && !c->chains[id].on_device && (rc = materialize(c, (int)id))) return rc;
+  return upload_chainp(c);
+}
+
+int get_chain(sopg_ctx* c, int chain, HostChain** h, ChainP** P) {
+  if (!c) return SOPG_ERR_ARG;
+  if (chain < 0 || chain >= (int)c->chains.size() || !c->chains[chain].active)
+    return fail(c, SOPG_ERR_ARG, "bad chain id");
+  int rc = ensure_device(c);
+  if (rc) return rc;
+  *h = &c->chains[chain];
+  *P = &c->cp[chain];
+  return SOPG_OK;
+}
+
+bool any_replay(const sopg_ctx* c) {
   for (auto& h : c->chains)
-    if (h.p.noise_mode == SOPG_NOISE_REPLAY) return true;
+    if (h.active && h.p.noise_mode == SOPG_NOISE_REPLAY) return true;
   return false;
-}
-
-int upload_chainp(sopg_ctx* c) {
-  for (size_t i = 0; i < c->chains.size(); i++) fill_chainp(c->chains[i], c->cp[i]);
-  return upload(c, c->D.cp, c->cp);
 }
 
 int ensure_noise(sopg_ctx* c, size_t n) {
   if (n <= c->noise_cap) return SOPG_OK;
-  if (c->d_noise) {
-    (void)hipFree(c->d_noise);
-    c->allocs.erase(std::find(c->allocs.begin(), c->allocs.end(), (void*)c->d_noise));
-  }
+  HIP_OK(c, hipStreamSynchronize(c->stream));
+  dfree(c, c->d_noise);
   c->d_noise = nullptr;
   c->noise_cap = 0;
   int rc = dalloc(c, &c->d_noise, n);
   if (rc) return rc;
   c->noise_cap = n;
-  return SOPG_OK;
-}
-
-int commit(sopg_ctx* c) {
-  if (c->committed) return SOPG_OK;
-  if (c->chains.empty()) return fail(c, SOPG_ERR_ARG, "no chains");
-  HIP_OK(c, hipSetDevice(c->device));
-  Dev& D = c->D;
-  // chain offsets
-  int nb = 0, na = 0;
-
-  int maxn = 0;
-  bool any_cell = false, any_none = false, any_explicit = false;
-  c->cp.assign(c->chains.size(), ChainP{});
-  for (size_t ic = 0; ic < c->chains.size(); ic++) {
-    HostChain& h = c->chains[ic];
-    ChainP& P = c->cp[ic];
-    P.off = nb;
-    P.n = h.n;
-    P.att_off = na;
-    P.natt = (int)h.ai.size();
-    P.rep_explicit = h.rep_explicit ? 1 : 0;
-    double mx = 0.0;
-    for (double r : h.ar) mx = std::max(mx, r * r);
-    P.thr_att = kTol * mx;
-    nb += h.n;
-    na += (int)h.ai.size();
-
-
-
-    maxn = std::max(maxn, h.n);
-    any_cell |= h.p.cutoff == SOPG_CUTOFF_CELL;
-    any_none |= h.p.cutoff == SOPG_CUTOFF_NONE;
-    any_explicit |= h.rep_explicit;
-  }
-  c->nb = nb;
-  int cap = c->cap_user;
-  if (cap <= 0) {
-    cap = std::min(maxn, any_cell ? 1024 : 512);
-    if (any_none) cap = maxn;
-  }
-  cap = std::max(cap, 1);
-  D.nb = nb;
-  D.cap = cap;
-
-  // concatenated host arrays
-  std::vector<int> chain(nb);
-  std::vector<int8_t> type(nb, 1);
-  std::vector<int> bi, bj, gi, gk, ai, aj, ri, rj;
-  std::vector<double> br0, r2, r6, r12, nlrc2, plrc2;
-  std::vector<float> rf;
-  std::vector<int8_t> at, rt;
-  for (size_t ic = 0; ic < c->chains.size(); ic++) {
-    HostChain& h = c->chains[ic];
-    const int off = c->cp[ic].off;
-    for (int b = 0; b < h.n; b++) { chain[off + b] = (int)ic; type[off + b] = h.type.empty() ? 1 : h.type[b]; }
-    for (size_t k = 0; k < h.bi.size(); k++) { bi.push_back(off + h.bi[k]); bj.push_back(off + h.bj[k]); br0.push_back(h.br0[k]); }
-    for (size_t k = 0; k < h.gi.size(); k++) { gi.push_back(off + h.gi[k]); gk.push_back(off + h.gk[k]); }
-    for (size_t k = 0; k < h.ai.size(); k++) {
-      ai.push_back(off + h.ai[k]);
-      aj.push_back(off + h.aj[k]);
-      at.push_back((int8_t)(h.ait[k] | (h.ajt[k] << 2)));
-      const double r = h.ar[k];
-      // params.cpp:203-208 and neighbor_list.cpp:44-47, sop.cpp:506-507
-      const double rr2 = r * r;
-      const double rr6 = rr2 * rr2 * rr2;
-      r2.push_back(rr2);
-      r6.push_back(rr6);
-      r12.push_back(rr6 * rr6);
-      const double rn = 3.2 * r, rp = 2.5 * r;
-      nlrc2.push_back(rn * rn);
-      plrc2.push_back(rp * rp);
-      rf.push_back((float)r);
-    }
-    if (h.rep_explicit)
-      for (size_t k = 0; k < h.ri.size(); k++) {
-        ri.push_back(off + h.ri[k]);
-        rj.push_back(off + h.rj[k]);
-        rt.push_back((int8_t)(h.rit[k] | (h.rjt[k] << 2)));
-      }
-  }
-  std::vector<int> bond_ptr, bond_code, ang_ptr, ang_code, att_ptr, att_code, rep_ptr, rep_code;
-  build_csr(nb, bi, bj, bond_ptr, bond_code);
-  build_csr(nb, gi, gk, ang_ptr, ang_code);
-  build_csr(nb, ai, aj, att_ptr, att_code);
-  std::vector<int> att_partner(att_code.size());
-  for (size_t q = 0; q < att_code.size(); q++) {
-    const int k = att_code[q] >> 1;
-    att_partner[q] = (att_code[q] & 1) ? ai[k] : aj[k];
-  }
-  if (any_explicit) build_csr(nb, ri, rj, rep_ptr, rep_code);
-  else rep_ptr.assign((size_t)nb + 1, 0);
-
-  int rc = 0;
-#define A(p, n) if ((rc = dalloc(c, &(p), (n)))) return rc
-  for (int s = 0; s < 2; s++) {
-    A(D.ux[s], nb); A(D.uy[s], nb); A(D.uz[s], nb);
-    A(D.px[s], nb); A(D.py[s], nb); A(D.pz[s], nb);
-  }
-  A(D.vx, nb); A(D.vy, nb); A(D.vz, nb);
-  A(D.fx, nb); A(D.fy, nb); A(D.fz, nb);
-  A(D.ix, nb); A(D.iy, nb); A(D.iz, nb);
-  A(D.chain, nb); A(D.type, nb);
-  A(D.bond_ptr, nb + 1); A(D.bond_code, bond_code.size()); A(D.bi, bi.size()); A(D.bj, bj.size()); A(D.br0, br0.size());
-  A(D.ang_ptr, nb + 1); A(D.ang_code, ang_code.size()); A(D.gi, gi.size()); A(D.gk, gk.size());
-  A(D.att_ptr, nb + 1); A(D.att_code, att_code.size()); A(D.att_partner, att_partner.size());
-  A(D.att_flag, att_code.size());
-  A(D.am_i, ai.size()); A(D.am_j, aj.size()); A(D.am_t, at.size());
-  A(D.am_r2, r2.size()); A(D.am_r6, r6.size()); A(D.am_r12, r12.size());
-  A(D.am_nlrc2, nlrc2.size()); A(D.am_plrc2, plrc2.size()); A(D.am_rf, rf.size());
-  A(D.am_nl, ai.size()); A(D.am_pl0, ai.size()); A(D.am_pl1, ai.size());
-  A(D.rep_ptr, nb + 1);
-  if (any_explicit) {
-    A(D.rep_code, rep_code.size()); A(D.rm_i, ri.size()); A(D.rm_j, rj.size()); A(D.rm_t, rt.size());
-    A(D.slot_k, (size_t)nb * cap);
-  } else {
-    D.rep_code = nullptr; D.rm_i = D.rm_j = nullptr; D.rm_t = nullptr; D.slot_k = nullptr;
-  }
-  A(D.slot, (size_t)nb * cap); A(D.cnt, nb);
-  A(D.cx, nb); A(D.cy, nb); A(D.cz, nb);
-  A(D.red, (size_t)nb * kRed);
-  A(D.err, 1);
-  A(D.cp, c->chains.size());
-  A(c->d_T, 1);
-  A(c->d_out, c->chains.size() * kRed);
-#undef A
-  hipStream_t s = c->stream;
-  for (int k = 0; k < 2; k++) {
-    HIP_OK(c, hipMemsetAsync(D.ux[k], 0, sizeof(double) * nb, s));
-    HIP_OK(c, hipMemsetAsync(D.uy[k], 0, sizeof(double) * nb, s));
-    HIP_OK(c, hipMemsetAsync(D.uz[k], 0, sizeof(double) * nb, s));
-    HIP_OK(c, hipMemsetAsync(D.px[k], 0, sizeof(double) * nb, s));
-    HIP_OK(c, hipMemsetAsync(D.py[k], 0, sizeof(double) * nb, s));
-    HIP_OK(c, hipMemsetAsync(D.pz[k], 0, sizeof(double) * nb, s));
-  }
-  for (double* p : {D.vx, D.vy, D.vz, D.fx, D.fy, D.fz, D.ix, D.iy, D.iz})
-    HIP_OK(c, hipMemsetAsync(p, 0, sizeof(double) * nb, s));
-  HIP_OK(c, hipMemsetAsync(D.red, 0, sizeof(double) * nb * kRed, s));
-  HIP_OK(c, hipMemsetAsync(D.cnt, 0, sizeof(int) * nb, s));
-  HIP_OK(c, hipMemsetAsync(D.att_flag, 0, att_code.size() ? att_code.size() : 1, s));
-  HIP_OK(c, hipMemsetAsync(D.am_nl, 0, ai.size() ? ai.size() : 1, s));
-  HIP_OK(c, hipMemsetAsync(D.am_pl0, 0, ai.size() ? ai.size() : 1, s));
-  HIP_OK(c, hipMemsetAsync(D.am_pl1, 0, ai.size() ? ai.size() : 1, s));
-  HIP_OK(c, hipMemsetAsync(D.err, 0, sizeof(int), s));
-  if ((rc = upload(c, D.chain, chain)) || (rc = upload(c, D.type, type)) || (rc = upload(c, D.bond_ptr, bond_ptr)) ||
-      (rc = upload(c, D.bond_code, bond_code)) || (rc = upload(c, D.bi, bi)) || (rc = upload(c, D.bj, bj)) ||
-      (rc = upload(c, D.br0, br0)) || (rc = upload(c, D.ang_ptr, ang_ptr)) || (rc = upload(c, D.ang_code, ang_code)) ||
-      (rc = upload(c, D.gi, gi)) || (rc = upload(c, D.gk, gk)) || (rc = upload(c, D.att_ptr, att_ptr)) ||
-      (rc = upload(c, D.att_code, att_code)) || (rc = upload(c, D.att_partner, att_partner)) ||
-      (rc = upload(c, D.am_i, ai)) || (rc = upload(c, D.am_j, aj)) || (rc = upload(c, D.am_t, at)) ||
-      (rc = upload(c, D.am_r2, r2)) || (rc = upload(c, D.am_r6, r6)) || (rc = upload(c, D.am_r12, r12)) ||
-      (rc = upload(c, D.am_nlrc2, nlrc2)) || (rc = upload(c, D.am_plrc2, plrc2)) || (rc = upload(c, D.am_rf, rf)) ||
-      (rc = upload(c, D.rep_ptr, rep_ptr)))
-    return rc;
-  if (any_explicit &&
-      ((rc = upload(c, D.rep_code, rep_code)) || (rc = upload(c, D.rm_i, ri)) || (rc = upload(c, D.rm_j, rj)) ||
-       (rc = upload(c, D.rm_t, rt))))
-    return rc;
-  if ((rc = upload_chainp(c))) return rc;
-  HIP_OK(c, hipMemcpyAsync(c->d_T, &c->T, sizeof(Tables), hipMemcpyHostToDevice, s));
-  HIP_OK(c, hipStreamSynchronize(s));
-  c->committed = true;
-  // No-cutoff chains interact through their full master lists, as the loader
-  // leaves them (params.cpp:210-230): build that list once.
-  if (any_none) {
-    for (size_t ic = 0; ic < c->chains.size(); ic++)
-      if (c->chains[ic].p.cutoff == SOPG_CUTOFF_NONE)
-        launch_rebuild(c->list_lanes, D, c->d_T, c->cur, c->par, 0, SOPG_CUTOFF_NONE, c->cp[ic].off,
-                       c->cp[ic].off + c->cp[ic].n, s);
-    HIP_OK(c, hipGetLastError());
-  }
-  return SOPG_OK;
-}
-
-int get_chain(sopg_ctx* c, int chain, HostChain** h, ChainP** P) {
-  if (!c) return SOPG_ERR_ARG;
-  if (chain < 0 || chain >= (int)c->chains.size()) return fail(c, SOPG_ERR_ARG, "bad chain id");
-  int rc = commit(c);
-  if (rc) return rc;
-  *h = &c->chains[chain];
-  *P = &c->cp[chain];
   return SOPG_OK;
 }
 
@@ -467,10 +644,11 @@ int rebuild_one(sopg_ctx* c, int chain, int mode, int* nnl_att, int* nnl_rep) {
   if (mode == SOPG_CUTOFF_CELL && (!(h->p.ncell > 0.0) || !(h->p.lcell > 0.0)))
     return fail(c, SOPG_ERR_ARG, "cell list needs ncell > 0 and lcell > 0");
   if (mode == SOPG_CUTOFF_CELL) launch_cellcoords(c->D, c->cur, P->off, P->off + P->n, c->stream);
-  launch_rebuild(c->list_lanes, c->D, c->d_T, c->cur, c->par, 0, mode, P->off, P->off + P->n, c->stream);
+  launch_rebuild(c->list_lanes, c->D, c->d_T, c->cur, c->par, 0, mode, P->off, P->off + P->n, chain, 1, c->binned,
+                 c->stream);
   HIP_OK(c, hipGetLastError());
   if ((rc = check_err_flag(c))) return rc;
-  c->pl_fresh = true;
+  h->pl_fresh = true;
   double out[kRed];
   if ((rc = list_counts(c, chain, out))) return rc;
   if (nnl_att) *nnl_att = (int)out[R_NLA];
@@ -489,7 +667,7 @@ const double* chain_noise(sopg_ctx* c, const ChainP& P, const HostChain& h, cons
   *rc = SOPG_OK;
   if (h.p.noise_mode == SOPG_NOISE_REPLAY) {
     if (!noise) { *rc = fail(c, SOPG_ERR_ARG, "chain uses replayed noise: pass its 3N gaussians"); return nullptr; }
-    if ((*rc = ensure_noise(c, 3 * (size_t)c->nb))) return nullptr;
+    if ((*rc = ensure_noise(c, 3 * (size_t)c->D.nb))) return nullptr;
     hipError_t e = hipMemcpyAsync(c->d_noise + 3 * (size_t)P.off, noise, sizeof(double) * 3 * (size_t)P.n,
                                   hipMemcpyHostToDevice, c->stream);
     if (e != hipSuccess) { *rc = fail(c, SOPG_ERR_HIP, hipGetErrorString(e)); return nullptr; }
@@ -497,6 +675,81 @@ const double* chain_noise(sopg_ctx* c, const ChainP& P, const HostChain& h, cons
   }
   if (noise) { *rc = fail(c, SOPG_ERR_ARG, "chain uses the device generator: pass noise = NULL"); return nullptr; }
   return nullptr;
+}
+
+int parse_topology(sopg_ctx* c, const sopg_topology* t, HostChain& h) {
+  const int n = t->nbead;
+  if (n < 1 || n >= kMaxChainBeads) return fail(c, SOPG_ERR_ARG, "nbead out of range");
+  h.n = n;
+  auto bead = [&](int x) { return x >= 1 && x <= n; };
+  auto typ = [&](int x) { return x >= 0 && x <= 2; };
+  for (int k = 0; k < t->nbnd; k++) {
+    if (!bead(t->ibnd[k]) || !bead(t->jbnd[k])) return fail(c, SOPG_ERR_ARG, "bond bead index out of range");
+    h.bi.push_back(t->ibnd[k] - 1); h.bj.push_back(t->jbnd[k] - 1); h.br0.push_back(t->r0bnd[k]);
+  }
+  for (int k = 0; k < t->nang; k++) {
+    if (!bead(t->iang[k]) || !bead(t->kang[k])) return fail(c, SOPG_ERR_ARG, "angle bead index out of range");
+    h.gi.push_back(t->iang[k] - 1); h.gk.push_back(t->kang[k] - 1);
+  }
+  bool lex = true;
+  long prev = -1;
+  for (int k = 0; k < t->ncon_att; k++) {
+    const int i = t->att_i[k], j = t->att_j[k];
+    if (!bead(i) || !bead(j) || !typ(t->att_it[k]) || !typ(t->att_jt[k]))
+      return fail(c, SOPG_ERR_ARG, "native contact index/type out of range");
+    h.ai.push_back(i - 1); h.aj.push_back(j - 1); h.ait.push_back(t->att_it[k]); h.ajt.push_back(t->att_jt[k]);
+    h.ar.push_back(t->att_r[k]);
+    const long key = (long)(i - 1) * n + (j - 1);
+    if (!(i < j && j - i > 2 && key > prev)) lex = false;
+    prev = key;
+  }
+  h.type.assign(n, 1);
+  if (t->bead_type)
+    for (int b = 0; b < n; b++) {
+      if (!typ(t->bead_type[b])) return fail(c, SOPG_ERR_ARG, "bead type out of range");
+      h.type[b] = (int8_t)t->bead_type[b];
+    }
+  if (t->ncon_rep < 0) {
+    // implicit complement list
+    if (!lex) return fail(c, SOPG_ERR_ARG, "implicit non-native list needs natives with j-i>2 in (i,j) order");
+    h.rep_explicit = false;
+    return SOPG_OK;
+  }
+  for (int k = 0; k < t->ncon_rep; k++)
+    if (!bead(t->rep_i[k]) || !bead(t->rep_j[k]) || !typ(t->rep_it[k]) || !typ(t->rep_jt[k]))
+      return fail(c, SOPG_ERR_ARG, "non-native contact index/type out of range");
+  // Is the given list exactly the implicit complement with per-bead types?
+  bool same = lex;
+  if (same) {
+    std::vector<int8_t> ty(n, -1);
+    auto settype = [&](int b, int tv) {
+      if (ty[b] < 0) ty[b] = (int8_t)tv;
+      return ty[b] == tv;
+    };
+    long k = 0, a = 0;
+    const long na = (long)h.ai.size();
+    for (int i = 0; i < n && same; i++)
+      for (int j = i + 3; j < n && same; j++) {
+        if (a < na && h.ai[a] == i && h.aj[a] == j) { a++; continue; }
+        if (k >= t->ncon_rep || t->rep_i[k] != i + 1 || t->rep_j[k] != j + 1 || !settype(i, t->rep_it[k]) ||
+            !settype(j, t->rep_jt[k]))
+          same = false;
+        k++;
+      }
+    if (same && k != t->ncon_rep) same = false;
+    if (same && t->bead_type)
+      for (int b = 0; b < n; b++)
+        if (ty[b] >= 0 && ty[b] != h.type[b]) same = false;
+    if (same && !t->bead_type)
+      for (int b = 0; b < n; b++) h.type[b] = ty[b] >= 0 ? ty[b] : 1;
+  }
+  h.rep_explicit = !same;
+  if (h.rep_explicit)
+    for (int k = 0; k < t->ncon_rep; k++) {
+      h.ri.push_back(t->rep_i[k] - 1); h.rj.push_back(t->rep_j[k] - 1);
+      h.rit.push_back(t->rep_it[k]); h.rjt.push_back(t->rep_jt[k]);
+    }
+  return SOPG_OK;
 }
 
 }  // namespace
@@ -528,6 +781,9 @@ int sopg_destroy(sopg_ctx* c) {
   if (!c) return SOPG_OK;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (int pi = 0; pi < NPOOL; pi++)
+    for (Arr& a : c->arrs[pi])
+      if (*a.p) (void)hipFree(*a.p);
   for (void* p : c->allocs) (void)hipFree(p);
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -567,9 +823,10 @@ int sopg_set_tables(sopg_ctx* c, const sopg_tables* t) {
   if (!c || !t) return SOPG_ERR_ARG;
   c->tab = *t;
   derive_tables(c->tab, c->T);
-  if (c->committed) {
+  if (c->device_ready) {
     HIP_OK(c, hipMemcpyAsync(c->d_T, &c->T, sizeof(Tables), hipMemcpyHostToDevice, c->stream));
     HIP_OK(c, hipStreamSynchronize(c->stream));
+    return upload_chainp(c);
   }
   return SOPG_OK;
 }
@@ -585,110 +842,97 @@ int sopg_set_lanes(sopg_ctx* c, int step_lanes, int list_lanes) {
   return SOPG_OK;
 }
 
+int sopg_get_lanes(const sopg_ctx* c, int* step_lanes, int* list_lanes) {
+  if (!c) return SOPG_ERR_ARG;
+  if (step_lanes) *step_lanes = c->lanes;
+  if (list_lanes) *list_lanes = c->list_lanes;
+  return SOPG_OK;
+}
+
+int sopg_set_binned(sopg_ctx* c, int enable) {
+  if (!c) return SOPG_ERR_ARG;
+  c->binned = enable ? 1 : 0;
+  return SOPG_OK;
+}
+
 int sopg_set_capacity(sopg_ctx* c, int cap) {
   if (!c) return SOPG_ERR_ARG;
-  if (c->committed) return fail(c, SOPG_ERR_ARG, "capacity must be set before the first device operation");
+  if (c->device_ready) return fail(c, SOPG_ERR_ARG, "capacity must be set before the first device operation");
   c->cap_user = cap;
+  return SOPG_OK;
+}
+
+int sopg_reserve(sopg_ctx* c, long beads, int chains) {
+  if (!c || beads < 0 || chains < 0) return SOPG_ERR_ARG;
+  if (c->device_ready) {
+    int rc;
+    if ((rc = grow_pool(c, P_BEAD, (size_t)beads)) || (rc = grow_pool(c, P_CHAIN, (size_t)chains))) return rc;
+    return upload_chainp(c);
+  }
+  c->reserve_beads = (size_t)beads;
+  c->reserve_chains = (size_t)chains;
   return SOPG_OK;
 }
 
 int sopg_add_chain(sopg_ctx* c, const sopg_topology* t, const sopg_chain_params* p, int* id) {
   if (!c || !t) return SOPG_ERR_ARG;
-  if (c->committed) return fail(c, SOPG_ERR_ARG, "chains must be added before the first device operation");
   int rc = check_params(c, p);
   if (rc) return rc;
-  const int n = t->nbead;
-  if (n < 1 || n >= kMaxChainBeads) return fail(c, SOPG_ERR_ARG, "nbead out of range");
   HostChain h;
   h.p = *p;
-  h.n = n;
-  auto bead = [&](int x) { return x >= 1 && x <= n; };
-  auto typ = [&](int x) { return x >= 0 && x <= 2; };
-  for (int k = 0; k < t->nbnd; k++) {
-    if (!bead(t->ibnd[k]) || !bead(t->jbnd[k])) return fail(c, SOPG_ERR_ARG, "bond bead index out of range");
-    h.bi.push_back(t->ibnd[k] - 1); h.bj.push_back(t->jbnd[k] - 1); h.br0.push_back(t->r0bnd[k]);
+  if ((rc = parse_topology(c, t, h))) return rc;
+  h.active = true;
+  // reuse the lowest free chain slot
+  size_t slot = c->chains.size();
+  for (size_t k = 0; k < c->chains.size(); k++)
+    if (!c->chains[k].active) { slot = k; break; }
+  if (slot == c->chains.size()) c->chains.emplace_back();
+  c->chains[slot] = std::move(h);
+  if (c->device_ready && (rc = materialize(c, (int)slot))) {
+    c->chains[slot].active = false;
+    return rc;
   }
-  for (int k = 0; k < t->nang; k++) {
-    if (!bead(t->iang[k]) || !bead(t->kang[k])) return fail(c, SOPG_ERR_ARG, "angle bead index out of range");
-    h.gi.push_back(t->iang[k] - 1); h.gk.push_back(t->kang[k] - 1);
-  }
-  bool lex = true;
-  long prev = -1;
-  for (int k = 0; k < t->ncon_att; k++) {
-    const int i = t->att_i[k], j = t->att_j[k];
-    if (!bead(i) || !bead(j) || !typ(t->att_it[k]) || !typ(t->att_jt[k]))
-      return fail(c, SOPG_ERR_ARG, "native contact index/type out of range");
-    h.ai.push_back(i - 1); h.aj.push_back(j - 1); h.ait.push_back(t->att_it[k]); h.ajt.push_back(t->att_jt[k]);
-    h.ar.push_back(t->att_r[k]);
-    const long key = (long)(i - 1) * n + (j - 1);
-    if (!(i < j && j - i > 2 && key > prev)) lex = false;
-    prev = key;
-  }
-  h.type.assign(n, 1);
-  if (t->bead_type)
-    for (int b = 0; b < n; b++) {
-      if (!typ(t->bead_type[b])) return fail(c, SOPG_ERR_ARG, "bead type out of range");
-      h.type[b] = (int8_t)t->bead_type[b];
-    }
-  if (t->ncon_rep < 0) {
-    // implicit complement list
-    if (!lex) return fail(c, SOPG_ERR_ARG, "implicit non-native list needs natives with j-i>2 in (i,j) order");
-    h.rep_explicit = false;
-  } else {
-    for (int k = 0; k < t->ncon_rep; k++) {
-      if (!bead(t->rep_i[k]) || !bead(t->rep_j[k]) || !typ(t->rep_it[k]) || !typ(t->rep_jt[k]))
-        return fail(c, SOPG_ERR_ARG, "non-native contact index/type out of range");
-    }
-    // Is the given list exactly the implicit complement with per-bead types?
-    bool same = lex;
-    if (same) {
-      std::vector<int8_t> ty(n, -1);
-      auto settype = [&](int b, int tv) {
-        if (ty[b] < 0) ty[b] = (int8_t)tv;
-        return ty[b] == tv;
-      };
-      long k = 0, a = 0;
-      const long na = (long)h.ai.size();
-      for (int i = 0; i < n && same; i++)
-        for (int j = i + 3; j < n && same; j++) {
-          if (a < na && h.ai[a] == i && h.aj[a] == j) { a++; continue; }
-          if (k >= t->ncon_rep || t->rep_i[k] != i + 1 || t->rep_j[k] != j + 1 || !settype(i, t->rep_it[k]) ||
-              !settype(j, t->rep_jt[k]))
-            same = false;
-          k++;
-        }
-      if (same && k != t->ncon_rep) same = false;
-      if (same && t->bead_type)
-        for (int b = 0; b < n; b++)
-          if (ty[b] >= 0 && ty[b] != h.type[b]) same = false;
-      if (same && !t->bead_type)
-        for (int b = 0; b < n; b++) h.type[b] = ty[b] >= 0 ? ty[b] : 1;
-    }
-    h.rep_explicit = !same;
-    if (h.rep_explicit)
-      for (int k = 0; k < t->ncon_rep; k++) {
-        h.ri.push_back(t->rep_i[k] - 1); h.rj.push_back(t->rep_j[k] - 1);
-        h.rit.push_back(t->rep_it[k]); h.rjt.push_back(t->rep_jt[k]);
-      }
-  }
-  c->chains.push_back(std::move(h));
-  if (id) *id = (int)c->chains.size() - 1;
+  if (id) *id = (int)slot;
   return SOPG_OK;
+}
+
+int sopg_remove_chain(sopg_ctx* c, int chain) {
+  if (!c) return SOPG_ERR_ARG;
+  if (chain < 0 || chain >= (int)c->chains.size() || !c->chains[chain].active)
+    return fail(c, SOPG_ERR_ARG, "bad chain id");
+  HostChain& h = c->chains[chain];
+  if (h.on_device) {
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    const std::vector<int> none(h.n, -1);
+    int rc = upload_at(c, c->D.chain, h.at[P_BEAD], none);
+    if (rc) return rc;
+    const size_t nb = h.bi.size(), na = h.gi.size(), nt = h.ai.size(), nr = h.rep_explicit ? h.ri.size() : 0;
+    const size_t len[NPOOL] = {(size_t)h.n, nb, 2 * nb, na, 2 * na, nt, 2 * nt, nr, 2 * nr, 0};
+    for (int pi = 0; pi < P_CHAIN; pi++) c->pool[pi].give(h.at[pi], len[pi]);
+  }
+  h = HostChain();
+  return upload_chainp(c, chain);
 }
 
 int sopg_set_chain_params(sopg_ctx* c, int chain, const sopg_chain_params* p) {
   if (!c) return SOPG_ERR_ARG;
-  if (chain < 0 || chain >= (int)c->chains.size()) return fail(c, SOPG_ERR_ARG, "bad chain id");
+  if (chain < 0 || chain >= (int)c->chains.size() || !c->chains[chain].active)
+    return fail(c, SOPG_ERR_ARG, "bad chain id");
   int rc = check_params(c, p);
   if (rc) return rc;
   c->chains[chain].p = *p;
-  if (c->committed) return upload_chainp(c);
-  return SOPG_OK;
+  return upload_chainp(c, chain);
 }
 
-int sopg_nchains(const sopg_ctx* c) { return c ? (int)c->chains.size() : 0; }
+int sopg_nchains(const sopg_ctx* c) {
+  if (!c) return 0;
+  int n = 0;
+  for (auto& h : c->chains) n += h.active ? 1 : 0;
+  return n;
+}
+
 int sopg_chain_nbead(const sopg_ctx* c, int chain) {
-  return (c && chain >= 0 && chain < (int)c->chains.size()) ? c->chains[chain].n : -1;
+  return (c && chain >= 0 && chain < (int)c->chains.size() && c->chains[chain].active) ? c->chains[chain].n : -1;
 }
 
 int sopg_put_state(sopg_ctx* c, int chain, const double* pos, const double* unc, const double* vel,
@@ -709,10 +953,11 @@ int sopg_put_state(sopg_ctx* c, int chain, const double* pos, const double* unc,
     return SOPG_OK;
   };
   Dev& D = c->D;
-  if ((rc = put(pos, D.px[c->cur], D.py[c->cur], D.pz[c->cur])) || (rc = put(unc, D.ux[c->cur], D.uy[c->cur], D.uz[c->cur])) ||
-      (rc = put(vel, D.vx, D.vy, D.vz)) || (rc = put(force, D.fx, D.fy, D.fz)))
+  if ((rc = put(pos, D.px[c->cur], D.py[c->cur], D.pz[c->cur])) ||
+      (rc = put(unc, D.ux[c->cur], D.uy[c->cur], D.uz[c->cur])) || (rc = put(vel, D.vx, D.vy, D.vz)) ||
+      (rc = put(force, D.fx, D.fy, D.fz)))
     return rc;
-  if (unc) c->pl_fresh = false;
+  if (unc) h->pl_fresh = false;
   return SOPG_OK;
 }
 
@@ -733,8 +978,9 @@ int sopg_get_state(sopg_ctx* c, int chain, double* pos, double* unc, double* vel
     return SOPG_OK;
   };
   Dev& D = c->D;
-  if ((rc = get(pos, D.px[c->cur], D.py[c->cur], D.pz[c->cur])) || (rc = get(unc, D.ux[c->cur], D.uy[c->cur], D.uz[c->cur])) ||
-      (rc = get(vel, D.vx, D.vy, D.vz)) || (rc = get(force, D.fx, D.fy, D.fz)))
+  if ((rc = get(pos, D.px[c->cur], D.py[c->cur], D.pz[c->cur])) ||
+      (rc = get(unc, D.ux[c->cur], D.uy[c->cur], D.uz[c->cur])) || (rc = get(vel, D.vx, D.vy, D.vz)) ||
+      (rc = get(force, D.fx, D.fy, D.fz)))
     return rc;
   return SOPG_OK;
 }
@@ -754,7 +1000,7 @@ int sopg_update_pair_list(sopg_ctx* c, int chain, int* nil_att, int* nil_rep) {
   if (rc) return rc;
   launch_pairfilter(c->list_lanes, c->D, c->d_T, c->cur, c->par, P->off, P->off + P->n, c->stream);
   HIP_OK(c, hipGetLastError());
-  c->pl_fresh = true;
+  h->pl_fresh = true;
   double out[kRed];
   if ((rc = list_counts(c, chain, out))) return rc;
   if (nil_att) *nil_att = (int)out[R_PLA];
@@ -767,7 +1013,7 @@ int sopg_force_eval(sopg_ctx* c, int chain, const double* noise) {
   ChainP* P;
   int rc = get_chain(c, chain, &h, &P);
   if (rc) return rc;
-  if ((rc = upload_chainp(c))) return rc;
+  if ((rc = upload_chainp(c, chain))) return rc;
   const double* dn = chain_noise(c, *P, *h, noise, &rc);
   if (rc) return rc;
   launch_force(c->lanes, c->D, c->d_T, c->cur, c->par, 0, F_STORE | F_ALLTERMS, false, dn, P->off, P->off + P->n,
@@ -785,7 +1031,7 @@ int sopg_force_terms(sopg_ctx* c, int chain, int terms, const double* noise) {
   if (rc) return rc;
   if (terms & ~(SOPG_TERM_RANDOM | SOPG_TERM_FENE | SOPG_TERM_ANGULAR | SOPG_TERM_VDW))
     return fail(c, SOPG_ERR_ARG, "unknown force term");
-  if ((rc = upload_chainp(c))) return rc;
+  if ((rc = upload_chainp(c, chain))) return rc;
   const double* dn = nullptr;
   if (terms & SOPG_TERM_RANDOM) {
     dn = chain_noise(c, *P, *h, noise, &rc);
@@ -808,10 +1054,9 @@ int sopg_energy_eval(sopg_ctx* c, int chain, sopg_energies* out) {
   ChainP* P;
   int rc = get_chain(c, chain, &h, &P);
   if (rc) return rc;
-  if ((rc = upload_chainp(c))) return rc;
+  if ((rc = upload_chainp(c, chain))) return rc;
   launch_force(c->lanes, c->D, c->d_T, c->cur, c->par, 0, F_ENERGY_ALL | F_ALLTERMS, false, nullptr, P->off,
-               P->off + P->n,
-               c->stream);
+               P->off + P->n, c->stream);
   HIP_OK(c, hipGetLastError());
   double r[kRed];
   if ((rc = reduce_chain(c, chain, r))) return rc;
@@ -831,17 +1076,16 @@ int sopg_underdamped_iteration(sopg_ctx* c, int chain, const double* noise) {
   ChainP* P;
   int rc = get_chain(c, chain, &h, &P);
   if (rc) return rc;
-  if ((rc = upload_chainp(c))) return rc;
+  if ((rc = upload_chainp(c, chain))) return rc;
   const double* dn = chain_noise(c, *P, *h, noise, &rc);
   if (rc) return rc;
   launch_posupdate(c->D, c->cur, c->cur, P->off, P->off + P->n, c->stream);
   launch_force(c->lanes, c->D, c->d_T, c->cur, c->par, 0, F_STORE | F_VEL | F_ALLTERMS, false, dn, P->off,
-               P->off + P->n,
-               c->stream);
+               P->off + P->n, c->stream);
   HIP_OK(c, hipGetLastError());
   HIP_OK(c, hipStreamSynchronize(c->stream));
   h->noise_base++;
-  c->pl_fresh = false;
+  h->pl_fresh = false;
   return SOPG_OK;
 }
 
@@ -882,8 +1126,7 @@ int sopg_get_list(sopg_ctx* c, int chain, int which, int* oi, int* oj, long cap,
   if (which == SOPG_LIST_NL_ATT || which == SOPG_LIST_PL_ATT) {
     std::vector<uint8_t> f(P->natt);
     const uint8_t* src = (which == SOPG_LIST_NL_ATT) ? c->D.am_nl : (c->par ? c->D.am_pl1 : c->D.am_pl0);
-    if (P->natt)
-      HIP_OK(c, hipMemcpyAsync(f.data(), src + P->att_off, P->natt, hipMemcpyDeviceToHost, c->stream));
+    if (P->natt) HIP_OK(c, hipMemcpyAsync(f.data(), src + P->att_off, P->natt, hipMemcpyDeviceToHost, c->stream));
     HIP_OK(c, hipStreamSynchronize(c->stream));
     for (int k = 0; k < P->natt; k++)
       if (f[k]) L.push_back({h->ai[k] + 1, h->aj[k] + 1});
@@ -927,7 +1170,8 @@ int sopg_get_list(sopg_ctx* c, int chain, int which, int* oi, int* oj, long cap,
 
 int sopg_set_step(sopg_ctx* c, int chain, double istep, int inlup, int iup) {
   if (!c) return SOPG_ERR_ARG;
-  if (chain < 0 || chain >= (int)c->chains.size()) return fail(c, SOPG_ERR_ARG, "bad chain id");
+  if (chain < 0 || chain >= (int)c->chains.size() || !c->chains[chain].active)
+    return fail(c, SOPG_ERR_ARG, "bad chain id");
   HostChain& h = c->chains[chain];
   // inlup = 0 (overdamped_ctrl starts from the global inlup = 0, sop.cpp:428)
   // rebuilds on the first step, as inlup = nnlup does
@@ -937,6 +1181,15 @@ int sopg_set_step(sopg_ctx* c, int chain, double istep, int inlup, int iup) {
   h.istep = istep;
   h.inlup = inlup;
   h.iup = iup;
+  return upload_chainp(c, chain);
+}
+
+int sopg_get_step(const sopg_ctx* c, int chain, double* istep, int* inlup, int* iup) {
+  if (!c || chain < 0 || chain >= (int)c->chains.size() || !c->chains[chain].active) return SOPG_ERR_ARG;
+  const HostChain& h = c->chains[chain];
+  if (istep) *istep = h.istep;
+  if (inlup) *inlup = h.inlup;
+  if (iup) *iup = h.iup;
   return SOPG_OK;
 }
 
@@ -976,42 +1229,87 @@ int sopg_profile_read(sopg_ctx* c, int kernel, double* total_ms, long* launches)
   return SOPG_OK;
 }
 
+// Autotuner: times the step kernel (all force terms, no stores) at every lane
+// count on the context's current chains and keeps the fastest.  Results do not
+// depend on the lane count, so this only changes speed.
+int sopg_autotune(sopg_ctx* c, int* step_lanes) {
+  if (!c) return SOPG_ERR_ARG;
+  int rc = ensure_device(c);
+  if (rc) return rc;
+  if ((rc = upload_chainp(c))) return rc;
+  const int nb = c->D.nb;
+  if (nb == 0) return fail(c, SOPG_ERR_ARG, "no chains");
+  hipEvent_t e0, e1;
+  HIP_OK(c, hipEventCreate(&e0));
+  HIP_OK(c, hipEventCreate(&e1));
+  double best = 1e300;
+  int bestL = c->lanes;
+  for (int L : {4, 8, 16, 32, 64}) {
+    launch_force(L, c->D, c->d_T, c->cur, c->par, 0, F_ALLTERMS, false, nullptr, 0, nb, c->stream);
+    (void)hipEventRecord(e0, c->stream);
+    for (int k = 0; k < 5; k++)
+      launch_force(L, c->D, c->d_T, c->cur, c->par, 0, F_ALLTERMS, false, nullptr, 0, nb, c->stream);
+    (void)hipEventRecord(e1, c->stream);
+    HIP_OK(c, hipEventSynchronize(e1));
+    float ms = 0.0f;
+    HIP_OK(c, hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best) { best = ms; bestL = L; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  HIP_OK(c, hipGetLastError());
+  c->lanes = bestL;
+  if (step_lanes) *step_lanes = bestL;
+  return SOPG_OK;
+}
+
 int sopg_run(sopg_ctx* c, long nsteps, const double* noise, sopg_record* rec, long rec_cap, long* nrec) {
   if (!c) return SOPG_ERR_ARG;
   if (nrec) *nrec = 0;
   if (nsteps < 0) return fail(c, SOPG_ERR_ARG, "nsteps < 0");
-  int rc = commit(c);
+  int rc = ensure_device(c);
   if (rc) return rc;
-  if (nsteps == 0) return SOPG_OK;
-  const bool replay = chain_has_replay(c);
+  if (nsteps == 0 || sopg_nchains(c) == 0) return SOPG_OK;
+  const bool replay = any_replay(c);
   if (replay && !noise) return fail(c, SOPG_ERR_ARG, "replayed-noise chains need the run's noise");
   if (!replay && noise) return fail(c, SOPG_ERR_ARG, "all chains use the device generator: noise must be NULL");
   if ((rc = upload_chainp(c))) return rc;
-  const int nb = c->nb, nc = (int)c->chains.size();
+  const int nb = c->D.nb, nc = (int)c->pool[P_CHAIN].cap;
   hipStream_t s = c->stream;
   if (noise) {
+    // caller order: per step, active chains by id, 3N each -> device layout 3 * bead index
     if ((rc = ensure_noise(c, (size_t)nsteps * 3 * nb))) return rc;
-    HIP_OK(c, hipMemcpyAsync(c->d_noise, noise, sizeof(double) * (size_t)nsteps * 3 * nb, hipMemcpyHostToDevice, s));
+    std::vector<double> lay((size_t)nsteps * 3 * nb, 0.0);
+    size_t q = 0;
+    for (long r = 0; r < nsteps; r++)
+      for (auto& h : c->chains) {
+        if (!h.active) continue;
+        std::memcpy(&lay[(size_t)r * 3 * nb + 3 * h.at[P_BEAD]], noise + q, sizeof(double) * 3 * h.n);
+        q += 3 * (size_t)h.n;
+      }
+    HIP_OK(c, hipMemcpyAsync(c->d_noise, lay.data(), sizeof(double) * lay.size(), hipMemcpyHostToDevice, s));
+    HIP_OK(c, hipStreamSynchronize(s));
   }
   auto due = [&](long r) {
     bool any = false, cell = false;
     for (auto& h : c->chains)
-      if (h.p.cutoff != SOPG_CUTOFF_NONE && (h.inlup + r) % h.p.nnlup == 0) { any = true; cell |= h.p.cutoff == SOPG_CUTOFF_CELL; }
+      if (h.active && h.p.cutoff != SOPG_CUTOFF_NONE && (h.inlup + r) % h.p.nnlup == 0) {
+        any = true;
+        cell |= h.p.cutoff == SOPG_CUTOFF_CELL;
+      }
     return std::make_pair(any, cell);
   };
   auto records = [&](long r) {
     for (auto& h : c->chains)
-      if ((h.iup + r) % h.p.nup == 0) return true;
+      if (h.active && (h.iup + r) % h.p.nup == 0) return true;
     return false;
   };
   long nrs = 0;
   for (long r = 0; r < nsteps; r++) nrs += records(r);
   if ((size_t)nrs * nc * kRed > c->rec_cap_d) {
-    if (c->d_rec) {
-      (void)hipFree(c->d_rec);
-      c->allocs.erase(std::find(c->allocs.begin(), c->allocs.end(), (void*)c->d_rec));
-      c->d_rec = nullptr;
-    }
+    HIP_OK(c, hipStreamSynchronize(s));
+    dfree(c, c->d_rec);
+    c->d_rec = nullptr;
     c->rec_cap_d = 0;
     if ((rc = dalloc(c, &c->d_rec, (size_t)nrs * nc * kRed))) return rc;
     c->rec_cap_d = (size_t)nrs * nc * kRed;
@@ -1024,10 +1322,15 @@ int sopg_run(sopg_ctx* c, long nsteps, const double* noise, sopg_record* rec, lo
   if (dr.first) {
     pe = prof_begin(c);
     if (dr.second) launch_cellcoords(D, c->cur, 0, nb, s);
-    launch_rebuild(c->list_lanes, D, c->d_T, c->cur, c->par, 0, -1, 0, nb, s);
+    launch_rebuild(c->list_lanes, D, c->d_T, c->cur, c->par, 0, -1, 0, nb, 0, nc, c->binned, s);
     prof_end(c, SOPG_KERNEL_REBUILD, pe);
   }
-  if (!c->pl_fresh) {
+  // pair lists at the current positions; rerunning the filter on a chain whose
+  // list is fresh reproduces it, so one launch covers every chain
+  bool fresh = true;
+  for (auto& h : c->chains)
+    if (h.active && !h.pl_fresh) fresh = false;
+  if (!fresh) {
     pe = prof_begin(c);
     launch_pairfilter(c->list_lanes, D, c->d_T, c->cur, c->par, 0, nb, s);
     prof_end(c, SOPG_KERNEL_PAIRLIST, pe);
@@ -1053,14 +1356,14 @@ int sopg_run(sopg_ctx* c, long nsteps, const double* noise, sopg_record* rec, lo
       if (dr.first) {
         pe = prof_begin(c);
         if (dr.second) launch_cellcoords(D, c->cur, 0, nb, s);
-        launch_rebuild(c->list_lanes, D, c->d_T, c->cur, c->par, (int)(r + 1), -1, 0, nb, s);
+        launch_rebuild(c->list_lanes, D, c->d_T, c->cur, c->par, (int)(r + 1), -1, 0, nb, 0, nc, c->binned, s);
         prof_end(c, SOPG_KERNEL_REBUILD, pe);
       }
       c->cur ^= 1;
     }
   }
   HIP_OK(c, hipGetLastError());
-  c->pl_fresh = false;
+  for (auto& h : c->chains) h.pl_fresh = false;
   std::vector<double> hr(rec_steps.size() * nc * kRed);
   if (!hr.empty())
     HIP_OK(c, hipMemcpyAsync(hr.data(), c->d_rec, sizeof(double) * hr.size(), hipMemcpyDeviceToHost, s));
@@ -1070,30 +1373,30 @@ int sopg_run(sopg_ctx* c, long nsteps, const double* noise, sopg_record* rec, lo
   long nr = 0;
   for (size_t q = 0; q < rec_steps.size(); q++) {
     const long r = rec_steps[q];
-    for (int ic = 0; ic < nc; ic++) {
+    for (int ic = 0; ic < (int)c->chains.size(); ic++) {
       HostChain& h = c->chains[ic];
-      if ((h.iup + r) % h.p.nup != 0) continue;
+      if (!h.active || (h.iup + r) % h.p.nup != 0) continue;
       if (rec && nr < rec_cap) {
         const double* v = &hr[(q * nc + ic) * kRed];
         sopg_record& o = rec[nr];
-        const ChainP& P = c->cp[ic];
         o.chain = ic;
         o.istep = h.istep + (double)r;
         o.T = h.p.T;
-        o.kinT = kin_temp(h.p, v[R_V2], P.n);
+        o.kinT = kin_temp(h.p, v[R_V2], h.n);
         o.e_bnd = v[R_EBND] * -c->T.e_bnd_coeff;
         o.e_ang_ss = v[R_EANG];
         o.e_vdw_rr = v[R_EATT] + v[R_EREP];
         o.rna_etot = o.e_bnd + o.e_ang_ss + o.e_vdw_rr;
         o.contct_nat = (int)v[R_CONT];
-        o.Q = (double)o.contct_nat / (double)P.natt;
-        o.rgsq = v[R_RG] / ((double)P.n * (double)P.n);
+        o.Q = (double)o.contct_nat / (double)h.ai.size();
+        o.rgsq = v[R_RG] / ((double)h.n * (double)h.n);
       }
       nr++;
     }
   }
   if (nrec) *nrec = nr;
   for (auto& h : c->chains) {
+    if (!h.active) continue;
     h.istep += (double)nsteps;
     h.inlup = (int)(((long)h.inlup - 1 + nsteps) % h.p.nnlup) + 1;
     h.iup = (int)(((long)h.iup - 1 + nsteps) % h.p.nup) + 1;

@@ -6,7 +6,7 @@
 //   unc_pos x/y/z  two buffers (the step kernel reads one, writes the next)
 //   pos x/y/z      two buffers (wrapped copy, output only)
 //   vel, force, incr x/y/z
-//   per-bead incidence lists (CSR, in the reference's list order):
+//   per-bead incidence lists ([beg, end) ranges, in the reference's list order):
 //     bonds, angles, native (attractive) contacts, and -- for explicit
 //     non-native master lists -- non-native contacts
 //   per-bead non-native neighbor slots: cap words per bead + count
@@ -40,36 +40,55 @@ struct ChainP {
   double thr_att;      // tol * max lj_nat_pdb_dist2 of the chain
   uint64_t seed, noise_base;  // Philox key, force evaluations done so far
   int sim_type;        // 1 underdamped, 2 overdamped
+  double gcell;        // binned neighbor-list rebuild: grid edge >= largest 3.2 sigma_rep of the chain
+};
+
+// Per-chain grid of the binned rebuild, recomputed at every rebuild.  A chain
+// uses it only when its extent keeps every pair in the exact min-image fast
+// path (|d| <= boxl/4, or a cell-index difference <= ncell/4 for the cell
+// list): then every pair the criterion keeps lies in adjacent grid cells.
+constexpr int kGridDim = 24;                       // cells per axis, at most
+constexpr int kGridCells = kGridDim * kGridDim * kGridDim;
+struct GridP {
+  double ox, oy, oz;   // origin (bead coordinate, or cell index for the cell list)
+  double inv;          // 1 / grid cell edge (1 for the cell list: cells are the reference's)
+  int nx, ny, nz;
+  int use;             // 1: binned rebuild, 0: full candidate scan
 };
 
 // Per-bead reduction fields (red[g * kRed + field]).
 enum { R_EBND = 0, R_EANG, R_EATT, R_EREP, R_CONT, R_RG, R_V2, R_NLA, R_PLA, R_NLR, R_PLR, kRed = 12 };
 
 struct Dev {
-  int nb;    // total beads
+  int nb;    // bead high-water mark: beads [0, nb) are live or free (chain -1)
   int cap;   // non-native slots per bead
   double *ux[2], *uy[2], *uz[2];
   double *px[2], *py[2], *pz[2];
   double *vx, *vy, *vz, *fx, *fy, *fz, *ix, *iy, *iz;
-  int* chain;
+  int* chain;   // chain id of each bead, -1 for a free bead
   int8_t* type;
   // bonds: incidence code = bond << 1 | side (0: bead is i, 1: bead is j)
-  int *bond_ptr, *bond_code, *bi, *bj; double* br0;
+  int *bond_beg, *bond_end, *bond_code, *bi, *bj; double* br0;
   // angles: code = angle << 1 | side (0: bead is i, 1: bead is k)
-  int *ang_ptr, *ang_code, *gi, *gk;
+  int *ang_beg, *ang_end, *ang_code, *gi, *gk;
   // natives: code = master << 1 | side; partner = the other bead (global)
-  int *att_ptr, *att_code, *att_partner; uint8_t* att_flag;  // flag: bit0 NL, bit1+p PL(parity p)
+  int *att_beg, *att_end, *att_code, *att_partner; uint8_t* att_flag;  // flag: bit0 NL, bit1+p PL(parity p)
   int *am_i, *am_j; int8_t* am_t;  // am_t = itype | jtype << 2
   double *am_r2, *am_r6, *am_r12, *am_nlrc2, *am_plrc2; float* am_rf;
   uint8_t *am_nl, *am_pl0, *am_pl1;
   // explicit non-natives
-  int *rep_ptr, *rep_code, *rm_i, *rm_j; int8_t* rm_t;
+  int *rep_beg, *rep_end, *rep_code, *rm_i, *rm_j; int8_t* rm_t;
   // non-native neighbor slots
   uint32_t* slot; int* slot_k; int* cnt;
   double *cx, *cy, *cz;  // cell coordinates (cell list)
   double* red;           // per-bead reduction fields
   int* err;              // bit0: slot overflow
   ChainP* cp;
+  // binned rebuild: per-chain grid (GridP), kGridCells cells per chain
+  GridP* grid;
+  int *cell_cnt, *cell_start, *cell_fill;  // per cell
+  int *bead_cell, *cell_beads;             // per bead: its cell, and beads sorted by cell
+  int bitmap_words;                        // ceil(max chain size / 32)
 };
 
 }  // namespace sopg

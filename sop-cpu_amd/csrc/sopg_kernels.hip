@@ -100,7 +100,7 @@ __device__ __forceinline__ bool cell_adjacent(const Dev& D, const ChainP& P, int
 // Cell index per bead: floor(unc_pos / lcell) (cell_list.cpp:24-31).
 __global__ void k_cellcoords(Dev D, int cur, int g0, int g1) {
   const int g = g0 + blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= g1) return;
+  if (g >= g1 || D.chain[g] < 0) return;  // past the end, or a free bead
   const ChainP& P = D.cp[D.chain[g]];
   D.cx[g] = floor((cur ? D.ux[1] : D.ux[0])[g] / P.lcell);
   D.cy[g] = floor((cur ? D.uy[1] : D.uy[0])[g] / P.lcell);
@@ -115,13 +115,14 @@ __global__ void __launch_bounds__(256) k_rebuild(Dev D, const Tables* __restrict
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int g = g0 + t / L;
   const int l = t & (L - 1);
-  if (g >= g1) return;
+  if (g >= g1 || D.chain[g] < 0) return;  // past the end, or a free bead
   const ChainP& P = D.cp[D.chain[g]];
   int crit = mode;
   if (mode < 0) {
     if (P.cutoff == SOPG_CUTOFF_NONE || (P.phase + rel) % P.nnlup != 0) return;
     crit = P.cutoff;
   }
+  if (D.grid[D.chain[g]].use) return;  // rebuilt by k_rebuild_grid
   const double* ux = cur ? D.ux[1] : D.ux[0];
   const double* uy = cur ? D.uy[1] : D.uy[0];
   const double* uz = cur ? D.uz[1] : D.uz[0];
@@ -129,7 +130,7 @@ __global__ void __launch_bounds__(256) k_rebuild(Dev D, const Tables* __restrict
   uint8_t* aplw = par ? D.am_pl1 : D.am_pl0;
 
   // native contacts (neighbor_list.cpp:17-67 / cell_list.cpp:16-57)
-  const int a0 = D.att_ptr[g], a1 = D.att_ptr[g + 1];
+  const int a0 = D.att_beg[g], a1 = D.att_end[g];
   for (int q = a0 + l; q < a1; q += L) {
     const int code = D.att_code[q], k = code >> 1, side = code & 1, o = D.att_partner[q];
     const int i = side ? o : g, j = side ? g : o;
@@ -184,7 +185,7 @@ __global__ void __launch_bounds__(256) k_rebuild(Dev D, const Tables* __restrict
       cnt += __popcll(m);
     }
   } else {
-    const int r0 = D.rep_ptr[g], r1 = D.rep_ptr[g + 1];
+    const int r0 = D.rep_beg[g], r1 = D.rep_end[g];
     for (int base = r0; base < r1; base += L) {
       const int q = base + l;
       bool keep = false;
@@ -220,13 +221,285 @@ __global__ void __launch_bounds__(256) k_rebuild(Dev D, const Tables* __restrict
   }
 }
 
+// ------------------------------------------------------- binned rebuild
+//
+// Same lists as k_rebuild, with O(neighbors) candidates per bead instead of
+// O(N): beads are binned into a per-chain grid; a wave per bead tests the 27
+// neighbouring cells, marks accepted partners in an LDS bitmap (one bit per
+// chain bead) and emits the bitmap in ascending order -- the reference's
+// lexicographic order -- with a wave prefix sum.  Chains whose extent leaves
+// the exact min-image fast path keep the full scan of k_rebuild.
+
+__device__ __forceinline__ bool rebuild_due(const ChainP& P, int rel, int mode, int& crit) {
+  crit = mode;
+  if (mode >= 0) return true;
+  crit = P.cutoff;
+  return P.cutoff != SOPG_CUTOFF_NONE && (P.phase + rel) % P.nnlup == 0;
+}
+
+// Per-chain extent and grid; one block per chain.  Chains of more than maxn
+// beads (whose LDS bitmap would not fit) keep the full scan.
+__global__ void __launch_bounds__(256) k_grid(Dev D, int cur, int rel, int mode, int c0, int maxn) {
+  const int c = c0 + blockIdx.x;
+  const ChainP& P = D.cp[c];
+  GridP& G = D.grid[c];
+  int crit;
+  if (!rebuild_due(P, rel, mode, crit)) return;
+  if (P.n > maxn || P.rep_explicit || crit == SOPG_CUTOFF_NONE) {
+    if (threadIdx.x == 0) G.use = 0;
+    return;
+  }
+  const bool cell = crit == SOPG_CUTOFF_CELL;
+  const double* ax = cell ? D.cx : (cur ? D.ux[1] : D.ux[0]);
+  const double* ay = cell ? D.cy : (cur ? D.uy[1] : D.uy[0]);
+  const double* az = cell ? D.cz : (cur ? D.uz[1] : D.uz[0]);
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int b = threadIdx.x; b < P.n; b += blockDim.x) {
+    const int g = P.off + b;
+    const double v[3] = {ax[g], ay[g], az[g]};
+    for (int a = 0; a < 3; a++) { lo[a] = fmin(lo[a], v[a]); hi[a] = fmax(hi[a], v[a]); }
+  }
+  __shared__ double sh[6][256];
+  for (int a = 0; a < 3; a++) { sh[a][threadIdx.x] = lo[a]; sh[3 + a][threadIdx.x] = hi[a]; }
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int a = 0; a < 3; a++) {
+        sh[a][threadIdx.x] = fmin(sh[a][threadIdx.x], sh[a][threadIdx.x + w]);
+        sh[3 + a][threadIdx.x] = fmax(sh[3 + a][threadIdx.x], sh[3 + a][threadIdx.x + w]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  // every pair difference stays below half the period: the min image is the
+  // plain difference, so all kept pairs are in adjacent cells of a local grid
+  const double limit = 0.49 * (cell ? P.ncell : P.boxl);
+  const double inv = cell ? 1.0 : 1.0 / P.gcell;
+  int dims[3];
+  bool use = true;
+  for (int a = 0; a < 3; a++) {
+    const double ext = sh[3 + a][0] - sh[a][0];
+    use = use && ext <= limit && isfinite(ext);
+    dims[a] = use ? (int)floor(ext * inv) + 1 : 0;
+    use = use && dims[a] <= kGridDim;
+  }
+  G.ox = sh[0][0]; G.oy = sh[1][0]; G.oz = sh[2][0];
+  G.inv = inv;
+  G.nx = dims[0]; G.ny = dims[1]; G.nz = dims[2];
+  G.use = use ? 1 : 0;
+}
+
+__device__ __forceinline__ int grid_coord(double v, double o, double inv, int n) {
+  int k = (int)floor((v - o) * inv);
+  return k < 0 ? 0 : (k >= n ? n - 1 : k);
+}
+
+// Counting sort of beads into cells: count, scan, fill.
+__global__ void k_bin_count(Dev D, int cur, int rel, int mode, int g0, int g1) {
+  const int g = g0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= g1) return;
+  const int c = D.chain[g];
+  if (c < 0) return;
+  const ChainP& P = D.cp[c];
+  const GridP& G = D.grid[c];
+  int crit;
+  if (!rebuild_due(P, rel, mode, crit) || !G.use) return;
+  const bool cell = crit == SOPG_CUTOFF_CELL;
+  const double x = cell ? D.cx[g] : (cur ? D.ux[1] : D.ux[0])[g];
+  const double y = cell ? D.cy[g] : (cur ? D.uy[1] : D.uy[0])[g];
+  const double z = cell ? D.cz[g] : (cur ? D.uz[1] : D.uz[0])[g];
+  const int id = (grid_coord(z, G.oz, G.inv, G.nz) * G.ny + grid_coord(y, G.oy, G.inv, G.ny)) * G.nx +
+                 grid_coord(x, G.ox, G.inv, G.nx);
+  D.bead_cell[g] = id;
+  atomicAdd(&D.cell_cnt[(size_t)c * kGridCells + id], 1);
+}
+
+__global__ void __launch_bounds__(256) k_bin_scan(Dev D, int rel, int mode, int c0) {
+  const int c = c0 + blockIdx.x;
+  const ChainP& P = D.cp[c];
+  const GridP& G = D.grid[c];
+  int crit;
+  if (!rebuild_due(P, rel, mode, crit) || !G.use) return;
+  const int ncell = G.nx * G.ny * G.nz;
+  int* cnt = D.cell_cnt + (size_t)c * kGridCells;
+  int* st = D.cell_start + (size_t)c * kGridCells;
+  int* fl = D.cell_fill + (size_t)c * kGridCells;
+  // block-wide exclusive scan in chunks of 256 cells
+  __shared__ int sh[256];
+  int carry = 0;
+  for (int base = 0; base < ncell; base += 256) {
+    const int k = base + threadIdx.x;
+    const int v = k < ncell ? cnt[k] : 0;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+      const int add = (int)threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
+      __syncthreads();
+      sh[threadIdx.x] += add;
+      __syncthreads();
+    }
+    if (k < ncell) {
+      st[k] = carry + sh[threadIdx.x] - v;
+      fl[k] = 0;
+      cnt[k] = 0;  // ready for the next rebuild
+    }
+    carry += sh[255];
+    __syncthreads();
+  }
+}
+
+__global__ void k_bin_fill(Dev D, int rel, int mode, int g0, int g1) {
+  const int g = g0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= g1) return;
+  const int c = D.chain[g];
+  if (c < 0) return;
+  const ChainP& P = D.cp[c];
+  int crit;
+  if (!rebuild_due(P, rel, mode, crit) || !D.grid[c].use) return;
+  const size_t cb = (size_t)c * kGridCells + D.bead_cell[g];
+  const int at = D.cell_start[cb] + atomicAdd(&D.cell_fill[cb], 1);
+  D.cell_beads[P.off + at] = g - P.off;
+}
+
+// One wave (64 lanes) per bead, four beads per 256-thread block.
+__global__ void __launch_bounds__(256) k_rebuild_grid(Dev D, const Tables* __restrict__ T, int cur, int par, int rel,
+                                                      int mode, int g0, int g1) {
+  extern __shared__ uint32_t bitmaps[];
+  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int g = g0 + blockIdx.x * 4 + wave;
+  uint32_t* bm = bitmaps + (size_t)wave * D.bitmap_words;
+  bool active = g < g1;
+  int crit = 0;
+  const ChainP* Pp = nullptr;
+  if (active) active = D.chain[g] >= 0;
+  if (active) {
+    Pp = &D.cp[D.chain[g]];
+    active = rebuild_due(*Pp, rel, mode, crit) && D.grid[D.chain[g]].use;
+  }
+  const double* ux = cur ? D.ux[1] : D.ux[0];
+  const double* uy = cur ? D.uy[1] : D.uy[0];
+  const double* uz = cur ? D.uz[1] : D.uz[0];
+  int words = 0, b = 0, a0 = 0, a1 = 0;
+  if (active) {
+    const ChainP& P = *Pp;
+    b = g - P.off;
+    words = (P.n + 31) >> 5;
+    a0 = D.att_beg[g];
+    a1 = D.att_end[g];
+    const double boxl = P.boxl, qbox = P.qbox;
+    uint8_t* aplw = par ? D.am_pl1 : D.am_pl0;
+    // natives: NL / PL flags, exactly as k_rebuild
+    for (int q = a0 + l; q < a1; q += 64) {
+      const int code = D.att_code[q], k = code >> 1, side = code & 1, o = D.att_partner[q];
+      const int i = side ? o : g, j = side ? g : o;
+      const double dx = mimg(ux[j] - ux[i], boxl, qbox);
+      const double dy = mimg(uy[j] - uy[i], boxl, qbox);
+      const double dz = mimg(uz[j] - uz[i], boxl, qbox);
+      const double d2 = dx * dx + dy * dy + dz * dz;
+      bool nl, pl;
+      if (crit == SOPG_CUTOFF_NEIGHBOR) { nl = d2 < D.am_nlrc2[k]; pl = nl && d2 < D.am_plrc2[k]; }
+      else { nl = cell_adjacent(D, P, i, j); pl = nl && d2 < D.am_plrc2[k]; }
+      D.att_flag[q] = (uint8_t)((nl ? 1 : 0) | (pl ? (2 << par) : 0));
+      if (side == 0) { D.am_nl[k] = nl; aplw[k] = pl; }
+    }
+    for (int w = l; w < words; w += 64) bm[w] = 0u;
+  }
+  __syncthreads();
+  if (active) {
+    // mark every partner of the 27 neighbouring cells that passes the criterion
+    const ChainP& P = *Pp;
+    const GridP& G = D.grid[D.chain[g]];
+    const int id = D.bead_cell[g];
+    const int cxi = id % G.nx, cyi = (id / G.nx) % G.ny, czi = id / (G.nx * G.ny);
+    const size_t cbase = (size_t)D.chain[g] * kGridCells;
+    const double boxl = P.boxl, qbox = P.qbox;
+    for (int dz = -1; dz <= 1; dz++) {
+      const int z = czi + dz;
+      if (z < 0 || z >= G.nz) continue;
+      for (int dy = -1; dy <= 1; dy++) {
+        const int y = cyi + dy;
+        if (y < 0 || y >= G.ny) continue;
+        for (int dx = -1; dx <= 1; dx++) {
+          const int x = cxi + dx;
+          if (x < 0 || x >= G.nx) continue;
+          const size_t cc = cbase + (size_t)((z * G.ny + y) * G.nx + x);
+          const int s0 = D.cell_start[cc], s1 = s0 + D.cell_fill[cc];
+          for (int s = s0 + l; s < s1; s += 64) {
+            const int c = D.cell_beads[P.off + s];
+            if (!(c - b > 2 || b - c > 2)) continue;
+            const int gc = P.off + c;
+            const int i = gc < g ? gc : g, j = gc < g ? g : gc;
+            bool keep;
+            if (crit == SOPG_CUTOFF_NEIGHBOR) {
+              const int it = D.type[i], jt = D.type[j];
+              const double ddx = mimg(ux[j] - ux[i], boxl, qbox);
+              const double ddy = mimg(uy[j] - uy[i], boxl, qbox);
+              const double ddz = mimg(uz[j] - uz[i], boxl, qbox);
+              keep = ddx * ddx + ddy * ddy + ddz * ddz < T->nlrc2[it][jt];
+            } else {
+              keep = cell_adjacent(D, P, i, j);
+            }
+            if (keep) atomicOr(&bm[c >> 5], 1u << (c & 31));
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (active) {
+    // natives are not in the non-native master list
+    for (int q = a0 + l; q < a1; q += 64) {
+      const int p = D.att_partner[q] - Pp->off;
+      atomicAnd(&bm[p >> 5], ~(1u << (p & 31)));
+    }
+  }
+  __syncthreads();
+  if (!active) return;
+  const ChainP& P = *Pp;
+  const double boxl = P.boxl, qbox = P.qbox;
+  uint32_t* S = D.slot + (size_t)g * D.cap;
+  int cnt = 0;
+  for (int base = 0; base < words; base += 64) {
+    const int w = base + l;
+    uint32_t bits = w < words ? bm[w] : 0u;
+    const int pc = __popc(bits);
+    int incl = pc;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int t = __shfl_up(incl, o, 64);
+      if (l >= o) incl += t;
+    }
+    int idx = cnt + incl - pc;
+    while (bits) {
+      const int c = w * 32 + __builtin_ctz(bits);
+      bits &= bits - 1u;
+      const int gc = P.off + c;
+      const int side = gc < g ? 1 : 0;
+      const int i = side ? gc : g, j = side ? g : gc;
+      const int it = D.type[i], jt = D.type[j];
+      const double ddx = mimg(ux[j] - ux[i], boxl, qbox);
+      const double ddy = mimg(uy[j] - uy[i], boxl, qbox);
+      const double ddz = mimg(uz[j] - uz[i], boxl, qbox);
+      const bool pl = ddx * ddx + ddy * ddy + ddz * ddz < T->plrc2[it][jt];
+      if (idx < D.cap)
+        S[idx] = slot_make((uint32_t)c, (uint32_t)side, (uint32_t)it, (uint32_t)jt) | (pl ? (1u << (30 + par)) : 0u);
+      idx++;
+    }
+    cnt += __shfl(incl, 63, 64);
+  }
+  if (l == 0) {
+    D.cnt[g] = cnt < D.cap ? cnt : D.cap;
+    if (cnt > D.cap) atomicOr(D.err, 1);
+  }
+}
+
 // Pair-list filter at the current positions (sop.cpp:473-553), parity par.
 template <int L>
 __global__ void __launch_bounds__(256) k_pairfilter(Dev D, const Tables* __restrict__ T, int cur, int par, int g0, int g1) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int g = g0 + t / L;
   const int l = t & (L - 1);
-  if (g >= g1) return;
+  if (g >= g1 || D.chain[g] < 0) return;  // past the end, or a free bead
   const ChainP& P = D.cp[D.chain[g]];
   const double* ux = cur ? D.ux[1] : D.ux[0];
   const double* uy = cur ? D.uy[1] : D.uy[0];
@@ -234,7 +507,7 @@ __global__ void __launch_bounds__(256) k_pairfilter(Dev D, const Tables* __restr
   const double boxl = P.boxl, qbox = P.qbox;
   uint8_t* aplw = par ? D.am_pl1 : D.am_pl0;
   const uint8_t pbit = (uint8_t)(2 << par);
-  const int a0 = D.att_ptr[g], a1 = D.att_ptr[g + 1];
+  const int a0 = D.att_beg[g], a1 = D.att_end[g];
   for (int q = a0 + l; q < a1; q += L) {
     const int code = D.att_code[q], k = code >> 1, side = code & 1, o = D.att_partner[q];
     const uint8_t f = D.att_flag[q];
@@ -269,7 +542,7 @@ __global__ void __launch_bounds__(256) k_pairfilter(Dev D, const Tables* __restr
 // Position half of underdamped_iteration (sop.cpp:288-310): incr, pos, unc_pos.
 __global__ void k_posupdate(Dev D, int src, int dst, int g0, int g1) {
   const int g = g0 + blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= g1) return;
+  if (g >= g1 || D.chain[g] < 0) return;  // past the end, or a free bead
   const ChainP& P = D.cp[D.chain[g]];
   // underdamped: incr = a1 v + a2 F (sop.cpp:292-294); overdamped: incr = a5 F (sop.cpp:339-341)
   const bool od = P.sim_type == 2;
@@ -288,7 +561,7 @@ __global__ void k_posupdate(Dev D, int src, int dst, int g0, int g1) {
 // Velocity half of underdamped_iteration (sop.cpp:316-328).
 __global__ void k_velupdate(Dev D, int g0, int g1) {
   const int g = g0 + blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= g1) return;
+  if (g >= g1 || D.chain[g] < 0) return;  // past the end, or a free bead
   const ChainP& P = D.cp[D.chain[g]];
   if (!P.vel_on || P.sim_type == 2) return;
   D.vx[g] = P.a3 * D.ix[g] + P.a4 * D.fx[g];
@@ -316,7 +589,7 @@ __global__ void __launch_bounds__(256) k_force(Dev D, const Tables* __restrict__
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int g = g0 + t / L;
   const int l = t & (L - 1);
-  if (g >= g1) return;
+  if (g >= g1 || D.chain[g] < 0) return;  // past the end, or a free bead
   const ChainP& P = D.cp[D.chain[g]];
   const double* __restrict__ ux = cur ? D.ux[1] : D.ux[0];
   const double* __restrict__ uy = cur ? D.uy[1] : D.uy[0];
@@ -362,7 +635,7 @@ __global__ void __launch_bounds__(256) k_force(Dev D, const Tables* __restrict__
 
   // FENE bonds (energy.cpp:402-447): F_i -= f, F_j += f
   if (flags & F_BOND) {
-    const int p0 = D.bond_ptr[g], p1 = D.bond_ptr[g + 1];
+    const int p0 = D.bond_beg[g], p1 = D.bond_end[g];
     for (int base = p0; base < p1; base += L) {
       const int q = base + l;
       const bool has = q < p1;
@@ -386,7 +659,7 @@ __global__ void __launch_bounds__(256) k_force(Dev D, const Tables* __restrict__
   }
   // soft-sphere (i, i+2) repulsion (energy.cpp:354-400): F_i -= f, F_k += f
   if (flags & F_ANGLE) {
-    const int p0 = D.ang_ptr[g], p1 = D.ang_ptr[g + 1];
+    const int p0 = D.ang_beg[g], p1 = D.ang_end[g];
     for (int base = p0; base < p1; base += L) {
       const int q = base + l;
       const bool has = q < p1;
@@ -409,7 +682,7 @@ __global__ void __launch_bounds__(256) k_force(Dev D, const Tables* __restrict__
   }
   // native LJ 12-6 over the pair list (energy.cpp:274-311): F_i += f, F_j -= f
   if (flags & F_VDW) {
-    const int p0 = D.att_ptr[g], p1 = D.att_ptr[g + 1];
+    const int p0 = D.att_beg[g], p1 = D.att_end[g];
     uint8_t* aplw = pw ? D.am_pl1 : D.am_pl0;
     for (int base = p0; base < p1; base += L) {
       const int q = base + l;
@@ -534,7 +807,7 @@ __global__ void __launch_bounds__(256) k_force(Dev D, const Tables* __restrict__
 enum { OBS_PHYS = 1, OBS_COUNTS = 2 };
 __global__ void k_observe(Dev D, int cur, int par, int rel, int all, int what, int g0, int g1) {
   const int g = g0 + blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= g1) return;
+  if (g >= g1 || D.chain[g] < 0) return;  // past the end, or a free bead
   const ChainP& P = D.cp[D.chain[g]];
   if (!all && (P.iup0 + rel) % P.nup != 0) return;
   const double* ux = cur ? D.ux[1] : D.ux[0];
@@ -544,7 +817,7 @@ __global__ void k_observe(Dev D, int cur, int par, int rel, int all, int what, i
   const uint8_t* apl = par ? D.am_pl1 : D.am_pl0;
   double* r = D.red + (size_t)g * kRed;
   int cont = 0, nla = 0, pla = 0, nlr = 0, plr = 0;
-  const int a0 = D.att_ptr[g], a1 = D.att_ptr[g + 1];
+  const int a0 = D.att_beg[g], a1 = D.att_end[g];
   for (int q = a0; q < a1; q++) {
     const int code = D.att_code[q];
     if (code & 1) continue;  // each native pair once, from its i bead
@@ -646,8 +919,22 @@ static void launch_force_L(const Dev& D, const Tables* T, int cur, int pr, int r
 void launch_cellcoords(const Dev& D, int cur, int g0, int g1, hipStream_t s) {
   hipLaunchKernelGGL(k_cellcoords, dim3(blocks_for(g1 - g0)), dim3(256), 0, s, D, cur, g0, g1);
 }
+// List rebuild of beads [g0, g1) = chains [c0, c0 + nc): the binned kernels for
+// chains whose grid applies (binned != 0), the full scan for the others.
+constexpr int kMaxBitmapWords = 4096;  // 4 waves x 16 KiB of LDS: chains up to 131072 beads
 void launch_rebuild(int L, const Dev& D, const Tables* T, int cur, int par, int rel, int mode, int g0, int g1,
-                    hipStream_t s) {
+                    int c0, int nc, int binned, hipStream_t s) {
+  const int words = D.bitmap_words < kMaxBitmapWords ? D.bitmap_words : kMaxBitmapWords;
+  hipLaunchKernelGGL(k_grid, dim3(nc), dim3(256), 0, s, D, cur, rel, mode, c0, binned ? 32 * words : 0);
+  if (binned) {
+    hipLaunchKernelGGL(k_bin_count, dim3(blocks_for(g1 - g0)), dim3(256), 0, s, D, cur, rel, mode, g0, g1);
+    hipLaunchKernelGGL(k_bin_scan, dim3(nc), dim3(256), 0, s, D, rel, mode, c0);
+    hipLaunchKernelGGL(k_bin_fill, dim3(blocks_for(g1 - g0)), dim3(256), 0, s, D, rel, mode, g0, g1);
+    Dev Dg = D;
+    Dg.bitmap_words = words;
+    hipLaunchKernelGGL(k_rebuild_grid, dim3((unsigned)((g1 - g0 + 3) / 4)), dim3(256),
+                       (size_t)4 * words * sizeof(uint32_t), s, Dg, T, cur, par, rel, mode, g0, g1);
+  }
 #define C_(X) launch_rebuild_L<X>(D, T, cur, par, rel, mode, g0, g1, s)
   SOPG_DISPATCH_L(L, C_)
 #undef C_

@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <unordered_map>
 #include <vector>
 
@@ -313,6 +314,49 @@ int sopgen_write_topology(int n, const double* xyz, const char* bonds_path, cons
     std::fclose(f);
   }
   return natt;
+}
+
+// Native contacts of a structure: pairs j-i>2 with distance < rcut, in (i,j)
+// lexicographic order, 1-based, with the distance exactly as the vdw file
+// carries it.  O(n) with a cell grid.  Returns the count (entries beyond cap
+// are counted, not written) or <0.
+long sopgen_natives(int n, const double* xyz, double rcut, int* oi, int* oj, double* orr, long cap) {
+  if (n < 1 || !xyz || !(rcut > 0)) return -1;
+  const double cs = rcut;
+  double lo[3] = {1e300, 1e300, 1e300};
+  for (int i = 0; i < n; i++)
+    for (int a = 0; a < 3; a++) lo[a] = std::fmin(lo[a], xyz[3 * i + a]);
+  std::unordered_map<uint64_t, std::vector<int>> cells;
+  auto key = [](int64_t a, int64_t b, int64_t c) {
+    return ((uint64_t)(a & 0x1FFFFF) << 42) | ((uint64_t)(b & 0x1FFFFF) << 21) | (uint64_t)(c & 0x1FFFFF);
+  };
+  std::vector<int64_t> ci(3 * (size_t)n);
+  for (int i = 0; i < n; i++) {
+    for (int a = 0; a < 3; a++) ci[3 * i + a] = (int64_t)std::floor((xyz[3 * i + a] - lo[a]) / cs);
+    cells[key(ci[3 * i], ci[3 * i + 1], ci[3 * i + 2])].push_back(i);
+  }
+  long cnt = 0;
+  std::vector<int> nb;
+  for (int i = 0; i < n; i++) {
+    nb.clear();
+    for (int64_t da = -1; da <= 1; da++)
+      for (int64_t db = -1; db <= 1; db++)
+        for (int64_t dc = -1; dc <= 1; dc++) {
+          auto it = cells.find(key(ci[3 * i] + da, ci[3 * i + 1] + db, ci[3 * i + 2] + dc));
+          if (it == cells.end()) continue;
+          for (int j : it->second)
+            if (j > i + 2) nb.push_back(j);
+        }
+    std::sort(nb.begin(), nb.end());
+    for (int j : nb) {
+      const double d = dist(xyz, i, j);
+      if (d < rcut) {
+        if (cnt < cap) { oi[cnt] = i + 1; oj[cnt] = j + 1; orr[cnt] = d; }
+        cnt++;
+      }
+    }
+  }
+  return cnt;
 }
 
 // "idx x y z" lines (coordinates or velocities, io.cpp:156-278 layout) at %.17g.

@@ -65,7 +65,20 @@ class Record(C.Structure):
                 ("rna_etot", C.c_double), ("Q", C.c_double), ("contct_nat", C.c_int), ("rgsq", C.c_double)]
 
 
-# every entry point of include/sopg.h and include/sopgen.h
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_double), ("max_chains", C.c_int), ("chunk", C.c_long), ("lanes", C.c_int),
+                ("capacity", C.c_int)]
+
+
+class JobInfo(C.Structure):
+    _fields_ = [("state", C.c_int), ("chain", C.c_int), ("nbead", C.c_long), ("nsteps", C.c_long),
+                ("steps_done", C.c_long), ("t_arrival", C.c_double), ("t_admit", C.c_double),
+                ("t_finish", C.c_double), ("footprint", C.c_double), ("nrec", C.c_long)]
+
+
+JOB_QUEUED, JOB_ACTIVE, JOB_DONE, JOB_REJECTED = 0, 1, 2, 3
+
+# every entry point of include/sopg.h, include/sopgen.h and include/sopserve.h
 EXPORTS = [
     "sopg_create", "sopg_destroy", "sopg_last_error", "sopg_default_tables", "sopg_set_tables",
     "sopg_set_lanes", "sopg_set_capacity", "sopg_add_chain", "sopg_set_chain_params", "sopg_put_state",
@@ -74,7 +87,11 @@ EXPORTS = [
     "sopg_underdamped_iteration", "sopg_overdamped_iteration", "sopg_calculate_observables", "sopg_get_list", "sopg_run",
     "sopg_set_step", "sopg_sync", "sopg_list_counts", "sopg_profile", "sopg_profile_read",
     "sopgen_saw", "sopgen_native", "sopgen_maxwell", "sopgen_check", "sopgen_write_topology",
-    "sopgen_write_xyz",
+    "sopgen_write_xyz", "sopgen_natives", "sopg_force_terms", "sopg_set_binned",
+    "sopg_get_lanes", "sopg_autotune", "sopg_reserve", "sopg_remove_chain", "sopg_get_step",
+    "sopg_server_create", "sopg_server_destroy", "sopg_server_last_error", "sopg_server_now",
+    "sopg_server_footprint", "sopg_server_submit", "sopg_server_poll", "sopg_server_job",
+    "sopg_server_job_records", "sopg_server_job_state", "sopg_server_release", "sopg_server_stats",
 ]
 
 _lib = None
@@ -96,6 +113,14 @@ def lib():
         L.sopg_set_tables.argtypes = [vp, C.POINTER(Tables)]
         L.sopg_set_lanes.argtypes = [vp, C.c_int, C.c_int]
         L.sopg_set_capacity.argtypes = [vp, C.c_int]
+        L.sopg_set_binned.argtypes = [vp, C.c_int]
+        L.sopg_get_lanes.argtypes = [vp, _IP, _IP]
+        L.sopg_autotune.argtypes = [vp, _IP]
+        L.sopg_reserve.argtypes = [vp, C.c_long, C.c_int]
+        L.sopg_remove_chain.argtypes = [vp, C.c_int]
+        L.sopg_get_step.argtypes = [vp, C.c_int, _DP, _IP, _IP]
+        L.sopg_force_terms.argtypes = [vp, C.c_int, C.c_int, vp]
+        L.sopg_overdamped_iteration.argtypes = [vp, C.c_int, vp]
         L.sopg_add_chain.argtypes = [vp, C.POINTER(Topology), C.POINTER(ChainParams), _IP]
         L.sopg_set_chain_params.argtypes = [vp, C.c_int, C.POINTER(ChainParams)]
         L.sopg_put_state.argtypes = [vp, C.c_int, vp, vp, vp, vp]
@@ -115,6 +140,22 @@ def lib():
         L.sopg_list_counts.argtypes = [vp, C.c_int, _IP, _IP, _IP, _IP]
         L.sopg_profile.argtypes = [vp, C.c_int]
         L.sopg_profile_read.argtypes = [vp, C.c_int, _DP, C.POINTER(C.c_long)]
+        L.sopg_server_create.argtypes = [C.c_int, C.POINTER(ServerConfig), C.POINTER(vp)]
+        L.sopg_server_destroy.argtypes = [vp]
+        L.sopg_server_last_error.argtypes = [vp]
+        L.sopg_server_last_error.restype = C.c_char_p
+        L.sopg_server_now.argtypes = [vp]
+        L.sopg_server_now.restype = C.c_double
+        L.sopg_server_footprint.argtypes = [vp, C.POINTER(Topology)]
+        L.sopg_server_footprint.restype = C.c_double
+        L.sopg_server_submit.argtypes = [vp, C.POINTER(Topology), C.POINTER(ChainParams), vp, vp, C.c_long,
+                                         C.c_double, C.POINTER(C.c_long)]
+        L.sopg_server_poll.argtypes = [vp, _IP, _IP]
+        L.sopg_server_job.argtypes = [vp, C.c_long, C.POINTER(JobInfo)]
+        L.sopg_server_job_records.argtypes = [vp, C.c_long, C.POINTER(Record), C.c_long, C.POINTER(C.c_long)]
+        L.sopg_server_job_state.argtypes = [vp, C.c_long, vp, vp, vp, vp]
+        L.sopg_server_release.argtypes = [vp, C.c_long]
+        L.sopg_server_stats.argtypes = [vp, _DP, C.POINTER(C.c_long), _DP]
         L.sopgen_saw.argtypes = [C.c_int, C.c_uint64, vp]
         L.sopgen_native.argtypes = [C.c_int, C.c_uint64, vp]
         L.sopgen_maxwell.argtypes = [C.c_int, C.c_double, C.c_uint64, vp]
@@ -122,6 +163,8 @@ def lib():
         L.sopgen_write_topology.argtypes = [C.c_int, vp, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p,
                                             C.c_double]
         L.sopgen_write_xyz.argtypes = [C.c_int, vp, C.c_char_p]
+        L.sopgen_natives.argtypes = [C.c_int, vp, C.c_double, _IP, _IP, _DP, C.c_long]
+        L.sopgen_natives.restype = C.c_long
         _lib = L
     return _lib
 
@@ -187,25 +230,24 @@ def chain_topology(xyz, rcut_nat=8.0):
     """Topology arrays of a generated chain (as the loaders would read its files):
     bonds (i,i+1), angles (i,i+1,i+2), natives = pairs j-i>2 with r < rcut (lexicographic).
     The r values are the %.17g round trip of the native distances, exactly like the files."""
-    X = np.asarray(xyz).reshape(-1, 3)
+    X = np.ascontiguousarray(np.asarray(xyz, dtype=np.float64).reshape(-1, 3))
     n = len(X)
-    d = lambda i, j: float(np.sqrt(((X[j] - X[i]) ** 2).sum()))
-    rt = lambda v: float("%.17g" % v)
     ib = np.arange(1, n, dtype=np.int32)
     jb = ib + 1
-    r0 = np.array([rt(_dist(X, i - 1, i)) for i in ib])
+    # same arithmetic as the generator: sqrt(dx*dx + dy*dy + dz*dz) in order
+    D = X[1:] - X[:-1]
+    r0 = np.sqrt(D[:, 0] * D[:, 0] + D[:, 1] * D[:, 1] + D[:, 2] * D[:, 2])
     ia = np.arange(1, n - 1, dtype=np.int32)
     ja = ia + 1
     ka = ia + 2
-    I, J = np.triu_indices(n, 3)
-    diff = X[J] - X[I]
-    # same arithmetic as the generator: sqrt(dx*dx + dy*dy + dz*dz) in order
-    dd = np.sqrt(diff[:, 0] * diff[:, 0] + diff[:, 1] * diff[:, 1] + diff[:, 2] * diff[:, 2])
-    nat = dd < rcut_nat
-    att_i = (I[nat] + 1).astype(np.int32)
-    att_j = (J[nat] + 1).astype(np.int32)
-    att_r = dd[nat].astype(np.float64)
-    return dict(n=n, ib=ib, jb=jb, r0=r0, ia=ia, ja=ja, ka=ka, att_i=att_i, att_j=att_j, att_r=att_r)
+    L = lib()
+    cnt = L.sopgen_natives(n, _ptr(X), rcut_nat, None, None, None, 0)
+    att_i = np.zeros(max(cnt, 1), np.int32)
+    att_j = np.zeros(max(cnt, 1), np.int32)
+    att_r = np.zeros(max(cnt, 1))
+    L.sopgen_natives(n, _ptr(X), rcut_nat, _ip(att_i), _ip(att_j), _dp(att_r), cnt)
+    return dict(n=n, ib=ib, jb=jb, r0=r0, ia=ia, ja=ja, ka=ka, att_i=att_i[:cnt], att_j=att_j[:cnt],
+                att_r=att_r[:cnt])
 
 
 def _dist(X, i, j):
@@ -238,10 +280,42 @@ def chain_params(T=0.6, zeta=50.0, h=0.05, boxl=500.0, ncell=55.0, lcell=None, c
     return p
 
 
+def make_topology(topo, implicit=True):
+    """Topology struct of a chain_topology() dict; returns (struct, arrays it points into)."""
+    t = Topology()
+    keep = []
+
+    def arr(a, dt):
+        a = np.ascontiguousarray(a, dtype=dt)
+        keep.append(a)
+        return a
+    n = int(topo["n"])
+    t.nbead = n
+    ib, jb, r0 = arr(topo["ib"], np.int32), arr(topo["jb"], np.int32), arr(topo["r0"], np.float64)
+    t.nbnd, t.ibnd, t.jbnd, t.r0bnd = len(ib), _ip(ib), _ip(jb), _dp(r0)
+    ia, ja, ka = arr(topo["ia"], np.int32), arr(topo["ja"], np.int32), arr(topo["ka"], np.int32)
+    t.nang, t.iang, t.jang, t.kang = len(ia), _ip(ia), _ip(ja), _ip(ka)
+    ai, aj, ar = arr(topo["att_i"], np.int32), arr(topo["att_j"], np.int32), arr(topo["att_r"], np.float64)
+    at = arr(topo.get("att_it", np.ones(len(ai))), np.int32)
+    ajt = arr(topo.get("att_jt", np.ones(len(ai))), np.int32)
+    t.ncon_att, t.att_i, t.att_j, t.att_it, t.att_jt, t.att_r = len(ai), _ip(ai), _ip(aj), _ip(at), _ip(ajt), _dp(ar)
+    if "rep_i" in topo and not implicit:
+        ri, rj = arr(topo["rep_i"], np.int32), arr(topo["rep_j"], np.int32)
+        rit = arr(topo.get("rep_it", np.ones(len(ri))), np.int32)
+        rjt = arr(topo.get("rep_jt", np.ones(len(ri))), np.int32)
+        t.ncon_rep, t.rep_i, t.rep_j, t.rep_it, t.rep_jt = len(ri), _ip(ri), _ip(rj), _ip(rit), _ip(rjt)
+    else:
+        t.ncon_rep = -1
+    if "bead_type" in topo:
+        bt = arr(topo["bead_type"], np.int32)
+        t.bead_type = _ip(bt)
+    return t, keep
+
+
 class Context:
     """One device context (a ragged batch of chains) on one GPU."""
 
-    def __init__(self, device=0, lanes=0, capacity=0, list_lanes=0):
+    def __init__(self, device=0, lanes=0, capacity=0, list_lanes=0, binned=True):
         self.L = lib()
         h = C.c_void_p()
         rc = self.L.sopg_create(device, C.byref(h))
@@ -253,6 +327,8 @@ class Context:
             self._chk(self.L.sopg_set_lanes(self.h, lanes, list_lanes))
         if capacity:
             self._chk(self.L.sopg_set_capacity(self.h, capacity))
+        if not binned:
+            self._chk(self.L.sopg_set_binned(self.h, 0))
 
     def _chk(self, rc):
         if rc != OK:
@@ -272,36 +348,31 @@ class Context:
 
     def add_chain(self, topo, params, implicit=True):
         """topo: dict from chain_topology() (+ optional 'rep_i','rep_j' explicit list)."""
-        t = Topology()
-        keep = []
-
-        def arr(a, dt):
-            a = np.ascontiguousarray(a, dtype=dt)
-            keep.append(a)
-            return a
-        n = int(topo["n"])
-        t.nbead = n
-        ib, jb, r0 = arr(topo["ib"], np.int32), arr(topo["jb"], np.int32), arr(topo["r0"], np.float64)
-        t.nbnd, t.ibnd, t.jbnd, t.r0bnd = len(ib), _ip(ib), _ip(jb), _dp(r0)
-        ia, ja, ka = arr(topo["ia"], np.int32), arr(topo["ja"], np.int32), arr(topo["ka"], np.int32)
-        t.nang, t.iang, t.jang, t.kang = len(ia), _ip(ia), _ip(ja), _ip(ka)
-        ai, aj, ar = arr(topo["att_i"], np.int32), arr(topo["att_j"], np.int32), arr(topo["att_r"], np.float64)
-        at = arr(topo.get("att_it", np.ones(len(ai))), np.int32)
-        ajt = arr(topo.get("att_jt", np.ones(len(ai))), np.int32)
-        t.ncon_att, t.att_i, t.att_j, t.att_it, t.att_jt, t.att_r = len(ai), _ip(ai), _ip(aj), _ip(at), _ip(ajt), _dp(ar)
-        if "rep_i" in topo and not implicit:
-            ri, rj = arr(topo["rep_i"], np.int32), arr(topo["rep_j"], np.int32)
-            rit = arr(topo.get("rep_it", np.ones(len(ri))), np.int32)
-            rjt = arr(topo.get("rep_jt", np.ones(len(ri))), np.int32)
-            t.ncon_rep, t.rep_i, t.rep_j, t.rep_it, t.rep_jt = len(ri), _ip(ri), _ip(rj), _ip(rit), _ip(rjt)
-        else:
-            t.ncon_rep = -1
-        if "bead_type" in topo:
-            bt = arr(topo["bead_type"], np.int32)
-            t.bead_type = _ip(bt)
+        t, keep = make_topology(topo, implicit)
         cid = C.c_int()
         self._chk(self.L.sopg_add_chain(self.h, C.byref(t), C.byref(params), C.byref(cid)))
+        del keep
         return cid.value
+
+    def remove_chain(self, chain):
+        self._chk(self.L.sopg_remove_chain(self.h, chain))
+
+    def reserve(self, beads, chains):
+        self._chk(self.L.sopg_reserve(self.h, beads, chains))
+
+    def autotune(self):
+        """Pick the fastest step-kernel lane count for the current chains; returns it."""
+        lanes = C.c_int()
+        self._chk(self.L.sopg_autotune(self.h, C.byref(lanes)))
+        return lanes.value
+
+    def lanes(self):
+        a, b = C.c_int(), C.c_int()
+        self._chk(self.L.sopg_get_lanes(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def nchains(self):
+        return self.L.sopg_nchains(self.h)
 
     def put_state(self, chain, pos=None, unc=None, vel=None, force=None):
         a = [None if x is None else np.ascontiguousarray(x, dtype=np.float64) for x in (pos, unc, vel, force)]
@@ -368,6 +439,11 @@ class Context:
     def set_step(self, chain, istep, inlup, iup):
         self._chk(self.L.sopg_set_step(self.h, chain, istep, inlup, iup))
 
+    def get_step(self, chain):
+        istep, inlup, iup = C.c_double(), C.c_int(), C.c_int()
+        self._chk(self.L.sopg_get_step(self.h, chain, C.byref(istep), C.byref(inlup), C.byref(iup)))
+        return istep.value, inlup.value, iup.value
+
     def sync(self):
         self._chk(self.L.sopg_sync(self.h))
 
@@ -383,3 +459,80 @@ class Context:
         ms, n = C.c_double(), C.c_long()
         self._chk(self.L.sopg_profile_read(self.h, kernel, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+
+class Server:
+    """Continuous-batching job server (include/sopserve.h) on one GPU."""
+
+    def __init__(self, device=0, hbm_budget=0.0, max_chains=0, chunk=0, lanes=0, capacity=0):
+        self.L = lib()
+        cfg = ServerConfig(float(hbm_budget), max_chains, chunk, lanes, capacity)
+        h = C.c_void_p()
+        rc = self.L.sopg_server_create(device, C.byref(cfg), C.byref(h))
+        if rc:
+            raise SopgError(f"sopg_server_create failed ({rc})")
+        self.h = h
+
+    def _chk(self, rc):
+        if rc != OK:
+            raise SopgError(f"server error {rc}: {self.L.sopg_server_last_error(self.h).decode()}")
+        return rc
+
+    def close(self):
+        if self.h:
+            self.L.sopg_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def now(self):
+        return self.L.sopg_server_now(self.h)
+
+    def footprint(self, topo):
+        t, keep = make_topology(topo)
+        return self.L.sopg_server_footprint(self.h, C.byref(t))
+
+    def submit(self, topo, params, pos, vel, nsteps, arrival=-1.0):
+        t, keep = make_topology(topo)
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        vel = np.ascontiguousarray(vel, dtype=np.float64)
+        job = C.c_long()
+        self._chk(self.L.sopg_server_submit(self.h, C.byref(t), C.byref(params), _ptr(pos), _ptr(vel), nsteps,
+                                            float(arrival), C.byref(job)))
+        return job.value
+
+    def poll(self):
+        a, q = C.c_int(), C.c_int()
+        self._chk(self.L.sopg_server_poll(self.h, C.byref(a), C.byref(q)))
+        return a.value, q.value
+
+    def job(self, job):
+        i = JobInfo()
+        self._chk(self.L.sopg_server_job(self.h, job, C.byref(i)))
+        return {f: getattr(i, f) for f, _ in JobInfo._fields_}
+
+    def records(self, job):
+        n = C.c_long()
+        self._chk(self.L.sopg_server_job_records(self.h, job, None, 0, C.byref(n)))
+        recs = (Record * max(n.value, 1))()
+        self._chk(self.L.sopg_server_job_records(self.h, job, recs, n.value, C.byref(n)))
+        return [{f: getattr(recs[k], f) for f, _ in Record._fields_} for k in range(n.value)]
+
+    def state(self, job):
+        n = self.job(job)["nbead"]
+        out = [np.zeros(3 * n) for _ in range(4)]
+        self._chk(self.L.sopg_server_job_state(self.h, job, *[_ptr(x) for x in out]))
+        return dict(pos=out[0], unc=out[1], vel=out[2], force=out[3])
+
+    def release(self, job):
+        self._chk(self.L.sopg_server_release(self.h, job))
+
+    def stats(self):
+        bs, busy = C.c_double(), C.c_double()
+        rounds = C.c_long()
+        self._chk(self.L.sopg_server_stats(self.h, C.byref(bs), C.byref(rounds), C.byref(busy)))
+        return dict(bead_steps=bs.value, rounds=rounds.value, busy_s=busy.value)

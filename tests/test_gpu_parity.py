@@ -297,6 +297,38 @@ def test_lanes_invariance(inputs_root, lanes):
         assert np.array_equal(a[k], b[k]), k
 
 
+@pytest.mark.parametrize("cutoff", [sopg.CUTOFF_NEIGHBOR, sopg.CUTOFF_CELL])
+def test_binned_rebuild_equals_full_scan(inputs_root, cutoff):
+    """The cell-grid list rebuild and the full candidate scan give identical lists
+    and therefore bit-identical trajectories (batch of sizes 80..1500)."""
+    chains = []
+    for n, seed in ((80, 41), (400, 42), (1500, 43)):
+        x, v, _ = chain_inputs(inputs_root, n, seed)
+        chains.append((x, v, 77 + seed))
+    res = []
+    for binned in (True, False):
+        ctx = sopg.Context(binned=binned)
+        for (x, v, seed) in chains:
+            ctx.add_chain(sopg.chain_topology(x),
+                          sopg.chain_params(boxl=504.0, ncell=42.0, cutoff=cutoff, nnlup=10, nup=50,
+                                            noise_mode=sopg.NOISE_PHILOX, seed=seed))
+        for c, (x, v, seed) in enumerate(chains):
+            ctx.put_state(c, pos=x, unc=x, vel=v, force=np.zeros_like(x))
+            (ctx.update_cell_list if cutoff == sopg.CUTOFF_CELL else ctx.update_neighbor_list)(c)
+            ctx.update_pair_list(c)
+            ctx.force_eval(c)
+        print(f"[gpu] binned={binned} cutoff={cutoff}", flush=True)
+        ctx.run(200)
+        res.append([(ctx.get_state(c), ctx.get_list(sopg.LIST_NL_REP, c), ctx.get_list(sopg.LIST_PL_REP, c))
+                    for c in range(len(chains))])
+        ctx.close()
+    for (sa, na, pa), (sb, nb, pb) in zip(*res):
+        for k in ("unc", "vel", "force"):
+            assert np.array_equal(sa[k], sb[k]), k
+        assert np.array_equal(na[0], nb[0]) and np.array_equal(na[1], nb[1])
+        assert np.array_equal(pa[0], pb[0]) and np.array_equal(pa[1], pb[1])
+
+
 def test_philox_noise_statistics(inputs_root):
     """Device noise: zero force field replaced by pure noise statistics check."""
     n = 2000
@@ -316,3 +348,59 @@ def test_philox_noise_statistics(inputs_root):
     g = (f2 - f1) / var  # difference of two independent deviates: N(0, 2)
     assert abs(g.mean()) < 0.05
     assert abs(g.var() - 2.0) < 0.1
+
+
+def _setup_chain(ctx, c, x, v):
+    ctx.put_state(c, pos=x, unc=x, vel=v, force=np.zeros_like(x))
+    ctx.update_neighbor_list(c)
+    ctx.update_pair_list(c)
+    ctx.force_eval(c)
+
+
+def test_continuous_batching_join_leave(inputs_root):
+    """Chains join and leave a running context (continuous batching): a chain
+    that stays, and one that joins mid-run into the memory a finished chain
+    freed, each evolve bit-identically to the same chain alone."""
+    spec = {"A": (90, 51), "B": (240, 52), "C": (130, 53), "D": (170, 54)}
+    data = {}
+    for k, (n, seed) in spec.items():
+        x, v, _ = chain_inputs(inputs_root, n, seed)
+        data[k] = (x, v, 500 + seed)
+    print("[gpu] continuous batching", flush=True)
+    mix = _philox_ctx([data["A"], data["B"], data["C"]])
+    r1 = mix.run(100)
+    mix.remove_chain(1)  # B leaves
+    assert mix.nchains() == 2
+    x, v, seed = data["D"]
+    d = mix.add_chain(sopg.chain_topology(x), sopg.chain_params(cutoff=sopg.CUTOFF_NEIGHBOR, nnlup=20, nup=100,
+                                                                noise_mode=sopg.NOISE_PHILOX, seed=seed))
+    assert d == 1  # the freed id is reused
+    _setup_chain(mix, d, x, v)
+    r2 = mix.run(150)
+    alone_a = _philox_ctx([data["A"]])
+    ra = alone_a.run(100) + alone_a.run(150)
+    alone_d = _philox_ctx([data["D"]])
+    rd = alone_d.run(150)
+    for got, ref in ((mix.get_state(0), alone_a.get_state(0)), (mix.get_state(d), alone_d.get_state(0))):
+        for k in ("unc", "vel", "force", "pos"):
+            assert np.array_equal(got[k], ref[k]), k
+    assert [r for r in r1 + r2 if r["chain"] == 0] == ra
+    assert [r for r in r2 if r["chain"] == d] == [dict(r, chain=d) for r in rd]
+
+
+def test_autotune_keeps_results(inputs_root):
+    """The autotuner changes the launch shape only: results stay bit-identical."""
+    chains = []
+    for n, seed in ((70, 61), (900, 62)):
+        x, v, _ = chain_inputs(inputs_root, n, seed)
+        chains.append((x, v, 9 + seed))
+    ref = _philox_ctx(chains, lanes=8)
+    ref.run(60)
+    tuned = _philox_ctx(chains, lanes=8)
+    lanes = tuned.autotune()
+    assert lanes in (4, 8, 16, 32, 64) and tuned.lanes()[0] == lanes
+    tuned.run(60)
+    for c in range(len(chains)):
+        a, b = ref.get_state(c), tuned.get_state(c)
+        for k in ("unc", "vel", "force"):
+            assert np.array_equal(a[k], b[k]), k
