@@ -112,9 +112,11 @@ def build_ctx(chains, w, lanes=0, list_lanes=0, topo_cache=None):
     return ctx, topos
 
 
-def timed_run(ctx, steps, warmup, profile):
+def timed_run(ctx, steps, warmup, profile, autotune=False):
     if warmup:
         ctx.run(warmup)
+    if autotune:
+        ctx.autotune()
     ctx.sync()
     if profile:
         ctx.profile(True)
@@ -130,7 +132,8 @@ def timed_run(ctx, steps, warmup, profile):
 
 def gpu_batched(chains, w, steps, warmup, lanes, list_lanes, topo_cache):
     ctx, topos = build_ctx(chains, w, lanes, list_lanes, topo_cache)
-    seconds, prof, recs = timed_run(ctx, steps, warmup, True)
+    seconds, prof, recs = timed_run(ctx, steps, warmup, True, autotune=not lanes)
+    step_lanes = ctx.lanes()[0]
     bytes_step = 0
     counts = []
     for c, t in enumerate(topos):
@@ -138,7 +141,8 @@ def gpu_batched(chains, w, steps, warmup, lanes, list_lanes, topo_cache):
         counts.append(k)
         bytes_step += (192 * t["n"] + 16 * len(t["ib"]) + 8 * len(t["ia"]) + 40 * k["nnl_att"] + 8 * k["nnl_rep"])
     ctx.close()
-    return dict(seconds=seconds, prof=prof, bytes_step=bytes_step, counts=counts, records=len(recs))
+    return dict(seconds=seconds, prof=prof, bytes_step=bytes_step, counts=counts, records=len(recs),
+                lanes=step_lanes)
 
 
 def gpu_plain(chains, w, steps, warmup, topo_cache, lanes=0, list_lanes=0):
@@ -146,7 +150,7 @@ def gpu_plain(chains, w, steps, warmup, topo_cache, lanes=0, list_lanes=0):
     total_bs, total_t = 0.0, 0.0
     for ch in chains:
         ctx, topos = build_ctx([ch], w, lanes, list_lanes, topo_cache)
-        seconds, _, _ = timed_run(ctx, steps, warmup, False)
+        seconds, _, _ = timed_run(ctx, steps, warmup, False, autotune=not lanes)
         total_bs += topos[0]["n"] * steps
         total_t += seconds
         ctx.close()
@@ -335,7 +339,8 @@ def main():
         "config": {"workload": f"{a.workload}: {w['desc']}", "chains": len(chains), "beads": nb,
                    "n_min": min(sizes), "n_max": max(sizes), "form": "batched (all chains per launch)",
                    "nnl_att": tot("nnl_att"), "nnl_rep": tot("nnl_rep"), "nil_att": tot("nil_att"),
-                   "nil_rep": tot("nil_rep"), "records": g["records"]},
+                   "nil_rep": tot("nil_rep"), "records": g["records"],
+                   "step_lanes": g["lanes"], "lanes_from": "flag" if a.lanes else "autotuner"},
         "plain": None if plain is None else {"value": plain, "unit": "bead-steps/s",
                                              "form": "plain (one chain per launch)", "steps_per_chain": a.plain_steps},
         "speedup_vs_plain": None if plain is None else value / plain,
@@ -351,14 +356,27 @@ def main():
     }
     print(json.dumps(line), flush=True)
     if a.calib_copy_mib:
-        import torch
-        n = a.calib_copy_mib * 2**20 // 8
-        src = torch.ones(n, dtype=torch.float64, device="cuda:0")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            torch.mul(src, 1.0, out=dst)  # reads and writes calib_copy_mib MiB each
-        torch.cuda.synchronize()
-        print(f"[bench] calibration copy: 3 x {a.calib_copy_mib} MiB read + written", file=sys.stderr, flush=True)
+        calib_copy(a.calib_copy_mib)
+
+
+def calib_copy(mib):
+    """Three device-to-device copies of `mib` MiB (HIP's blit kernel): each
+    reads and writes exactly mib MiB, the reference for HBM counter scaling."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    nbytes = ctypes.c_size_t(mib * 2**20)
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(ctypes.byref(p), nbytes):
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(src, 1, nbytes)
+    for _ in range(3):
+        if hip.hipMemcpy(dst, src, nbytes, 3):  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(f"[bench] calibration copy: 3 x {mib} MiB read + written", file=sys.stderr, flush=True)
 
 
 if __name__ == "__main__":

@@ -119,7 +119,7 @@ struct sopg_ctx {
   std::string err;
   sopg_tables tab{};
   Tables T{};
-  int lanes = 16;       // lanes per bead, step / force kernels
+  int lanes = 8;        // lanes per bead, step / force kernels
   int list_lanes = 64;  // lanes per bead, full-scan list rebuild
   int binned = 1;       // binned list rebuild where a chain's extent allows it
   int cap_user = 0;
@@ -834,7 +834,7 @@ int sopg_set_tables(sopg_ctx* c, const sopg_tables* t) {
 int sopg_set_lanes(sopg_ctx* c, int step_lanes, int list_lanes) {
   if (!c) return SOPG_ERR_ARG;
   auto ok = [](int l) { return l >= 1 && l <= 64 && !(l & (l - 1)); };
-  if (step_lanes == 0) step_lanes = 16;
+  if (step_lanes == 0) step_lanes = 8;
   if (list_lanes == 0) list_lanes = 64;
   if (!ok(step_lanes) || !ok(list_lanes)) return fail(c, SOPG_ERR_ARG, "lanes: power of two in 1..64");
   c->lanes = step_lanes;

@@ -61,6 +61,18 @@ __device__ __forceinline__ double group_sum(double v) {
   return v;
 }
 
+// Workgroups are dispatched round-robin over the chip's 8 XCDs, each with its
+// own L2.  Logical block k of the returned numbering runs on one XCD together
+// with its neighbours k-1, k+1, ..., so the contiguous beads of a chain -- and
+// the partner positions they gather -- stay in one L2.  A bijection on
+// [0, gridDim.x).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block() {
+  const int n = gridDim.x, b = blockIdx.x;
+  const int q = n / kXcds, r = n % kXcds, x = b % kXcds;
+  return x * q + (x < r ? x : r) + b / kXcds;
+}
+
 // ------------------------------------------------------------------ rare paths
 
 // energy.cpp:293: the native skip test reads lj_nat_pdb_dist2[i] with i the
@@ -108,14 +120,14 @@ __global__ void k_cellcoords(Dev D, int cur, int g0, int g1) {
 }
 
 // Neighbor-list (distance) or cell-list rebuild plus the pair-list filter at
-// the same positions.  mode < 0: each chain's own cutoff, only chains whose
-// rebuild is due at relative step rel; mode >= 0: that criterion, all chains.
+// the same positions, scanning every candidate of bead g (the full scan; the
+// binned kernel below covers compact chains).  mode < 0: each chain's own
+// cutoff, only chains whose rebuild is due at relative step rel; mode >= 0:
+// that criterion, all chains.
 template <int L>
-__global__ void __launch_bounds__(256) k_rebuild(Dev D, const Tables* __restrict__ T, int cur, int par, int rel, int mode, int g0, int g1) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int g = g0 + t / L;
-  const int l = t & (L - 1);
-  if (g >= g1 || D.chain[g] < 0) return;  // past the end, or a free bead
+__device__ __forceinline__ void rebuild_scan_bead(const Dev& D, const Tables* __restrict__ T, int cur, int par,
+                                                  int rel, int mode, int g, int l) {
+  if (D.chain[g] < 0) return;  // a free bead
   const ChainP& P = D.cp[D.chain[g]];
   int crit = mode;
   if (mode < 0) {
@@ -219,6 +231,17 @@ __global__ void __launch_bounds__(256) k_rebuild(Dev D, const Tables* __restrict
     D.cnt[g] = cnt < D.cap ? cnt : D.cap;
     if (cnt > D.cap) atomicOr(D.err, 1);
   }
+}
+
+// Grid-stride over beads: a bounded grid, so that a launch in which every due
+// chain is binned (the usual case) costs little more than its dispatch.
+template <int L>
+__global__ void __launch_bounds__(256) k_rebuild(Dev D, const Tables* __restrict__ T, int cur, int par, int rel,
+                                                 int mode, int g0, int g1) {
+  const int l = threadIdx.x & (L - 1);
+  const int stride = (int)(gridDim.x * blockDim.x / L);
+  for (int g = g0 + (xcd_block() * (int)blockDim.x + (int)threadIdx.x) / L; g < g1; g += stride)
+    rebuild_scan_bead<L>(D, T, cur, par, rel, mode, g, l);
 }
 
 // ------------------------------------------------------- binned rebuild
@@ -361,17 +384,25 @@ __global__ void k_bin_fill(Dev D, int rel, int mode, int g0, int g1) {
   D.cell_beads[P.off + at] = g - P.off;
 }
 
-// One wave (64 lanes) per bead, four beads per 256-thread block.
+// One group of LG lanes per bead, 256 / LG beads per block.  Each bead keeps
+// two LDS bitmaps with one bit per bead of its chain: partners that pass the
+// list criterion, and those of them inside the pair-list cutoff.  The 27
+// neighbouring grid cells are walked as 9 x-runs: cells (x-1..x+1, y, z) are
+// consecutive in the cell-sorted bead array.
+template <int LG>
 __global__ void __launch_bounds__(256) k_rebuild_grid(Dev D, const Tables* __restrict__ T, int cur, int par, int rel,
                                                       int mode, int g0, int g1) {
+  constexpr int BPB = 256 / LG;
   extern __shared__ uint32_t bitmaps[];
-  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const int g = g0 + blockIdx.x * 4 + wave;
-  uint32_t* bm = bitmaps + (size_t)wave * D.bitmap_words;
+  const int grp = (int)threadIdx.x / LG, l = (int)threadIdx.x & (LG - 1);
+  const int g = g0 + xcd_block() * BPB + grp;
+  const int W = D.bitmap_words;
+  uint32_t* nlb = bitmaps + (size_t)grp * 2 * W;
+  uint32_t* plb = nlb + W;
   bool active = g < g1;
+  if (active) active = D.chain[g] >= 0;
   int crit = 0;
   const ChainP* Pp = nullptr;
-  if (active) active = D.chain[g] >= 0;
   if (active) {
     Pp = &D.cp[D.chain[g]];
     active = rebuild_due(*Pp, rel, mode, crit) && D.grid[D.chain[g]].use;
@@ -388,8 +419,8 @@ __global__ void __launch_bounds__(256) k_rebuild_grid(Dev D, const Tables* __res
     a1 = D.att_end[g];
     const double boxl = P.boxl, qbox = P.qbox;
     uint8_t* aplw = par ? D.am_pl1 : D.am_pl0;
-    // natives: NL / PL flags, exactly as k_rebuild
-    for (int q = a0 + l; q < a1; q += 64) {
+    // natives: NL / PL flags, exactly as the full scan
+    for (int q = a0 + l; q < a1; q += LG) {
       const int code = D.att_code[q], k = code >> 1, side = code & 1, o = D.att_partner[q];
       const int i = side ? o : g, j = side ? g : o;
       const double dx = mimg(ux[j] - ux[i], boxl, qbox);
@@ -402,16 +433,18 @@ __global__ void __launch_bounds__(256) k_rebuild_grid(Dev D, const Tables* __res
       D.att_flag[q] = (uint8_t)((nl ? 1 : 0) | (pl ? (2 << par) : 0));
       if (side == 0) { D.am_nl[k] = nl; aplw[k] = pl; }
     }
-    for (int w = l; w < words; w += 64) bm[w] = 0u;
+    for (int w = l; w < words; w += LG) { nlb[w] = 0u; plb[w] = 0u; }
   }
   __syncthreads();
   if (active) {
-    // mark every partner of the 27 neighbouring cells that passes the criterion
     const ChainP& P = *Pp;
-    const GridP& G = D.grid[D.chain[g]];
+    const int c = D.chain[g];
+    const GridP& G = D.grid[c];
     const int id = D.bead_cell[g];
     const int cxi = id % G.nx, cyi = (id / G.nx) % G.ny, czi = id / (G.nx * G.ny);
-    const size_t cbase = (size_t)D.chain[g] * kGridCells;
+    const int x0 = cxi > 0 ? cxi - 1 : 0, x1 = cxi + 1 < G.nx ? cxi + 1 : G.nx - 1;
+    const size_t cbase = (size_t)c * kGridCells;
+    const int* cb = D.cell_beads + P.off;
     const double boxl = P.boxl, qbox = P.qbox;
     for (int dz = -1; dz <= 1; dz++) {
       const int z = czi + dz;
@@ -419,27 +452,24 @@ __global__ void __launch_bounds__(256) k_rebuild_grid(Dev D, const Tables* __res
       for (int dy = -1; dy <= 1; dy++) {
         const int y = cyi + dy;
         if (y < 0 || y >= G.ny) continue;
-        for (int dx = -1; dx <= 1; dx++) {
-          const int x = cxi + dx;
-          if (x < 0 || x >= G.nx) continue;
-          const size_t cc = cbase + (size_t)((z * G.ny + y) * G.nx + x);
-          const int s0 = D.cell_start[cc], s1 = s0 + D.cell_fill[cc];
-          for (int s = s0 + l; s < s1; s += 64) {
-            const int c = D.cell_beads[P.off + s];
-            if (!(c - b > 2 || b - c > 2)) continue;
-            const int gc = P.off + c;
-            const int i = gc < g ? gc : g, j = gc < g ? g : gc;
-            bool keep;
-            if (crit == SOPG_CUTOFF_NEIGHBOR) {
-              const int it = D.type[i], jt = D.type[j];
-              const double ddx = mimg(ux[j] - ux[i], boxl, qbox);
-              const double ddy = mimg(uy[j] - uy[i], boxl, qbox);
-              const double ddz = mimg(uz[j] - uz[i], boxl, qbox);
-              keep = ddx * ddx + ddy * ddy + ddz * ddz < T->nlrc2[it][jt];
-            } else {
-              keep = cell_adjacent(D, P, i, j);
-            }
-            if (keep) atomicOr(&bm[c >> 5], 1u << (c & 31));
+        const int row = (z * G.ny + y) * G.nx;
+        const int s0 = D.cell_start[cbase + row + x0];
+        const int s1 = D.cell_start[cbase + row + x1] + D.cell_fill[cbase + row + x1];
+        for (int s = s0 + l; s < s1; s += LG) {
+          const int cc = cb[s];
+          if (!(cc - b > 2 || b - cc > 2)) continue;
+          const int gc = P.off + cc;
+          const int i = gc < g ? gc : g, j = gc < g ? g : gc;
+          const int it = D.type[i], jt = D.type[j];
+          const double ddx = mimg(ux[j] - ux[i], boxl, qbox);
+          const double ddy = mimg(uy[j] - uy[i], boxl, qbox);
+          const double ddz = mimg(uz[j] - uz[i], boxl, qbox);
+          const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+          const bool keep = (crit == SOPG_CUTOFF_NEIGHBOR) ? d2 < T->nlrc2[it][jt] : cell_adjacent(D, P, i, j);
+          if (keep) {
+            const uint32_t bit = 1u << (cc & 31);
+            atomicOr(&nlb[cc >> 5], bit);
+            if (d2 < T->plrc2[it][jt]) atomicOr(&plb[cc >> 5], bit);
           }
         }
       }
@@ -448,44 +478,45 @@ __global__ void __launch_bounds__(256) k_rebuild_grid(Dev D, const Tables* __res
   __syncthreads();
   if (active) {
     // natives are not in the non-native master list
-    for (int q = a0 + l; q < a1; q += 64) {
+    for (int q = a0 + l; q < a1; q += LG) {
       const int p = D.att_partner[q] - Pp->off;
-      atomicAnd(&bm[p >> 5], ~(1u << (p & 31)));
+      const uint32_t m = ~(1u << (p & 31));
+      atomicAnd(&nlb[p >> 5], m);
+      atomicAnd(&plb[p >> 5], m);
     }
   }
   __syncthreads();
   if (!active) return;
+  // emit in ascending partner order (the master list's order for bead b)
   const ChainP& P = *Pp;
-  const double boxl = P.boxl, qbox = P.qbox;
   uint32_t* S = D.slot + (size_t)g * D.cap;
+  const uint32_t pbit = 1u << (30 + par);
   int cnt = 0;
-  for (int base = 0; base < words; base += 64) {
+  for (int base = 0; base < words; base += LG) {
     const int w = base + l;
-    uint32_t bits = w < words ? bm[w] : 0u;
+    uint32_t bits = w < words ? nlb[w] : 0u;
+    const uint32_t pls = w < words ? plb[w] : 0u;
     const int pc = __popc(bits);
     int incl = pc;
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const int t = __shfl_up(incl, o, 64);
+    for (int o = 1; o < LG; o <<= 1) {
+      const int t = __shfl_up(incl, o, LG);
       if (l >= o) incl += t;
     }
     int idx = cnt + incl - pc;
     while (bits) {
-      const int c = w * 32 + __builtin_ctz(bits);
+      const int k = __builtin_ctz(bits);
       bits &= bits - 1u;
+      const int c = w * 32 + k;
       const int gc = P.off + c;
       const int side = gc < g ? 1 : 0;
       const int i = side ? gc : g, j = side ? g : gc;
-      const int it = D.type[i], jt = D.type[j];
-      const double ddx = mimg(ux[j] - ux[i], boxl, qbox);
-      const double ddy = mimg(uy[j] - uy[i], boxl, qbox);
-      const double ddz = mimg(uz[j] - uz[i], boxl, qbox);
-      const bool pl = ddx * ddx + ddy * ddy + ddz * ddz < T->plrc2[it][jt];
       if (idx < D.cap)
-        S[idx] = slot_make((uint32_t)c, (uint32_t)side, (uint32_t)it, (uint32_t)jt) | (pl ? (1u << (30 + par)) : 0u);
+        S[idx] = slot_make((uint32_t)c, (uint32_t)side, (uint32_t)D.type[i], (uint32_t)D.type[j]) |
+                 (((pls >> k) & 1u) ? pbit : 0u);
       idx++;
     }
-    cnt += __shfl(incl, 63, 64);
+    cnt += __shfl(incl, LG - 1, LG);
   }
   if (l == 0) {
     D.cnt[g] = cnt < D.cap ? cnt : D.cap;
@@ -496,7 +527,7 @@ __global__ void __launch_bounds__(256) k_rebuild_grid(Dev D, const Tables* __res
 // Pair-list filter at the current positions (sop.cpp:473-553), parity par.
 template <int L>
 __global__ void __launch_bounds__(256) k_pairfilter(Dev D, const Tables* __restrict__ T, int cur, int par, int g0, int g1) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = xcd_block() * blockDim.x + threadIdx.x;
   const int g = g0 + t / L;
   const int l = t & (L - 1);
   if (g >= g1 || D.chain[g] < 0) return;  // past the end, or a free bead
@@ -586,7 +617,7 @@ enum { F_STORE = 1, F_VEL = 2, F_ENERGY = 4, F_ENERGY_ALL = 8, F_NOISE = 16, F_B
 template <int L, bool NEXT>
 __global__ void __launch_bounds__(256) k_force(Dev D, const Tables* __restrict__ T, int cur, int pr, int rel, int flags,
                                                const double* __restrict__ noise, int g0, int g1) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = xcd_block() * blockDim.x + threadIdx.x;
   const int g = g0 + t / L;
   const int l = t & (L - 1);
   if (g >= g1 || D.chain[g] < 0) return;  // past the end, or a free bead
@@ -887,7 +918,8 @@ static inline unsigned blocks_for(long threads) { return (unsigned)((threads + 2
 template <int L>
 static void launch_rebuild_L(const Dev& D, const Tables* T, int cur, int par, int rel, int mode, int g0, int g1,
                              hipStream_t s) {
-  hipLaunchKernelGGL(k_rebuild<L>, dim3(blocks_for((long)(g1 - g0) * L)), dim3(256), 0, s, D, T, cur, par, rel,
+  const unsigned blocks = blocks_for((long)(g1 - g0) * L);
+  hipLaunchKernelGGL(k_rebuild<L>, dim3(blocks < 4096u ? blocks : 4096u), dim3(256), 0, s, D, T, cur, par, rel,
                      mode, g0, g1);
 }
 template <int L>
@@ -921,7 +953,9 @@ void launch_cellcoords(const Dev& D, int cur, int g0, int g1, hipStream_t s) {
 }
 // List rebuild of beads [g0, g1) = chains [c0, c0 + nc): the binned kernels for
 // chains whose grid applies (binned != 0), the full scan for the others.
-constexpr int kMaxBitmapWords = 4096;  // 4 waves x 16 KiB of LDS: chains up to 131072 beads
+constexpr int kGridLanes = 16;                       // lanes per bead in the binned rebuild
+constexpr int kGridBeads = 256 / kGridLanes;         // beads per block
+constexpr int kMaxBitmapWords = 512;                 // 16 beads x 2 bitmaps x 2 KiB = 64 KiB of LDS
 void launch_rebuild(int L, const Dev& D, const Tables* T, int cur, int par, int rel, int mode, int g0, int g1,
                     int c0, int nc, int binned, hipStream_t s) {
   const int words = D.bitmap_words < kMaxBitmapWords ? D.bitmap_words : kMaxBitmapWords;
@@ -932,8 +966,9 @@ void launch_rebuild(int L, const Dev& D, const Tables* T, int cur, int par, int 
     hipLaunchKernelGGL(k_bin_fill, dim3(blocks_for(g1 - g0)), dim3(256), 0, s, D, rel, mode, g0, g1);
     Dev Dg = D;
     Dg.bitmap_words = words;
-    hipLaunchKernelGGL(k_rebuild_grid, dim3((unsigned)((g1 - g0 + 3) / 4)), dim3(256),
-                       (size_t)4 * words * sizeof(uint32_t), s, Dg, T, cur, par, rel, mode, g0, g1);
+    hipLaunchKernelGGL(k_rebuild_grid<kGridLanes>, dim3((unsigned)((g1 - g0 + kGridBeads - 1) / kGridBeads)),
+                       dim3(256), (size_t)kGridBeads * 2 * words * sizeof(uint32_t), s, Dg, T, cur, par, rel, mode,
+                       g0, g1);
   }
 #define C_(X) launch_rebuild_L<X>(D, T, cur, par, rel, mode, g0, g1, s)
   SOPG_DISPATCH_L(L, C_)
