@@ -11,8 +11,10 @@ every 1000 steps, device Philox noise.  Other workloads: --workload plain3000
 (1024 trajectories of one N=76 chain, rebuild every 50 steps).
 
 serve: a Poisson stream of jobs (20 jobs/s; N log-uniform in [50, 10000];
-steps log-uniform in [1e4, 1e6] scaled by --serve-step-scale so the stream
-finishes in seconds; kT uniform in [0.5, 1.3]) replayed in real time against
+steps log-uniform in [1e4, 1e6] scaled by --serve-step-scale (default 0.1)
+so the stream finishes in seconds, rounded to a multiple of 1000; kT uniform
+in [0.5, 1.3]; half folding from a SAW, half unfolding from native) replayed
+in real time against
 the continuous-batching server (include/sopserve.h); p50/p99 job latency =
 finish - arrival, throughput = job bead-steps / (last finish - first arrival).
 
@@ -159,22 +161,27 @@ def gpu_plain(chains, w, steps, warmup, topo_cache, lanes=0, list_lanes=0):
 
 def serve_jobs(njobs, step_scale, threads, seed=5, rate=20.0):
     """The serving stream: Poisson arrivals (rate jobs/s), N log-uniform in
-    [50, 10000], steps log-uniform in [1e4, 1e6] times step_scale, kT uniform
-    in [0.5, 1.3]; each job its own native structure and Maxwell velocities."""
+    [50, 10000], steps log-uniform in [1e4, 1e6] times step_scale and rounded
+    to a multiple of 1000 (the record period), kT uniform in [0.5, 1.3].
+    Each job has its own native structure (the topology); unfolding jobs
+    (even ids) start from it, folding jobs (odd ids) from a self-avoiding walk."""
     rng = np.random.default_rng(seed)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, njobs))
     sizes = np.round(np.exp(rng.uniform(np.log(50), np.log(10000), njobs))).astype(int)
-    steps = np.maximum(1, np.round(np.exp(rng.uniform(np.log(1e4), np.log(1e6), njobs)) * step_scale)).astype(int)
+    raw = np.exp(rng.uniform(np.log(1e4), np.log(1e6), njobs)) * step_scale
+    steps = np.maximum(1000, np.round(raw / 1000.0) * 1000).astype(int)
     kts = rng.uniform(0.5, 1.3, njobs)
 
     def one(j):
         n = int(sizes[j])
-        x = sopg.gen_native(n, 20_000 + j)
-        return x, sopg.gen_maxwell(n, float(kts[j]), 20_000 + j), sopg.chain_topology(x)
+        nat = sopg.gen_native(n, 20_000 + j)
+        x0 = nat if j % 2 == 0 else sopg.gen_saw(n, 30_000 + j)
+        return x0, sopg.gen_maxwell(n, float(kts[j]), 20_000 + j), sopg.chain_topology(nat)
     with ThreadPoolExecutor(max_workers=threads) as ex:
         built = list(ex.map(one, range(njobs)))
     return [dict(arrival=float(arrivals[j]), n=int(sizes[j]), steps=int(steps[j]), kT=float(kts[j]), x=built[j][0],
-                 v=built[j][1], topo=built[j][2], seed=20_000 + j) for j in range(njobs)]
+                 v=built[j][1], topo=built[j][2], seed=20_000 + j, kind="unfold" if j % 2 == 0 else "fold")
+            for j in range(njobs)]
 
 
 def gpu_serve(jobs, w, hbm_budget, max_chains, chunk, lanes):
@@ -204,6 +211,7 @@ def gpu_serve(jobs, w, hbm_budget, max_chains, chunk, lanes):
     nrec = sum(len(srv.records(j)) for j in ids)
     srv.close()
     done = [k for k in info if k["state"] == sopg.JOB_DONE]
+    offered = sum(j["n"] * j["steps"] for j in jobs) / (jobs[-1]["arrival"] - jobs[0]["arrival"])
     lat = np.array([k["t_finish"] - k["t_arrival"] for k in done])
     wait = np.array([k["t_admit"] - k["t_arrival"] for k in done])
     span = max(k["t_finish"] for k in done) - (t0 + jobs[0]["arrival"])
@@ -213,7 +221,8 @@ def gpu_serve(jobs, w, hbm_budget, max_chains, chunk, lanes):
                 p50_latency_s=float(np.percentile(lat, 50)), p99_latency_s=float(np.percentile(lat, 99)),
                 p50_wait_s=float(np.percentile(wait, 50)), p99_wait_s=float(np.percentile(wait, 99)),
                 makespan_s=span, rounds=st["rounds"], busy_s=st["busy_s"], peak_resident=peak_active,
-                records=nrec)
+                records=nrec, offered_load=offered,
+                saturated=bool(work / span < 0.9 * offered))
 
 
 def cpu_baseline(chains, w, budget_s=20.0, max_n=3200):
@@ -278,7 +287,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-jobs", type=int, default=100)
-    ap.add_argument("--serve-step-scale", type=float, default=0.01,
+    ap.add_argument("--serve-step-scale", type=float, default=0.1,
                     help="job step counts are log-uniform in [1e4, 1e6] times this factor")
     ap.add_argument("--serve-budget-gib", type=float, default=8.0)
     ap.add_argument("--serve-chunk", type=int, default=1000)
@@ -316,7 +325,8 @@ def main():
         print(f"[bench] generated {len(jobs)} jobs in {time.perf_counter() - t1:.1f} s", file=sys.stderr, flush=True)
         serve = gpu_serve(jobs, WORKLOADS["mixed256"], a.serve_budget_gib * 2**30, 1024, a.serve_chunk, a.lanes)
         serve.update(stream=f"Poisson arrivals 20 jobs/s, {a.serve_jobs} jobs (seed 5); N log-uniform [50, 10000]; "
-                            f"steps log-uniform [1e4, 1e6] x {a.serve_step_scale:g}; kT uniform [0.5, 1.3]; "
+                            f"steps log-uniform [1e4, 1e6] x {a.serve_step_scale:g}, rounded to a multiple of 1000; "
+                            f"kT uniform [0.5, 1.3]; half unfolding (from native), half folding (from a SAW); "
                             f"neighbor list every 20 steps; Q/Rg/energy record every {NUP} steps",
                      hbm_budget_gib=a.serve_budget_gib, chunk=a.serve_chunk)
         print(f"[bench] serve {serve['value']:.4g} bead-steps/s p50 {serve['p50_latency_s']:.3f} s "

@@ -86,7 +86,7 @@ struct Arr {
   size_t esz, per;
 };
 
-enum { P_BEAD = 0, P_BOND, P_BINC, P_ANG, P_AINC, P_ATT, P_TINC, P_REP, P_RINC, P_CHAIN, NPOOL };
+enum { P_BEAD = 0, P_BINC, P_AINC, P_ATT, P_TINC, P_REP, P_RINC, P_CHAIN, NPOOL };
 
 struct HostChain {
   bool active = false;
@@ -100,6 +100,7 @@ struct HostChain {
   bool rep_explicit = false;
   std::vector<int> ri, rj, rit, rjt;
   std::vector<int8_t> type;
+  std::vector<int> q0;  // device incidence index of each native entry's first bead
   double istep = 1.0;
   int inlup = 1, iup = 1;
   uint64_t noise_base = 0;
@@ -275,15 +276,14 @@ void register_arrays(sopg_ctx* c) {
   add(P_BEAD, &D.slot, 4, cap);
   add(P_BEAD, &D.slot_k, 4, cap);
   add(P_BEAD, &D.red, 8, kRed);
-  add(P_BOND, &D.bi, 4, 1); add(P_BOND, &D.bj, 4, 1); add(P_BOND, &D.br0, 8, 1);
-  add(P_BINC, &D.bond_code, 4, 1);
-  add(P_ANG, &D.gi, 4, 1); add(P_ANG, &D.gk, 4, 1);
-  add(P_AINC, &D.ang_code, 4, 1);
-  add(P_ATT, &D.am_i, 4, 1); add(P_ATT, &D.am_j, 4, 1); add(P_ATT, &D.am_t, 1, 1);
-  for (double** p : {&D.am_r2, &D.am_r6, &D.am_r12, &D.am_nlrc2, &D.am_plrc2}) add(P_ATT, p, 8, 1);
+  add(P_BINC, &D.bond_pq, 4, 1); add(P_BINC, &D.bond_r0, 8, 1);
+  add(P_AINC, &D.ang_pq, 4, 1);
+  add(P_ATT, &D.am_i, 4, 1); add(P_ATT, &D.am_j, 4, 1); add(P_ATT, &D.am_q0, 4, 1); add(P_ATT, &D.am_t, 1, 1);
+  for (double** p : {&D.am_r2, &D.am_nlrc2}) add(P_ATT, p, 8, 1);
   add(P_ATT, &D.am_rf, 4, 1);
-  add(P_ATT, &D.am_nl, 1, 1); add(P_ATT, &D.am_pl0, 1, 1); add(P_ATT, &D.am_pl1, 1, 1);
   add(P_TINC, &D.att_code, 4, 1); add(P_TINC, &D.att_partner, 4, 1); add(P_TINC, &D.att_flag, 1, 1);
+  add(P_TINC, &D.att_t, 1, 1);
+  for (double** p : {&D.att_r6, &D.att_r12, &D.att_plrc2}) add(P_TINC, p, 8, 1);
   add(P_REP, &D.rm_i, 4, 1); add(P_REP, &D.rm_j, 4, 1); add(P_REP, &D.rm_t, 1, 1);
   add(P_RINC, &D.rep_code, 4, 1);
   add(P_CHAIN, &D.cp, sizeof(ChainP), 1);
@@ -432,8 +432,7 @@ int materialize(sopg_ctx* c, int id) {
                                      " beads exceeds the per-bead list capacity " + std::to_string(D.cap) +
                                      " (sopg_set_capacity before the first device operation)");
   const size_t nb = h.bi.size(), na = h.gi.size(), nt = h.ai.size(), nr = h.rep_explicit ? h.ri.size() : 0;
-  if ((rc = take_range(c, P_BEAD, h.n, h.at[P_BEAD])) || (rc = take_range(c, P_BOND, nb, h.at[P_BOND])) ||
-      (rc = take_range(c, P_BINC, 2 * nb, h.at[P_BINC])) || (rc = take_range(c, P_ANG, na, h.at[P_ANG])) ||
+  if ((rc = take_range(c, P_BEAD, h.n, h.at[P_BEAD])) || (rc = take_range(c, P_BINC, 2 * nb, h.at[P_BINC])) ||
       (rc = take_range(c, P_AINC, 2 * na, h.at[P_AINC])) || (rc = take_range(c, P_ATT, nt, h.at[P_ATT])) ||
       (rc = take_range(c, P_TINC, 2 * nt, h.at[P_TINC])) || (rc = take_range(c, P_REP, nr, h.at[P_REP])) ||
       (rc = take_range(c, P_RINC, 2 * nr, h.at[P_RINC])))
@@ -452,23 +451,28 @@ int materialize(sopg_ctx* c, int id) {
     return g;
   };
   if ((rc = upload_at(c, D.chain, off, chain)) || (rc = upload_at(c, D.type, off, type))) return rc;
-  build_csr(n, h.bi, h.bj, h.at[P_BOND], h.at[P_BINC], off, beg, end, code);
+  // bonds and angles: per incidence the partner (global) << 1 | side
+  auto pq_of = [&](const std::vector<int>& code, const std::vector<int>& a, const std::vector<int>& b) {
+    std::vector<int> pq(code.size());
+    for (size_t q = 0; q < code.size(); q++) {
+      const size_t k = (size_t)(code[q] >> 1), side = (size_t)(code[q] & 1);
+      pq[q] = ((off + (side ? a[k] : b[k])) << 1) | (int)side;
+    }
+    return pq;
+  };
+  build_csr(n, h.bi, h.bj, 0, h.at[P_BINC], off, beg, end, code);
+  std::vector<double> r0q(code.size());
+  for (size_t q = 0; q < code.size(); q++) r0q[q] = h.br0[(size_t)(code[q] >> 1)];
   if ((rc = upload_at(c, D.bond_beg, off, beg)) || (rc = upload_at(c, D.bond_end, off, end)) ||
-      (rc = upload_at(c, D.bond_code, h.at[P_BINC], code)) || (rc = upload_at(c, D.bi, h.at[P_BOND], glob(h.bi))) ||
-      (rc = upload_at(c, D.bj, h.at[P_BOND], glob(h.bj))) || (rc = upload_at(c, D.br0, h.at[P_BOND], h.br0)))
+      (rc = upload_at(c, D.bond_pq, h.at[P_BINC], pq_of(code, h.bi, h.bj))) ||
+      (rc = upload_at(c, D.bond_r0, h.at[P_BINC], r0q)))
     return rc;
-  build_csr(n, h.gi, h.gk, h.at[P_ANG], h.at[P_AINC], off, beg, end, code);
+  build_csr(n, h.gi, h.gk, 0, h.at[P_AINC], off, beg, end, code);
   if ((rc = upload_at(c, D.ang_beg, off, beg)) || (rc = upload_at(c, D.ang_end, off, end)) ||
-      (rc = upload_at(c, D.ang_code, h.at[P_AINC], code)) || (rc = upload_at(c, D.gi, h.at[P_ANG], glob(h.gi))) ||
-      (rc = upload_at(c, D.gk, h.at[P_ANG], glob(h.gk))))
+      (rc = upload_at(c, D.ang_pq, h.at[P_AINC], pq_of(code, h.gi, h.gk))))
     return rc;
   // natives
   build_csr(n, h.ai, h.aj, h.at[P_ATT], h.at[P_TINC], off, beg, end, code);
-  std::vector<int> partner(code.size());
-  for (size_t q = 0; q < code.size(); q++) {
-    const size_t k = (size_t)(code[q] >> 1) - h.at[P_ATT];
-    partner[q] = off + ((code[q] & 1) ? h.ai[k] : h.aj[k]);
-  }
   std::vector<int8_t> at(nt);
   std::vector<double> r2(nt), r6(nt), r12(nt), nlrc2(nt), plrc2(nt);
   std::vector<float> rf(nt);
@@ -484,20 +488,32 @@ int materialize(sopg_ctx* c, int id) {
     plrc2[k] = rp * rp;
     rf[k] = (float)r;
   }
-  const size_t ta = h.at[P_ATT];
-  if ((rc = upload_at(c, D.att_beg, off, beg)) || (rc = upload_at(c, D.att_end, off, end)) ||
-      (rc = upload_at(c, D.att_code, h.at[P_TINC], code)) || (rc = upload_at(c, D.att_partner, h.at[P_TINC], partner)) ||
-      (rc = upload_at(c, D.am_i, ta, glob(h.ai))) || (rc = upload_at(c, D.am_j, ta, glob(h.aj))) ||
-      (rc = upload_at(c, D.am_t, ta, at)) || (rc = upload_at(c, D.am_r2, ta, r2)) || (rc = upload_at(c, D.am_r6, ta, r6)) ||
-      (rc = upload_at(c, D.am_r12, ta, r12)) || (rc = upload_at(c, D.am_nlrc2, ta, nlrc2)) ||
-      (rc = upload_at(c, D.am_plrc2, ta, plrc2)) || (rc = upload_at(c, D.am_rf, ta, rf)))
-    return rc;
-  if (nt) {
-    HIP_OK(c, hipMemsetAsync(D.att_flag + h.at[P_TINC], 0, 2 * nt, c->stream));
-    HIP_OK(c, hipMemsetAsync(D.am_nl + ta, 0, nt, c->stream));
-    HIP_OK(c, hipMemsetAsync(D.am_pl0 + ta, 0, nt, c->stream));
-    HIP_OK(c, hipMemsetAsync(D.am_pl1 + ta, 0, nt, c->stream));
+  // per incidence: partner, and the pair's constants in the bead's own order
+  const size_t nq = code.size();
+  std::vector<int> partner(nq), q0(nt);
+  std::vector<int8_t> qt(nq);
+  std::vector<double> qr6(nq), qr12(nq), qpl(nq);
+  for (size_t q = 0; q < nq; q++) {
+    const size_t k = (size_t)(code[q] >> 1) - h.at[P_ATT];
+    partner[q] = off + ((code[q] & 1) ? h.ai[k] : h.aj[k]);
+    if (!(code[q] & 1)) q0[k] = (int)(h.at[P_TINC] + q);
+    qt[q] = (int8_t)(at[k] | ((code[q] & 1) << 4));
+    qr6[q] = r6[k];
+    qr12[q] = r12[k];
+    qpl[q] = plrc2[k];
   }
+  h.q0.assign(q0.begin(), q0.end());
+  const size_t ta = h.at[P_ATT], tq = h.at[P_TINC];
+  if ((rc = upload_at(c, D.att_beg, off, beg)) || (rc = upload_at(c, D.att_end, off, end)) ||
+      (rc = upload_at(c, D.att_code, tq, code)) || (rc = upload_at(c, D.att_partner, tq, partner)) ||
+      (rc = upload_at(c, D.att_t, tq, qt)) || (rc = upload_at(c, D.att_r6, tq, qr6)) ||
+      (rc = upload_at(c, D.att_r12, tq, qr12)) || (rc = upload_at(c, D.att_plrc2, tq, qpl)) ||
+      (rc = upload_at(c, D.am_i, ta, glob(h.ai))) || (rc = upload_at(c, D.am_j, ta, glob(h.aj))) ||
+      (rc = upload_at(c, D.am_q0, ta, q0)) || (rc = upload_at(c, D.am_t, ta, at)) ||
+      (rc = upload_at(c, D.am_r2, ta, r2)) || (rc = upload_at(c, D.am_nlrc2, ta, nlrc2)) ||
+      (rc = upload_at(c, D.am_rf, ta, rf)))
+    return rc;
+  if (nt) HIP_OK(c, hipMemsetAsync(D.att_flag + tq, 0, 2 * nt, c->stream));
   // explicit non-natives
   if (h.rep_explicit) {
     build_csr(n, h.ri, h.rj, h.at[P_REP], h.at[P_RINC], off, beg, end, code);
@@ -567,7 +583,7 @@ int ensure_device(sopg_ctx* c) {
   D.bitmap_words = 1;
   register_arrays(c);
   int rc;
-  const size_t init[NPOOL] = {beads, nbnd, 2 * nbnd, nang, 2 * nang, natt, 2 * natt, nrep, 2 * nrep, nch};
+  const size_t init[NPOOL] = {beads, 2 * nbnd, 2 * nang, natt, 2 * natt, nrep, 2 * nrep, nch};
   for (int pi = 0; pi < NPOOL; pi++)
     if ((rc = grow_pool(c, pi, std::max<size_t>(init[pi], 1)))) return rc;
   if ((rc = dalloc(c, &D.err, 1)) || (rc = dalloc(c, &c->d_T, 1))) return rc;
@@ -907,7 +923,7 @@ int sopg_remove_chain(sopg_ctx* c, int chain) {
     int rc = upload_at(c, c->D.chain, h.at[P_BEAD], none);
     if (rc) return rc;
     const size_t nb = h.bi.size(), na = h.gi.size(), nt = h.ai.size(), nr = h.rep_explicit ? h.ri.size() : 0;
-    const size_t len[NPOOL] = {(size_t)h.n, nb, 2 * nb, na, 2 * na, nt, 2 * nt, nr, 2 * nr, 0};
+    const size_t len[NPOOL] = {(size_t)h.n, 2 * nb, 2 * na, nt, 2 * nt, nr, 2 * nr, 0};
     for (int pi = 0; pi < P_CHAIN; pi++) c->pool[pi].give(h.at[pi], len[pi]);
   }
   h = HostChain();
@@ -1124,12 +1140,13 @@ int sopg_get_list(sopg_ctx* c, int chain, int which, int* oi, int* oj, long cap,
   if (which < SOPG_LIST_NL_ATT || which > SOPG_LIST_PL_REP) return fail(c, SOPG_ERR_ARG, "bad list id");
   std::vector<std::pair<int, int>> L;
   if (which == SOPG_LIST_NL_ATT || which == SOPG_LIST_PL_ATT) {
-    std::vector<uint8_t> f(P->natt);
-    const uint8_t* src = (which == SOPG_LIST_NL_ATT) ? c->D.am_nl : (c->par ? c->D.am_pl1 : c->D.am_pl0);
-    if (P->natt) HIP_OK(c, hipMemcpyAsync(f.data(), src + P->att_off, P->natt, hipMemcpyDeviceToHost, c->stream));
+    const size_t nq = 2 * (size_t)P->natt, tq = h->at[P_TINC];
+    std::vector<uint8_t> f(nq);
+    if (nq) HIP_OK(c, hipMemcpyAsync(f.data(), c->D.att_flag + tq, nq, hipMemcpyDeviceToHost, c->stream));
     HIP_OK(c, hipStreamSynchronize(c->stream));
+    const int bit = (which == SOPG_LIST_NL_ATT) ? 1 : (2 << c->par);
     for (int k = 0; k < P->natt; k++)
-      if (f[k]) L.push_back({h->ai[k] + 1, h->aj[k] + 1});
+      if (f[(size_t)h->q0[k] - tq] & bit) L.push_back({h->ai[k] + 1, h->aj[k] + 1});
   } else {
     const int n = P->n, cap_s = c->D.cap;
     std::vector<int> cnt(n);
@@ -1244,7 +1261,7 @@ int sopg_autotune(sopg_ctx* c, int* step_lanes) {
   HIP_OK(c, hipEventCreate(&e1));
   double best = 1e300;
   int bestL = c->lanes;
-  for (int L : {4, 8, 16, 32, 64}) {
+  for (int L : {1, 2, 4, 8, 16, 32, 64}) {
     launch_force(L, c->D, c->d_T, c->cur, c->par, 0, F_ALLTERMS, false, nullptr, 0, nb, c->stream);
     (void)hipEventRecord(e0, c->stream);
     for (int k = 0; k < 5; k++)

@@ -67,15 +67,19 @@ struct Dev {
   double *vx, *vy, *vz, *fx, *fy, *fz, *ix, *iy, *iz;
   int* chain;   // chain id of each bead, -1 for a free bead
   int8_t* type;
-  // bonds: incidence code = bond << 1 | side (0: bead is i, 1: bead is j)
-  int *bond_beg, *bond_end, *bond_code, *bi, *bj; double* br0;
-  // angles: code = angle << 1 | side (0: bead is i, 1: bead is k)
-  int *ang_beg, *ang_end, *ang_code, *gi, *gk;
-  // natives: code = master << 1 | side; partner = the other bead (global)
+  // bonds, per incidence: partner << 1 | side (0: bead is i, 1: bead is j), r0
+  int *bond_beg, *bond_end, *bond_pq; double* bond_r0;
+  // angles (i, k) pairs, per incidence: partner << 1 | side (0: bead is i, 1: bead is k)
+  int *ang_beg, *ang_end, *ang_pq;
+  // natives: code = master << 1 | side; partner = the other bead (global).
+  // Per incidence also the pair's constants the step kernel reads, so that a
+  // bead's lanes load them contiguously: att_t = itype | jtype << 2 | side << 4,
+  // sigma^6, sigma^12, (2.5 sigma)^2.
   int *att_beg, *att_end, *att_code, *att_partner; uint8_t* att_flag;  // flag: bit0 NL, bit1+p PL(parity p)
-  int *am_i, *am_j; int8_t* am_t;  // am_t = itype | jtype << 2
-  double *am_r2, *am_r6, *am_r12, *am_nlrc2, *am_plrc2; float* am_rf;
-  uint8_t *am_nl, *am_pl0, *am_pl1;
+  int8_t* att_t; double *att_r6, *att_r12, *att_plrc2;
+  // master native entries (am_t = itype | jtype << 2; am_q0 = incidence of the first bead)
+  int *am_i, *am_j, *am_q0; int8_t* am_t;
+  double *am_r2, *am_nlrc2; float* am_rf;
   // explicit non-natives
   int *rep_beg, *rep_end, *rep_code, *rm_i, *rm_j; int8_t* rm_t;
   // non-native neighbor slots

@@ -75,23 +75,30 @@ __device__ __forceinline__ int xcd_block() {
 
 // ------------------------------------------------------------------ rare paths
 
+// Native pair-list membership of master entry k at parity pr: the flag of its
+// first-bead incidence.
+__device__ __forceinline__ bool att_in_pl(const uint8_t* flag, const int* __restrict__ q0, int k, int pr) {
+  return (flag[q0[k]] >> (1 + pr)) & 1;
+}
+
 // energy.cpp:293: the native skip test reads lj_nat_pdb_dist2[i] with i the
 // PAIR-LIST position of the entry, not its master index.  Only reachable when
 // d2 < tol * max(lj_nat_pdb_dist2); then the position is counted exactly.
-__device__ __noinline__ bool att_tol_skip(const uint8_t* __restrict__ pl, const double* __restrict__ r2, int att_off,
-                                          int k, double d2) {
+__device__ __noinline__ bool att_tol_skip(const uint8_t* flag, const int* __restrict__ q0,
+                                          const double* __restrict__ r2, int att_off, int k, int pr, double d2) {
   int p = 0;
-  for (int kk = att_off; kk < k; kk++) p += pl[kk];
+  for (int kk = att_off; kk < k; kk++) p += att_in_pl(flag, q0, kk, pr);
   return d2 < kTol * r2[att_off + p];
 }
 
 // energy.cpp:292, 331: rep_tol comes from the LAST native pair-list entry.
 // With an empty native pair list the reference reads it uninitialised; here no
 // non-native pair is skipped in that case.
-__device__ __noinline__ bool rep_tol_skip(const uint8_t* __restrict__ pl, const int8_t* __restrict__ am_t,
-                                          const Tables* __restrict__ T, int att_off, int natt, double d2) {
+__device__ __noinline__ bool rep_tol_skip(const uint8_t* flag, const int* __restrict__ q0,
+                                          const int8_t* __restrict__ am_t, const Tables* __restrict__ T, int att_off,
+                                          int natt, int pr, double d2) {
   for (int kk = att_off + natt - 1; kk >= att_off; kk--) {
-    if (pl[kk]) {
+    if (att_in_pl(flag, q0, kk, pr)) {
       const int tt = am_t[kk];
       return d2 < T->s2[tt & 3][tt >> 2] * kTol;
     }
@@ -139,7 +146,6 @@ __device__ __forceinline__ void rebuild_scan_bead(const Dev& D, const Tables* __
   const double* uy = cur ? D.uy[1] : D.uy[0];
   const double* uz = cur ? D.uz[1] : D.uz[0];
   const double boxl = P.boxl, qbox = P.qbox;
-  uint8_t* aplw = par ? D.am_pl1 : D.am_pl0;
 
   // native contacts (neighbor_list.cpp:17-67 / cell_list.cpp:16-57)
   const int a0 = D.att_beg[g], a1 = D.att_end[g];
@@ -151,11 +157,10 @@ __device__ __forceinline__ void rebuild_scan_bead(const Dev& D, const Tables* __
     const double dz = mimg(uz[j] - uz[i], boxl, qbox);
     const double d2 = dx * dx + dy * dy + dz * dz;
     bool nl, pl;
-    if (crit == SOPG_CUTOFF_NEIGHBOR) { nl = d2 < D.am_nlrc2[k]; pl = nl && d2 < D.am_plrc2[k]; }
-    else if (crit == SOPG_CUTOFF_CELL) { nl = cell_adjacent(D, P, i, j); pl = nl && d2 < D.am_plrc2[k]; }
+    if (crit == SOPG_CUTOFF_NEIGHBOR) { nl = d2 < D.am_nlrc2[k]; pl = nl && d2 < D.att_plrc2[q]; }
+    else if (crit == SOPG_CUTOFF_CELL) { nl = cell_adjacent(D, P, i, j); pl = nl && d2 < D.att_plrc2[q]; }
     else { nl = true; pl = true; }
     D.att_flag[q] = (uint8_t)((nl ? 1 : 0) | (pl ? (2 << par) : 0));
-    if (side == 0) { D.am_nl[k] = nl; aplw[k] = pl; }
   }
 
   // non-native contacts: stable (ballot + prefix) compaction into the slots
@@ -418,7 +423,6 @@ __global__ void __launch_bounds__(256) k_rebuild_grid(Dev D, const Tables* __res
     a0 = D.att_beg[g];
     a1 = D.att_end[g];
     const double boxl = P.boxl, qbox = P.qbox;
-    uint8_t* aplw = par ? D.am_pl1 : D.am_pl0;
     // natives: NL / PL flags, exactly as the full scan
     for (int q = a0 + l; q < a1; q += LG) {
       const int code = D.att_code[q], k = code >> 1, side = code & 1, o = D.att_partner[q];
@@ -428,10 +432,9 @@ __global__ void __launch_bounds__(256) k_rebuild_grid(Dev D, const Tables* __res
       const double dz = mimg(uz[j] - uz[i], boxl, qbox);
       const double d2 = dx * dx + dy * dy + dz * dz;
       bool nl, pl;
-      if (crit == SOPG_CUTOFF_NEIGHBOR) { nl = d2 < D.am_nlrc2[k]; pl = nl && d2 < D.am_plrc2[k]; }
-      else { nl = cell_adjacent(D, P, i, j); pl = nl && d2 < D.am_plrc2[k]; }
+      if (crit == SOPG_CUTOFF_NEIGHBOR) { nl = d2 < D.am_nlrc2[k]; pl = nl && d2 < D.att_plrc2[q]; }
+      else { nl = cell_adjacent(D, P, i, j); pl = nl && d2 < D.att_plrc2[q]; }
       D.att_flag[q] = (uint8_t)((nl ? 1 : 0) | (pl ? (2 << par) : 0));
-      if (side == 0) { D.am_nl[k] = nl; aplw[k] = pl; }
     }
     for (int w = l; w < words; w += LG) { nlb[w] = 0u; plb[w] = 0u; }
   }
@@ -536,11 +539,10 @@ __global__ void __launch_bounds__(256) k_pairfilter(Dev D, const Tables* __restr
   const double* uy = cur ? D.uy[1] : D.uy[0];
   const double* uz = cur ? D.uz[1] : D.uz[0];
   const double boxl = P.boxl, qbox = P.qbox;
-  uint8_t* aplw = par ? D.am_pl1 : D.am_pl0;
   const uint8_t pbit = (uint8_t)(2 << par);
   const int a0 = D.att_beg[g], a1 = D.att_end[g];
   for (int q = a0 + l; q < a1; q += L) {
-    const int code = D.att_code[q], k = code >> 1, side = code & 1, o = D.att_partner[q];
+    const int side = D.att_code[q] & 1, o = D.att_partner[q];
     const uint8_t f = D.att_flag[q];
     bool pl = false;
     if (f & 1) {
@@ -549,10 +551,9 @@ __global__ void __launch_bounds__(256) k_pairfilter(Dev D, const Tables* __restr
       const double dy = mimg(uy[j] - uy[i], boxl, qbox);
       const double dz = mimg(uz[j] - uz[i], boxl, qbox);
       const double d2 = dx * dx + dy * dy + dz * dz;
-      pl = (P.cutoff == SOPG_CUTOFF_NONE) ? true : d2 < D.am_plrc2[k];
+      pl = (P.cutoff == SOPG_CUTOFF_NONE) ? true : d2 < D.att_plrc2[q];
     }
     D.att_flag[q] = (uint8_t)((f & ~pbit) | (pl ? pbit : 0));
-    if (side == 0) aplw[k] = pl;
   }
   const int cnt = D.cnt[g];
   uint32_t* S = D.slot + (size_t)g * D.cap;
@@ -614,20 +615,48 @@ __global__ void k_velupdate(Dev D, int g0, int g1) {
 enum { F_STORE = 1, F_VEL = 2, F_ENERGY = 4, F_ENERGY_ALL = 8, F_NOISE = 16, F_BOND = 32, F_ANGLE = 64,
        F_VDW = 128, F_ACCUM = 256 };
 
+// Pair displacement u[j] - u[i] seen from bead g: c is g's own coordinate, o
+// the partner's.  side 0 (g is i): o - c; side 1 (g is j): c - o -- the same
+// subtraction the reference performs, so the same bits.
+__device__ __forceinline__ double pair_d(int side, double c, double o, double boxl, double qbox) {
+  return mimg(side ? c - o : o - c, boxl, qbox);
+}
+
+// Type-pair tables the pair loops read, staged in LDS (index it * 3 + jt).
+enum { TP_FREP = 0, TP_S6, TP_S12, TP_PLRC2, TP_CREP, TP_FATT, TP_CATT, TP_N };
+
 template <int L, bool NEXT>
 __global__ void __launch_bounds__(256) k_force(Dev D, const Tables* __restrict__ T, int cur, int pr, int rel, int flags,
                                                const double* __restrict__ noise, int g0, int g1) {
+  __shared__ double tp[9][TP_N];
+  if (threadIdx.x < 9) {
+    const int it = threadIdx.x / 3, jt = threadIdx.x % 3;
+    tp[threadIdx.x][TP_FREP] = T->frep[it][jt];
+    tp[threadIdx.x][TP_S6] = T->s6[it][jt];
+    tp[threadIdx.x][TP_S12] = T->s12[it][jt];
+    tp[threadIdx.x][TP_PLRC2] = T->plrc2[it][jt];
+    tp[threadIdx.x][TP_CREP] = T->crep[it][jt];
+    tp[threadIdx.x][TP_FATT] = T->fatt[it][jt];
+    tp[threadIdx.x][TP_CATT] = T->catt[it][jt];
+  }
+  __syncthreads();
   const int t = xcd_block() * blockDim.x + threadIdx.x;
   const int g = g0 + t / L;
   const int l = t & (L - 1);
-  if (g >= g1 || D.chain[g] < 0) return;  // past the end, or a free bead
-  const ChainP& P = D.cp[D.chain[g]];
+  if (g >= g1) return;
+  const int chain = D.chain[g];
+  if (chain < 0) return;  // a free bead
+  const ChainP* Pp = D.cp + chain;
+  // chain fields the loops use, in registers (the loops' stores could alias them)
+  const int off = Pp->off, att_off = Pp->att_off, natt = Pp->natt;
+  const bool no_cut = Pp->cutoff == SOPG_CUTOFF_NONE;
+  const double boxl = Pp->boxl, qbox = Pp->qbox, thr_att = Pp->thr_att;
   const double* __restrict__ ux = cur ? D.ux[1] : D.ux[0];
   const double* __restrict__ uy = cur ? D.uy[1] : D.uy[0];
   const double* __restrict__ uz = cur ? D.uz[1] : D.uz[0];
-  const double boxl = P.boxl, qbox = P.qbox;
+  const double cx = ux[g], cy = uy[g], cz = uz[g];
   const int pw = pr ^ 1;
-  const bool energy = (flags & F_ENERGY_ALL) || ((flags & F_ENERGY) && ((P.iup0 + rel) % P.nup == 0));
+  const bool energy = (flags & F_ENERGY_ALL) || ((flags & F_ENERGY) && ((Pp->iup0 + rel) % Pp->nup == 0));
   double eb = 0.0, ea = 0.0, eatt = 0.0, erep = 0.0;
 
   // clear_forces, then random_force (energy.cpp:35-45, 449-465):
@@ -636,20 +665,20 @@ __global__ void __launch_bounds__(256) k_force(Dev D, const Tables* __restrict__
   const bool with_noise = (flags & F_STORE) && (flags & F_NOISE);
   if (!with_noise) {
     // energy only, or the random term not requested
-  } else if (P.noise_mode == SOPG_NOISE_REPLAY) {
+  } else if (Pp->noise_mode == SOPG_NOISE_REPLAY) {
     gx = noise[3 * (size_t)g];
     gy = noise[3 * (size_t)g + 1];
     gz = noise[3 * (size_t)g + 2];
   } else {
-    const uint64_t ctr = P.noise_base + (uint64_t)rel;
-    const uint32_t bead = (uint32_t)(g - P.off);
+    const uint64_t ctr = Pp->noise_base + (uint64_t)rel;
+    const uint32_t bead = (uint32_t)(g - off);
     if constexpr (L == 1) {
       double u;
-      philox_gauss2(P.seed, ctr, bead, 0, gx, gy);
-      philox_gauss2(P.seed, ctr, bead, 1, gz, u);
+      philox_gauss2(Pp->seed, ctr, bead, 0, gx, gy);
+      philox_gauss2(Pp->seed, ctr, bead, 1, gz, u);
     } else {
       double a, c;
-      philox_gauss2(P.seed, ctr, bead, (uint32_t)(l & 1), a, c);
+      philox_gauss2(Pp->seed, ctr, bead, (uint32_t)(l & 1), a, c);
       gx = __shfl(a, 0, L);
       gy = __shfl(c, 0, L);
       gz = __shfl(a, 1, L);
@@ -659,9 +688,10 @@ __global__ void __launch_bounds__(256) k_force(Dev D, const Tables* __restrict__
   double Fy = (flags & F_ACCUM) ? D.fy[g] : 0.0;
   double Fz = (flags & F_ACCUM) ? D.fz[g] : 0.0;
   if (with_noise) {
-    Fx += P.var * gx;
-    Fy += P.var * gy;
-    Fz += P.var * gz;
+    const double var = Pp->var;
+    Fx += var * gx;
+    Fy += var * gy;
+    Fz += var * gz;
   }
 
   // FENE bonds (energy.cpp:402-447): F_i -= f, F_j += f
@@ -672,13 +702,12 @@ __global__ void __launch_bounds__(256) k_force(Dev D, const Tables* __restrict__
       const bool has = q < p1;
       double fx = 0.0, fy = 0.0, fz = 0.0;
       if (has) {
-        const int code = D.bond_code[q], kb = code >> 1, side = code & 1;
-        const int i = D.bi[kb], j = D.bj[kb];
-        const double dx = mimg(ux[j] - ux[i], boxl, qbox);
-        const double dy = mimg(uy[j] - uy[i], boxl, qbox);
-        const double dz = mimg(uz[j] - uz[i], boxl, qbox);
+        const int pq = D.bond_pq[q], o = pq >> 1, side = pq & 1;
+        const double dx = pair_d(side, cx, ux[o], boxl, qbox);
+        const double dy = pair_d(side, cy, uy[o], boxl, qbox);
+        const double dz = pair_d(side, cz, uz[o], boxl, qbox);
         const double d = sqrt(dx * dx + dy * dy + dz * dz);
-        const double dev = d - D.br0[kb];
+        const double dev = d - D.bond_r0[q];
         const double dev2 = dev * dev;
         const double temp = -T->k_bnd * dev / d / (1.0 - dev2 / T->R0sq);
         fx = temp * dx; fy = temp * dy; fz = temp * dz;
@@ -696,11 +725,10 @@ __global__ void __launch_bounds__(256) k_force(Dev D, const Tables* __restrict__
       const bool has = q < p1;
       double fx = 0.0, fy = 0.0, fz = 0.0;
       if (has) {
-        const int code = D.ang_code[q], ka = code >> 1, side = code & 1;
-        const int i = D.gi[ka], k = D.gk[ka];
-        const double dx = mimg(ux[k] - ux[i], boxl, qbox);
-        const double dy = mimg(uy[k] - uy[i], boxl, qbox);
-        const double dz = mimg(uz[k] - uz[i], boxl, qbox);
+        const int pq = D.ang_pq[q], o = pq >> 1, side = pq & 1;
+        const double dx = pair_d(side, cx, ux[o], boxl, qbox);
+        const double dy = pair_d(side, cy, uy[o], boxl, qbox);
+        const double dz = pair_d(side, cz, uz[o], boxl, qbox);
         const double d = sqrt(dx * dx + dy * dy + dz * dz);
         const double d8 = pow8_cr(d);
         const double co1 = T->f_ang_ss_coeff / d8;
@@ -714,43 +742,38 @@ __global__ void __launch_bounds__(256) k_force(Dev D, const Tables* __restrict__
   // native LJ 12-6 over the pair list (energy.cpp:274-311): F_i += f, F_j -= f
   if (flags & F_VDW) {
     const int p0 = D.att_beg[g], p1 = D.att_end[g];
-    uint8_t* aplw = pw ? D.am_pl1 : D.am_pl0;
     for (int base = p0; base < p1; base += L) {
       const int q = base + l;
       bool contrib = false;
       double fx = 0.0, fy = 0.0, fz = 0.0;
       if (q < p1) {
         const uint8_t f = D.att_flag[q];
-        const int code = D.att_code[q], k = code >> 1, side = code & 1;
         if (f & 1) {
+          const int tt = D.att_t[q], side = (tt >> 4) & 1, tpi = (tt & 3) * 3 + ((tt >> 2) & 3);
           const int o = D.att_partner[q];
-          const int i = side ? o : g, j = side ? g : o;
-          const double dx = mimg(ux[j] - ux[i], boxl, qbox);
-          const double dy = mimg(uy[j] - uy[i], boxl, qbox);
-          const double dz = mimg(uz[j] - uz[i], boxl, qbox);
+          const double dx = pair_d(side, cx, ux[o], boxl, qbox);
+          const double dy = pair_d(side, cy, uy[o], boxl, qbox);
+          const double dz = pair_d(side, cz, uz[o], boxl, qbox);
           const double d2 = dx * dx + dy * dy + dz * dz;
           if ((f >> (1 + pr)) & 1) {
-            const int tt = D.am_t[k], it = tt & 3, jt = tt >> 2;
+            const double r6 = D.att_r6[q], r12 = D.att_r12[q];
             const double d6 = d2 * d2 * d2;
             const double d12 = d6 * d6;
-            const bool skip = (d2 < P.thr_att) && att_tol_skip(pr ? D.am_pl1 : D.am_pl0, D.am_r2, P.att_off, k, d2);
+            const bool skip = (d2 < thr_att) &&
+                              att_tol_skip(D.att_flag, D.am_q0, D.am_r2, att_off, D.att_code[q] >> 1, pr, d2);
             if (!skip) {
-              const double co1 = T->fatt[it][jt] / d2 * ((D.am_r12[k] / d12) - (D.am_r6[k] / d6));
+              const double co1 = tp[tpi][TP_FATT] / d2 * ((r12 / d12) - (r6 / d6));
               fx = co1 * dx; fy = co1 * dy; fz = co1 * dz;
               if (side == 1) { fx = -fx; fy = -fy; fz = -fz; }
               contrib = true;
             }
-            if (energy && side == 0)
-              eatt += T->catt[it][jt] * (D.am_r12[k] / d12) - 2.0 * (D.am_r6[k] / d6);
+            if (energy && side == 0) eatt += tp[tpi][TP_CATT] * (r12 / d12) - 2.0 * (r6 / d6);
           }
           if constexpr (NEXT) {
-            const bool npl = (P.cutoff == SOPG_CUTOFF_NONE) || d2 < D.am_plrc2[k];
+            const bool npl = no_cut || d2 < D.att_plrc2[q];
             const uint8_t nf = (uint8_t)((f & ~(2 << pw)) | (npl ? (2 << pw) : 0));
             if (nf != f) D.att_flag[q] = nf;
-            if (side == 0) aplw[k] = npl;
           }
-        } else if constexpr (NEXT) {
-          if (side == 0) aplw[k] = 0;
         }
       }
       fold3<L>(Fx, Fy, Fz, contrib, fx, fy, fz);
@@ -761,32 +784,35 @@ __global__ void __launch_bounds__(256) k_force(Dev D, const Tables* __restrict__
     const int cnt = D.cnt[g];
     uint32_t* S = D.slot + (size_t)g * D.cap;
     const uint32_t rbit = 1u << (30 + pr), wbit = 1u << (30 + pw);
+    const double thr_rep = T->thr_rep;
     for (int base = 0; base < cnt; base += L) {
       const int q = base + l;
       bool contrib = false;
       double fx = 0.0, fy = 0.0, fz = 0.0;
       if (q < cnt) {
         const uint32_t w = S[q];
-        const int o = P.off + (int)(w & kPartnerMask), side = (w >> 25) & 1, it = (w >> 26) & 3, jt = (w >> 28) & 3;
-        const int i = side ? o : g, j = side ? g : o;
-        const double dx = mimg(ux[j] - ux[i], boxl, qbox);
-        const double dy = mimg(uy[j] - uy[i], boxl, qbox);
-        const double dz = mimg(uz[j] - uz[i], boxl, qbox);
+        const int o = off + (int)(w & kPartnerMask), side = (w >> 25) & 1;
+        const int tpi = (int)((w >> 26) & 3) * 3 + (int)((w >> 28) & 3);
+        const double dx = pair_d(side, cx, ux[o], boxl, qbox);
+        const double dy = pair_d(side, cy, uy[o], boxl, qbox);
+        const double dz = pair_d(side, cz, uz[o], boxl, qbox);
         const double d2 = dx * dx + dy * dy + dz * dz;
         if (w & rbit) {
+          const double s6 = tp[tpi][TP_S6], s12 = tp[tpi][TP_S12];
           const double d6 = d2 * d2 * d2;
           const double d12 = d6 * d6;
-          const bool skip = (d2 < T->thr_rep) && rep_tol_skip(pr ? D.am_pl1 : D.am_pl0, D.am_t, T, P.att_off, P.natt, d2);
+          const bool skip =
+              (d2 < thr_rep) && rep_tol_skip(D.att_flag, D.am_q0, D.am_t, T, att_off, natt, pr, d2);
           if (!skip) {
-            const double co1 = T->frep[it][jt] / d2 * (2.0 * T->s12[it][jt] / d12 + T->s6[it][jt] / d6);
+            const double co1 = tp[tpi][TP_FREP] / d2 * (2.0 * s12 / d12 + s6 / d6);
             fx = co1 * dx; fy = co1 * dy; fz = co1 * dz;
             if (side == 1) { fx = -fx; fy = -fy; fz = -fz; }
             contrib = true;
           }
-          if (energy && side == 0) erep += T->crep[it][jt] * (T->s12[it][jt] / d12 + T->s6[it][jt] / d6);
+          if (energy && side == 0) erep += tp[tpi][TP_CREP] * (s12 / d12 + s6 / d6);
         }
         if constexpr (NEXT) {
-          const bool npl = (P.cutoff == SOPG_CUTOFF_NONE) || d2 < T->plrc2[it][jt];
+          const bool npl = no_cut || d2 < tp[tpi][TP_PLRC2];
           const uint32_t nw = (w & ~wbit) | (npl ? wbit : 0u);
           if (nw != w) S[q] = nw;
         }
@@ -804,33 +830,35 @@ __global__ void __launch_bounds__(256) k_force(Dev D, const Tables* __restrict__
     r[R_EBND] = eb; r[R_EANG] = ea; r[R_EATT] = eatt; r[R_EREP] = erep;
   }
   if (!(flags & F_STORE)) return;
+  const bool vel = (flags & F_VEL) && Pp->vel_on && Pp->sim_type != 2;
+  const bool od = Pp->sim_type == 2;
+  const double a1 = Pp->a1, a2 = Pp->a2, a3 = Pp->a3, a4 = Pp->a4, a5 = Pp->a5;
   D.fx[g] = Fx; D.fy[g] = Fy; D.fz[g] = Fz;
   double vx = D.vx[g], vy = D.vy[g], vz = D.vz[g];
-  if ((flags & F_VEL) && P.vel_on && P.sim_type != 2) {
-    vx = P.a3 * D.ix[g] + P.a4 * Fx;
-    vy = P.a3 * D.iy[g] + P.a4 * Fy;
-    vz = P.a3 * D.iz[g] + P.a4 * Fz;
+  if (vel) {
+    vx = a3 * D.ix[g] + a4 * Fx;
+    vy = a3 * D.iy[g] + a4 * Fy;
+    vz = a3 * D.iz[g] + a4 * Fz;
     D.vx[g] = vx; D.vy[g] = vy; D.vz[g] = vz;
   }
   if constexpr (NEXT) {
     // the next step's position half (sop.cpp:288-310) into buffer cur^1
-    const bool od = P.sim_type == 2;
-    const double ix = od ? P.a5 * Fx : P.a1 * vx + P.a2 * Fx;
-    const double iy = od ? P.a5 * Fy : P.a1 * vy + P.a2 * Fy;
-    const double iz = od ? P.a5 * Fz : P.a1 * vz + P.a2 * Fz;
+    const double ix = od ? a5 * Fx : a1 * vx + a2 * Fx;
+    const double iy = od ? a5 * Fy : a1 * vy + a2 * Fy;
+    const double iz = od ? a5 * Fz : a1 * vz + a2 * Fz;
     D.ix[g] = ix; D.iy[g] = iy; D.iz[g] = iz;
-    double* pxs = cur ? D.px[1] : D.px[0];
-    double* pys = cur ? D.py[1] : D.py[0];
-    double* pzs = cur ? D.pz[1] : D.pz[0];
+    const double* pxs = cur ? D.px[1] : D.px[0];
+    const double* pys = cur ? D.py[1] : D.py[0];
+    const double* pzs = cur ? D.pz[1] : D.pz[0];
     double* pxd = cur ? D.px[0] : D.px[1];
     double* pyd = cur ? D.py[0] : D.py[1];
     double* pzd = cur ? D.pz[0] : D.pz[1];
     pxd[g] = mimg(pxs[g] + ix, boxl, qbox);
     pyd[g] = mimg(pys[g] + iy, boxl, qbox);
     pzd[g] = mimg(pzs[g] + iz, boxl, qbox);
-    (cur ? D.ux[0] : D.ux[1])[g] = ux[g] + ix;
-    (cur ? D.uy[0] : D.uy[1])[g] = uy[g] + iy;
-    (cur ? D.uz[0] : D.uz[1])[g] = uz[g] + iz;
+    (cur ? D.ux[0] : D.ux[1])[g] = cx + ix;
+    (cur ? D.uy[0] : D.uy[1])[g] = cy + iy;
+    (cur ? D.uz[0] : D.uz[1])[g] = cz + iz;
   }
 }
 
@@ -845,7 +873,6 @@ __global__ void k_observe(Dev D, int cur, int par, int rel, int all, int what, i
   const double* uy = cur ? D.uy[1] : D.uy[0];
   const double* uz = cur ? D.uz[1] : D.uz[0];
   const double boxl = P.boxl, qbox = P.qbox;
-  const uint8_t* apl = par ? D.am_pl1 : D.am_pl0;
   double* r = D.red + (size_t)g * kRed;
   int cont = 0, nla = 0, pla = 0, nlr = 0, plr = 0;
   const int a0 = D.att_beg[g], a1 = D.att_end[g];
@@ -853,8 +880,9 @@ __global__ void k_observe(Dev D, int cur, int par, int rel, int all, int what, i
     const int code = D.att_code[q];
     if (code & 1) continue;  // each native pair once, from its i bead
     const int k = code >> 1, j = D.att_partner[q];
-    nla += D.am_nl[k];
-    pla += apl[k];
+    const uint8_t f = D.att_flag[q];
+    nla += f & 1;
+    pla += (f >> (1 + par)) & 1;
     if (!(what & OBS_PHYS)) continue;
     // Q: d / (float) r_ij < 1.25 with d from unc_pos[i] - unc_pos[j] (sop.cpp:223-238)
     const double dx = mimg(ux[g] - ux[j], boxl, qbox);

@@ -91,7 +91,8 @@ double footprint(int cap, const sopg_topology* t) {
   const double per_bead = 12 * 8 + 12 * 8 + 12 * 4 + 1 + 8.0 * cap + 12 * 8;
   const double per_chain = 512 + 3.0 * 24 * 24 * 24 * 4 + 12 * 8;
   const double nrep = t->ncon_rep > 0 ? t->ncon_rep : 0;
-  return per_bead * t->nbead + 24.0 * t->nbnd + 16.0 * t->nang + 74.0 * t->ncon_att + 17.0 * nrep + per_chain;
+  const double per_native = 4 + 4 + 4 + 1 + 8 + 8 + 4 + 2 * (4 + 4 + 1 + 1 + 8 + 8 + 8);
+  return per_bead * t->nbead + 24.0 * t->nbnd + 8.0 * t->nang + per_native * t->ncon_att + 17.0 * nrep + per_chain;
 }
 
 int admit(sopg_server* s, Job& j, long id) {
