@@ -106,6 +106,13 @@ __device__ __noinline__ bool rep_tol_skip(const uint8_t* flag, const int* __rest
   return false;
 }
 
+// Pair displacement u[j] - u[i] seen from bead g: c is g's own coordinate, o
+// the partner's.  side 0 (g is i): o - c; side 1 (g is j): c - o -- the same
+// subtraction the reference performs, so the same bits.
+__device__ __forceinline__ double pair_d(int side, double c, double o, double boxl, double qbox) {
+  return mimg(side ? c - o : o - c, boxl, qbox);
+}
+
 __device__ __forceinline__ bool cell_adjacent(const Dev& D, const ChainP& P, int i, int j) {
   // cell_list.cpp:24-44 with the cell indices precomputed per bead
   const double cx = mimg(D.cx[i] - D.cx[j], P.ncell, P.qncell);
@@ -399,6 +406,11 @@ __global__ void __launch_bounds__(256) k_rebuild_grid(Dev D, const Tables* __res
                                                       int mode, int g0, int g1) {
   constexpr int BPB = 256 / LG;
   extern __shared__ uint32_t bitmaps[];
+  __shared__ double cut2[9][2];  // per type pair (it * 3 + jt): (3.2 sigma)^2, (2.5 sigma)^2
+  if (threadIdx.x < 9) {
+    cut2[threadIdx.x][0] = T->nlrc2[threadIdx.x / 3][threadIdx.x % 3];
+    cut2[threadIdx.x][1] = T->plrc2[threadIdx.x / 3][threadIdx.x % 3];
+  }
   const int grp = (int)threadIdx.x / LG, l = (int)threadIdx.x & (LG - 1);
   const int g = g0 + xcd_block() * BPB + grp;
   const int W = D.bitmap_words;
@@ -448,7 +460,11 @@ __global__ void __launch_bounds__(256) k_rebuild_grid(Dev D, const Tables* __res
     const int x0 = cxi > 0 ? cxi - 1 : 0, x1 = cxi + 1 < G.nx ? cxi + 1 : G.nx - 1;
     const size_t cbase = (size_t)c * kGridCells;
     const int* cb = D.cell_beads + P.off;
+    const int off = P.off;
     const double boxl = P.boxl, qbox = P.qbox;
+    const double ox = ux[g], oy = uy[g], oz = uz[g];
+    const int gt = D.type[g];
+    const bool nlcrit = crit == SOPG_CUTOFF_NEIGHBOR;
     for (int dz = -1; dz <= 1; dz++) {
       const int z = czi + dz;
       if (z < 0 || z >= G.nz) continue;
@@ -461,18 +477,19 @@ __global__ void __launch_bounds__(256) k_rebuild_grid(Dev D, const Tables* __res
         for (int s = s0 + l; s < s1; s += LG) {
           const int cc = cb[s];
           if (!(cc - b > 2 || b - cc > 2)) continue;
-          const int gc = P.off + cc;
-          const int i = gc < g ? gc : g, j = gc < g ? g : gc;
-          const int it = D.type[i], jt = D.type[j];
-          const double ddx = mimg(ux[j] - ux[i], boxl, qbox);
-          const double ddy = mimg(uy[j] - uy[i], boxl, qbox);
-          const double ddz = mimg(uz[j] - uz[i], boxl, qbox);
+          const int gc = off + cc;
+          const int side = gc < g ? 1 : 0;  // 1: the partner is the pair's first bead
+          const int ct = D.type[gc];
+          const int tpi = side ? ct * 3 + gt : gt * 3 + ct;
+          const double ddx = pair_d(side, ox, ux[gc], boxl, qbox);
+          const double ddy = pair_d(side, oy, uy[gc], boxl, qbox);
+          const double ddz = pair_d(side, oz, uz[gc], boxl, qbox);
           const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-          const bool keep = (crit == SOPG_CUTOFF_NEIGHBOR) ? d2 < T->nlrc2[it][jt] : cell_adjacent(D, P, i, j);
+          const bool keep = nlcrit ? d2 < cut2[tpi][0] : cell_adjacent(D, P, side ? gc : g, side ? g : gc);
           if (keep) {
             const uint32_t bit = 1u << (cc & 31);
             atomicOr(&nlb[cc >> 5], bit);
-            if (d2 < T->plrc2[it][jt]) atomicOr(&plb[cc >> 5], bit);
+            if (d2 < cut2[tpi][1]) atomicOr(&plb[cc >> 5], bit);
           }
         }
       }
@@ -492,6 +509,8 @@ __global__ void __launch_bounds__(256) k_rebuild_grid(Dev D, const Tables* __res
   if (!active) return;
   // emit in ascending partner order (the master list's order for bead b)
   const ChainP& P = *Pp;
+  const int off = P.off;
+  const uint32_t gt = (uint32_t)D.type[g];
   uint32_t* S = D.slot + (size_t)g * D.cap;
   const uint32_t pbit = 1u << (30 + par);
   int cnt = 0;
@@ -511,11 +530,11 @@ __global__ void __launch_bounds__(256) k_rebuild_grid(Dev D, const Tables* __res
       const int k = __builtin_ctz(bits);
       bits &= bits - 1u;
       const int c = w * 32 + k;
-      const int gc = P.off + c;
+      const int gc = off + c;
       const int side = gc < g ? 1 : 0;
-      const int i = side ? gc : g, j = side ? g : gc;
+      const uint32_t ct = (uint32_t)D.type[gc];
       if (idx < D.cap)
-        S[idx] = slot_make((uint32_t)c, (uint32_t)side, (uint32_t)D.type[i], (uint32_t)D.type[j]) |
+        S[idx] = slot_make((uint32_t)c, (uint32_t)side, side ? ct : gt, side ? gt : ct) |
                  (((pls >> k) & 1u) ? pbit : 0u);
       idx++;
     }
@@ -614,13 +633,6 @@ __global__ void k_velupdate(Dev D, int g0, int g1) {
 //        F_ACCUM   add to the stored force instead of starting from zero
 enum { F_STORE = 1, F_VEL = 2, F_ENERGY = 4, F_ENERGY_ALL = 8, F_NOISE = 16, F_BOND = 32, F_ANGLE = 64,
        F_VDW = 128, F_ACCUM = 256 };
-
-// Pair displacement u[j] - u[i] seen from bead g: c is g's own coordinate, o
-// the partner's.  side 0 (g is i): o - c; side 1 (g is j): c - o -- the same
-// subtraction the reference performs, so the same bits.
-__device__ __forceinline__ double pair_d(int side, double c, double o, double boxl, double qbox) {
-  return mimg(side ? c - o : o - c, boxl, qbox);
-}
 
 // Type-pair tables the pair loops read, staged in LDS (index it * 3 + jt).
 enum { TP_FREP = 0, TP_S6, TP_S12, TP_PLRC2, TP_CREP, TP_FATT, TP_CATT, TP_N };
@@ -742,18 +754,31 @@ __global__ void __launch_bounds__(256) k_force(Dev D, const Tables* __restrict__
   // native LJ 12-6 over the pair list (energy.cpp:274-311): F_i += f, F_j -= f
   if (flags & F_VDW) {
     const int p0 = D.att_beg[g], p1 = D.att_end[g];
+    // software pipeline as in the non-native loop below
+    uint8_t fn = 0;
+    double pxn = 0.0, pyn = 0.0, pzn = 0.0;
+    if (p0 + l < p1) {
+      fn = D.att_flag[p0 + l];
+      const int o = D.att_partner[p0 + l];
+      pxn = ux[o]; pyn = uy[o]; pzn = uz[o];
+    }
     for (int base = p0; base < p1; base += L) {
       const int q = base + l;
+      const uint8_t f = fn;
+      const double px = pxn, py = pyn, pz = pzn;
+      if (q + L < p1) {
+        fn = D.att_flag[q + L];
+        const int o = D.att_partner[q + L];
+        pxn = ux[o]; pyn = uy[o]; pzn = uz[o];
+      }
       bool contrib = false;
       double fx = 0.0, fy = 0.0, fz = 0.0;
       if (q < p1) {
-        const uint8_t f = D.att_flag[q];
         if (f & 1) {
           const int tt = D.att_t[q], side = (tt >> 4) & 1, tpi = (tt & 3) * 3 + ((tt >> 2) & 3);
-          const int o = D.att_partner[q];
-          const double dx = pair_d(side, cx, ux[o], boxl, qbox);
-          const double dy = pair_d(side, cy, uy[o], boxl, qbox);
-          const double dz = pair_d(side, cz, uz[o], boxl, qbox);
+          const double dx = pair_d(side, cx, px, boxl, qbox);
+          const double dy = pair_d(side, cy, py, boxl, qbox);
+          const double dz = pair_d(side, cz, pz, boxl, qbox);
           const double d2 = dx * dx + dy * dy + dz * dz;
           if ((f >> (1 + pr)) & 1) {
             const double r6 = D.att_r6[q], r12 = D.att_r12[q];
@@ -785,17 +810,32 @@ __global__ void __launch_bounds__(256) k_force(Dev D, const Tables* __restrict__
     uint32_t* S = D.slot + (size_t)g * D.cap;
     const uint32_t rbit = 1u << (30 + pr), wbit = 1u << (30 + pw);
     const double thr_rep = T->thr_rep;
+    // software pipeline: the next chunk's slot word and partner coordinates
+    // are loaded before the current chunk is evaluated and folded
+    uint32_t wn = 0u;
+    double pxn = 0.0, pyn = 0.0, pzn = 0.0;
+    if (l < cnt) {
+      wn = S[l];
+      const int o = off + (int)(wn & kPartnerMask);
+      pxn = ux[o]; pyn = uy[o]; pzn = uz[o];
+    }
     for (int base = 0; base < cnt; base += L) {
       const int q = base + l;
+      const uint32_t w = wn;
+      const double px = pxn, py = pyn, pz = pzn;
+      if (q + L < cnt) {
+        wn = S[q + L];
+        const int o = off + (int)(wn & kPartnerMask);
+        pxn = ux[o]; pyn = uy[o]; pzn = uz[o];
+      }
       bool contrib = false;
       double fx = 0.0, fy = 0.0, fz = 0.0;
       if (q < cnt) {
-        const uint32_t w = S[q];
-        const int o = off + (int)(w & kPartnerMask), side = (w >> 25) & 1;
+        const int side = (w >> 25) & 1;
         const int tpi = (int)((w >> 26) & 3) * 3 + (int)((w >> 28) & 3);
-        const double dx = pair_d(side, cx, ux[o], boxl, qbox);
-        const double dy = pair_d(side, cy, uy[o], boxl, qbox);
-        const double dz = pair_d(side, cz, uz[o], boxl, qbox);
+        const double dx = pair_d(side, cx, px, boxl, qbox);
+        const double dy = pair_d(side, cy, py, boxl, qbox);
+        const double dz = pair_d(side, cz, pz, boxl, qbox);
         const double d2 = dx * dx + dy * dy + dz * dz;
         if (w & rbit) {
           const double s6 = tp[tpi][TP_S6], s12 = tp[tpi][TP_S12];
