@@ -30,8 +30,9 @@ sum over chains of 192 N + 16 nbnd + 8 nang + 40 nnl_att + 8 nnl_rep, against
 8 TB/s HBM.
 
 cpu_baseline: the reference itself (oracle/_ref, compiled from SOP-CPU's
-sources with its own flags), one host core, on a bounded sample of the same
-workload (named in `sample`).
+sources with its own flags) as --cpu-workers single-thread processes at once
+(one chain each, at evenly spaced size quantiles), on a bounded sample of the
+same workload (named in `sample`).
 """
 import argparse
 import json
@@ -222,55 +223,95 @@ def gpu_serve(jobs, w, hbm_budget, max_chains, chunk, lanes):
                 p50_wait_s=float(np.percentile(wait, 50)), p99_wait_s=float(np.percentile(wait, 99)),
                 makespan_s=span, rounds=st["rounds"], busy_s=st["busy_s"], peak_resident=peak_active,
                 records=nrec, offered_load=offered,
-                saturated=bool(work / span < 0.9 * offered))
+                jobs_waited_for_capacity=sum(k["rounds_waited"] > 0 for k in done),
+                max_rounds_waited=max(k["rounds_waited"] for k in done))
 
 
-def cpu_baseline(chains, w, budget_s=20.0, max_n=3200):
-    """The reference on a size-stratified sample of the workload's chains, one core."""
+def _cpu_worker(job):
+    """One reference process: set up one chain from its files and run the
+    underdamped_ctrl loop body (sop.cpp:153-189) until the time budget is
+    spent (at least one rebuild period).  Returns (N, steps, seconds)."""
+    d, n, v, cut, nnlup, extra, budget = job
+    # the reference reports its loading on stdout; keep this process's stdout
+    # (shared with the benchmark's single JSON line) clean
+    devnull = os.open(os.devnull, os.O_WRONLY)
+    os.dup2(devnull, 1)
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
     from refs import Ref, setup_system, std_cmds
+    r = Ref()
+    setup_system(r, d, std_cmds(temp=KT, cutoff=cut, nnlup=nnlup, nup=NUP, extra=extra))
+    r.set_coords(2, v)
+    rebuild = "update_cell_list" if cut == "celllist" else "update_neighbor_list"
+    r.call(rebuild)
+    r.call("update_pair_list")
+    r.call("set_potential_forces")
+    r.call("energy_eval")
+    r.call("force_eval")
+    inlup, steps = 1, 0
+    t0 = time.perf_counter()
+    while steps < nnlup or time.perf_counter() - t0 < budget:
+        if inlup % nnlup == 0:
+            r.call(rebuild)
+            inlup = 0
+        inlup += 1
+        r.call("update_pair_list")
+        r.call("underdamped_iteration")
+        steps += 1
+    return n, steps, time.perf_counter() - t0
+
+
+def cpu_baseline(chains, w, workers, budget_s=15.0, max_n=3200):
+    """The reference (oracle/_ref: SOP-CPU compiled from its own sources with
+    its own flags) on `workers` host cores at once: one single-thread process
+    per chain, chains at evenly spaced size quantiles of the workload."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from refs import Ref
     if not Ref.available():
         return None
-    order = sorted(range(len(chains)), key=lambda c: len(chains[c][0]))
-    picks = []
-    for q in (0.1, 0.3, 0.5, 0.7, 0.9):
-        c = order[min(int(q * len(order)), len(order) - 1)]
-        if len(chains[c][0]) // 3 <= max_n and c not in picks:
-            picks.append(c)
+    import multiprocessing as mp
+    order = sorted((c for c in range(len(chains)) if len(chains[c][0]) // 3 <= max_n),
+                   key=lambda c: len(chains[c][0]))
+    k = min(workers, len(order))
+    picks = sorted({order[min(int((i + 0.5) * len(order) / k), len(order) - 1)] for i in range(k)})
     cut = "celllist" if w["cutoff"] == sopg.CUTOFF_CELL else "neighborlist"
     extra = [("boxl", str(int(w["boxl"]))), ("ncell", str(int(w["ncell"])))] if cut == "celllist" else []
-    per = budget_s / max(len(picks), 1)
-    bs, secs, desc = 0.0, 0.0, []
-    for c in picks:
-        x, v, _ = chains[c]
-        n = len(x) // 3
-        with tempfile.TemporaryDirectory() as d:
-            sopg.write_inputs(x, d, vel=v)
-            r = Ref()
-            setup_system(r, d, std_cmds(temp=KT, cutoff=cut, nnlup=w["nnlup"], nup=NUP, extra=extra))
-        r.set_coords(2, v)
-        r.call("update_cell_list" if cut == "celllist" else "update_neighbor_list")
-        r.call("update_pair_list")
-        r.call("set_potential_forces")
-        r.call("energy_eval")
-        r.call("force_eval")
-        inlup, steps = 1, 0
+    with tempfile.TemporaryDirectory() as tmp:
+        jobs = []
+        for c in picks:
+            x, v, _ = chains[c]
+            jobs.append((os.path.join(tmp, f"c{c}"), len(x) // 3, v, cut, w["nnlup"], extra, budget_s))
+        with ThreadPoolExecutor(max_workers=len(jobs)) as ex:
+            list(ex.map(lambda jc: sopg.write_inputs(chains[jc[1]][0], jc[0][0], vel=chains[jc[1]][1]),
+                        zip(jobs, picks)))
         t0 = time.perf_counter()
-        while steps < 20 * w["nnlup"] and (steps < w["nnlup"] or time.perf_counter() - t0 < per):
-            if inlup % w["nnlup"] == 0:
-                r.call("update_cell_list" if cut == "celllist" else "update_neighbor_list")
-                inlup = 0
-            inlup += 1
-            r.call("update_pair_list")
-            r.call("underdamped_iteration")
-            steps += 1
-        dt = time.perf_counter() - t0
-        bs += n * steps
-        secs += dt
-        desc.append(f"N={n}:{steps}")
-    return dict(value=bs / secs, unit="bead-steps/s", cores=1, kind="reference",
-                sample=f"SOP-CPU compiled from its sources (g++ -O3), one core, chains at the 10/30/50/70/90% "
-                       f"size quantiles of the workload (steps run: {', '.join(desc)}), {secs:.1f} s; "
-                       f"rate = sum(N*steps)/sum(seconds)")
+        with mp.get_context("spawn").Pool(len(jobs)) as pool:
+            res = pool.map(_cpu_worker, jobs)
+        wall = time.perf_counter() - t0
+    rate = sum(n * st / sec for n, st, sec in res)
+    desc = ", ".join(f"N={n}:{st} steps/{sec:.1f} s" for n, st, sec in res)
+    return dict(value=rate, unit="bead-steps/s", cores=len(jobs), kind="reference",
+                sample=f"SOP-CPU compiled from its sources (g++ -O3, its own flags), {len(jobs)} single-thread "
+                       f"processes at once, one chain each at evenly spaced size quantiles of the workload "
+                       f"(N <= {max_n}; {desc}); value = sum over processes of N*steps/seconds "
+                       f"({wall:.1f} s wall with process start and setup)")
+
+
+def profiled_traffic(workload):
+    """HBM bytes per k_force launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE
+    (scaled by a copy kernel of known size) for this workload, as committed in
+    profiles/, with the launch time that profile saw."""
+    path = os.path.join(ROOT, "profiles", f"k_force_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if ":" in line and line.split(":")[0] in ("traffic_bytes_per_launch", "avg_launch_us"):
+            k, v = line.split(":", 1)
+            vals[k] = float(v)
+    if "traffic_bytes_per_launch" not in vals:
+        return None
+    return dict(bytes=vals["traffic_bytes_per_launch"], avg_launch_us=vals.get("avg_launch_us"),
+                source=os.path.relpath(path, ROOT))
 
 
 def main():
@@ -283,6 +324,8 @@ def main():
     ap.add_argument("--list-lanes", type=int, default=0)
     ap.add_argument("--plain-steps", type=int, default=100)
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-workers", type=int, default=min(16, os.cpu_count() or 1),
+                    help="reference processes run at once for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
@@ -331,8 +374,12 @@ def main():
                      hbm_budget_gib=a.serve_budget_gib, chunk=a.serve_chunk)
         print(f"[bench] serve {serve['value']:.4g} bead-steps/s p50 {serve['p50_latency_s']:.3f} s "
               f"p99 {serve['p99_latency_s']:.3f} s", file=sys.stderr, flush=True)
-    cpu = None if a.no_cpu else cpu_baseline(chains, w)
+    cpu = None if a.no_cpu else cpu_baseline(chains, w, a.cpu_workers)
+    if cpu:
+        print(f"[bench] cpu baseline {cpu['value']:.4g} bead-steps/s on {cpu['cores']} cores", file=sys.stderr,
+              flush=True)
     tot = lambda k: sum(c[k] for c in g["counts"])
+    prof_t = profiled_traffic(a.workload)
     line = {
         "metric": "bead-steps/s (batched vs plain) at FP64 force parity; p50/p99 job latency",
         "value": value,
@@ -359,7 +406,11 @@ def main():
         "p99_latency_s": None if serve is None else serve["p99_latency_s"],
         "roofline": {"bound": "hbm", "kernel": "k_force (fused step)", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "bytes_per_launch": g["bytes_step"], "avg_launch_us": avg_s * 1e6,
+                     "traffic": None if prof_t is None else prof_t["bytes"],
+                     "traffic_source": None if prof_t is None else
+                     f"{prof_t['source']} (rocprofv3 FETCH_SIZE + WRITE_SIZE per k_force launch, scaled by a "
+                     f"1 GiB copy; that profile's k_force average {prof_t['avg_launch_us']} us)",
+                     "bytes_per_launch": g["bytes_step"], "avg_launch_us": avg_s * 1e6,
                      "launches": step_n, "rebuild_avg_us": reb_ms / max(reb_n, 1) * 1e3,
                      "rebuilds": reb_n, "observe_avg_us": obs_ms / max(obs_n, 1) * 1e3},
         "cpu_baseline": cpu,

@@ -32,7 +32,7 @@ typedef struct {
   double hbm_budget;  /* bytes of device memory jobs may hold together (0: 64 GiB) */
   int max_chains;     /* jobs resident at once (0: 4096) */
   long chunk;         /* steps per scheduling round, at most (0: 1000) */
-  int lanes;          /* step-kernel lanes per bead (0: library default) */
+  int lanes;          /* step-kernel lanes per bead (0: autotuned as the resident set changes) */
   int capacity;       /* per-bead non-native list capacity (0: library default) */
 } sopg_server_config;
 
@@ -48,6 +48,7 @@ typedef struct {
   double t_arrival, t_admit, t_finish;
   double footprint;           /* bytes charged against the budget */
   long nrec;                  /* records collected so far */
+  long rounds_waited;         /* scheduling rounds that ended with the job still queued */
 } sopg_job_info;
 
 int sopg_server_create(int device, const sopg_server_config* cfg, sopg_server** out);

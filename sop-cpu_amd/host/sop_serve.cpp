@@ -23,6 +23,7 @@ struct Job {
   int state = SOPG_JOB_QUEUED;
   int chain = -1;
   long nbead = 0, nsteps = 0, done = 0;
+  long waited = 0;  // rounds that ended with this job still queued
   double t_arrival = 0, t_admit = -1, t_finish = -1, footprint = 0;
   sopg_chain_params p{};
   // topology copy
@@ -67,6 +68,7 @@ struct sopg_server {
   int resident = 0;
   double bead_steps = 0, busy = 0;
   long rounds = 0;
+  double tuned_beads = 0;  // resident beads when the lanes were last autotuned
   std::vector<sopg_record> recbuf;
   double now() const {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -250,6 +252,7 @@ int sopg_server_poll(sopg_server* s, int* active, int* queued) {
       if (rc == SOPG_ERR_HIP) return rc;
     }
   }
+  for (long id : s->queue) s->jobs[id].waited++;
   if (s->resident > 0) {
     long chunk = s->cfg.chunk;
     double beads = 0;
@@ -259,6 +262,12 @@ int sopg_server_poll(sopg_server* s, int* active, int* queued) {
         chunk = std::min(chunk, j.nsteps - j.done);
         beads += (double)j.nbead;
       }
+    // the fastest launch shape depends on the batch: retune when the resident
+    // bead count has moved by more than a quarter since the last tuning
+    if (s->cfg.lanes <= 0 && (s->tuned_beads == 0 || beads > 1.25 * s->tuned_beads || beads < 0.8 * s->tuned_beads)) {
+      if ((rc = sopg_autotune(s->ctx, nullptr))) return ctx_fail(s, rc, "sopg_autotune");
+      s->tuned_beads = beads;
+    }
     long nrec_max = 0;
     for (long id : s->by_chain)
       if (id >= 0) nrec_max += chunk / s->jobs[id].p.nup + 1;
@@ -299,6 +308,7 @@ int sopg_server_job(const sopg_server* s, long id, sopg_job_info* info) {
   info->t_finish = j->t_finish;
   info->footprint = j->footprint;
   info->nrec = (long)j->rec.size();
+  info->rounds_waited = j->waited;
   return SOPG_OK;
 }
 
@@ -336,6 +346,7 @@ int sopg_server_release(sopg_server* s, long id) {
   light.t_admit = j.t_admit;
   light.t_finish = j.t_finish;
   light.footprint = j.footprint;
+  light.waited = j.waited;
   j = std::move(light);
   return SOPG_OK;
 }

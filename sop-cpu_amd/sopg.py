@@ -73,7 +73,8 @@ class ServerConfig(C.Structure):
 class JobInfo(C.Structure):
     _fields_ = [("state", C.c_int), ("chain", C.c_int), ("nbead", C.c_long), ("nsteps", C.c_long),
                 ("steps_done", C.c_long), ("t_arrival", C.c_double), ("t_admit", C.c_double),
-                ("t_finish", C.c_double), ("footprint", C.c_double), ("nrec", C.c_long)]
+                ("t_finish", C.c_double), ("footprint", C.c_double), ("nrec", C.c_long),
+                ("rounds_waited", C.c_long)]
 
 
 JOB_QUEUED, JOB_ACTIVE, JOB_DONE, JOB_REJECTED = 0, 1, 2, 3

@@ -61,6 +61,7 @@ def test_served_jobs_equal_alone(inputs_root):
         held = sum(k["footprint"] for k in info if k["t_admit"] <= t < k["t_finish"])
         assert held <= budget
     assert max(i["t_admit"] - i["t_arrival"] for i in info) > 0.0  # someone waited
+    assert any(i["rounds_waited"] > 0 for i in info)  # ... for memory, across rounds
     for (n, seed, nsteps, kT), (x, v), j in zip(JOBS, data, ids):
         print(f"[gpu] job {j} alone N={n} steps={nsteps}", flush=True)
         st, recs = _alone(x, v, kT, 900 + seed, nsteps)
