@@ -113,6 +113,20 @@ __device__ __forceinline__ double pair_d(int side, double c, double o, double bo
   return mimg(side ? c - o : o - c, boxl, qbox);
 }
 
+// All three components at once, with one branch for the common case in which
+// every component takes mimg's exact fast path (|d| <= boxl/4).
+__device__ __forceinline__ void pair_d3(int side, double cx, double cy, double cz, double ox, double oy, double oz,
+                                        double boxl, double qbox, double& dx, double& dy, double& dz) {
+  const double rx = side ? cx - ox : ox - cx;
+  const double ry = side ? cy - oy : oy - cy;
+  const double rz = side ? cz - oz : oz - cz;
+  if (fabs(rx) <= qbox && fabs(ry) <= qbox && fabs(rz) <= qbox) {
+    dx = rx + 0.0; dy = ry + 0.0; dz = rz + 0.0;
+  } else {
+    dx = mimg(rx, boxl, qbox); dy = mimg(ry, boxl, qbox); dz = mimg(rz, boxl, qbox);
+  }
+}
+
 __device__ __forceinline__ bool cell_adjacent(const Dev& D, const ChainP& P, int i, int j) {
   // cell_list.cpp:24-44 with the cell indices precomputed per bead
   const double cx = mimg(D.cx[i] - D.cx[j], P.ncell, P.qncell);
@@ -481,9 +495,8 @@ __global__ void __launch_bounds__(256) k_rebuild_grid(Dev D, const Tables* __res
           const int side = gc < g ? 1 : 0;  // 1: the partner is the pair's first bead
           const int ct = D.type[gc];
           const int tpi = side ? ct * 3 + gt : gt * 3 + ct;
-          const double ddx = pair_d(side, ox, ux[gc], boxl, qbox);
-          const double ddy = pair_d(side, oy, uy[gc], boxl, qbox);
-          const double ddz = pair_d(side, oz, uz[gc], boxl, qbox);
+          double ddx, ddy, ddz;
+          pair_d3(side, ox, oy, oz, ux[gc], uy[gc], uz[gc], boxl, qbox, ddx, ddy, ddz);
           const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
           const bool keep = nlcrit ? d2 < cut2[tpi][0] : cell_adjacent(D, P, side ? gc : g, side ? g : gc);
           if (keep) {
@@ -715,9 +728,8 @@ __global__ void __launch_bounds__(256) k_force(Dev D, const Tables* __restrict__
       double fx = 0.0, fy = 0.0, fz = 0.0;
       if (has) {
         const int pq = D.bond_pq[q], o = pq >> 1, side = pq & 1;
-        const double dx = pair_d(side, cx, ux[o], boxl, qbox);
-        const double dy = pair_d(side, cy, uy[o], boxl, qbox);
-        const double dz = pair_d(side, cz, uz[o], boxl, qbox);
+        double dx, dy, dz;
+        pair_d3(side, cx, cy, cz, ux[o], uy[o], uz[o], boxl, qbox, dx, dy, dz);
         const double d = sqrt(dx * dx + dy * dy + dz * dz);
         const double dev = d - D.bond_r0[q];
         const double dev2 = dev * dev;
@@ -738,9 +750,8 @@ __global__ void __launch_bounds__(256) k_force(Dev D, const Tables* __restrict__
       double fx = 0.0, fy = 0.0, fz = 0.0;
       if (has) {
         const int pq = D.ang_pq[q], o = pq >> 1, side = pq & 1;
-        const double dx = pair_d(side, cx, ux[o], boxl, qbox);
-        const double dy = pair_d(side, cy, uy[o], boxl, qbox);
-        const double dz = pair_d(side, cz, uz[o], boxl, qbox);
+        double dx, dy, dz;
+        pair_d3(side, cx, cy, cz, ux[o], uy[o], uz[o], boxl, qbox, dx, dy, dz);
         const double d = sqrt(dx * dx + dy * dy + dz * dz);
         const double d8 = pow8_cr(d);
         const double co1 = T->f_ang_ss_coeff / d8;
@@ -776,9 +787,8 @@ __global__ void __launch_bounds__(256) k_force(Dev D, const Tables* __restrict__
       if (q < p1) {
         if (f & 1) {
           const int tt = D.att_t[q], side = (tt >> 4) & 1, tpi = (tt & 3) * 3 + ((tt >> 2) & 3);
-          const double dx = pair_d(side, cx, px, boxl, qbox);
-          const double dy = pair_d(side, cy, py, boxl, qbox);
-          const double dz = pair_d(side, cz, pz, boxl, qbox);
+          double dx, dy, dz;
+          pair_d3(side, cx, cy, cz, px, py, pz, boxl, qbox, dx, dy, dz);
           const double d2 = dx * dx + dy * dy + dz * dz;
           if ((f >> (1 + pr)) & 1) {
             const double r6 = D.att_r6[q], r12 = D.att_r12[q];
@@ -833,9 +843,8 @@ __global__ void __launch_bounds__(256) k_force(Dev D, const Tables* __restrict__
       if (q < cnt) {
         const int side = (w >> 25) & 1;
         const int tpi = (int)((w >> 26) & 3) * 3 + (int)((w >> 28) & 3);
-        const double dx = pair_d(side, cx, px, boxl, qbox);
-        const double dy = pair_d(side, cy, py, boxl, qbox);
-        const double dz = pair_d(side, cz, pz, boxl, qbox);
+        double dx, dy, dz;
+        pair_d3(side, cx, cy, cz, px, py, pz, boxl, qbox, dx, dy, dz);
         const double d2 = dx * dx + dy * dy + dz * dz;
         if (w & rbit) {
           const double s6 = tp[tpi][TP_S6], s12 = tp[tpi][TP_S12];
