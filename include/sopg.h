@@ -184,6 +184,14 @@ int sopg_run(sopg_ctx* ctx, long nsteps, const double* noise, sopg_record* rec, 
  * reference's inlup / iup counters as they stand before that step. */
 int sopg_set_step(sopg_ctx* ctx, int chain, double istep, int inlup, int iup);
 int sopg_get_step(const sopg_ctx* ctx, int chain, double* istep, int* inlup, int* iup);
+/* Lossless checkpoint of one chain (no counterpart in the reference, whose
+ * restart files -- coordinates, velocities and the generator state, io.cpp
+ * and random_generator.cpp:24-70 -- rebuild the lists at restart): the state,
+ * the step and noise counters and the current neighbor lists, so that
+ * restoring it into a chain of the same topology, in any context, continues
+ * the run exactly as if it had not stopped.  buf = NULL: *size = bytes needed. */
+int sopg_checkpoint(sopg_ctx* ctx, int chain, void* buf, long cap, long* size);
+int sopg_restore(sopg_ctx* ctx, int chain, const void* buf, long size);
 /* Block until all work queued on the context's stream has finished. */
 int sopg_sync(sopg_ctx* ctx);
 

@@ -100,7 +100,7 @@ struct HostChain {
   bool rep_explicit = false;
   std::vector<int> ri, rj, rit, rjt;
   std::vector<int8_t> type;
-  std::vector<int> q0;  // device incidence index of each native entry's first bead
+  std::vector<int> q0, q1;  // device incidence index of each native entry's first / second bead
   double istep = 1.0;
   int inlup = 1, iup = 1;
   uint64_t noise_base = 0;
@@ -490,19 +490,20 @@ int materialize(sopg_ctx* c, int id) {
   }
   // per incidence: partner, and the pair's constants in the bead's own order
   const size_t nq = code.size();
-  std::vector<int> partner(nq), q0(nt);
+  std::vector<int> partner(nq), q0(nt), q1(nt);
   std::vector<int8_t> qt(nq);
   std::vector<double> qr6(nq), qr12(nq), qpl(nq);
   for (size_t q = 0; q < nq; q++) {
     const size_t k = (size_t)(code[q] >> 1) - h.at[P_ATT];
     partner[q] = off + ((code[q] & 1) ? h.ai[k] : h.aj[k]);
-    if (!(code[q] & 1)) q0[k] = (int)(h.at[P_TINC] + q);
+    ((code[q] & 1) ? q1 : q0)[k] = (int)(h.at[P_TINC] + q);
     qt[q] = (int8_t)(at[k] | ((code[q] & 1) << 4));
     qr6[q] = r6[k];
     qr12[q] = r12[k];
     qpl[q] = plrc2[k];
   }
   h.q0.assign(q0.begin(), q0.end());
+  h.q1.assign(q1.begin(), q1.end());
   const size_t ta = h.at[P_ATT], tq = h.at[P_TINC];
   if ((rc = upload_at(c, D.att_beg, off, beg)) || (rc = upload_at(c, D.att_end, off, end)) ||
       (rc = upload_at(c, D.att_code, tq, code)) || (rc = upload_at(c, D.att_partner, tq, partner)) ||
@@ -1198,6 +1199,152 @@ int sopg_set_step(sopg_ctx* c, int chain, double istep, int inlup, int iup) {
   h.istep = istep;
   h.inlup = inlup;
   h.iup = iup;
+  return upload_chainp(c, chain);
+}
+
+// ------------------------------------------------------------ checkpoints
+//
+// Blob: header, then pos, unc_pos, vel, force (AoS, 3N doubles each), the
+// native NL flag of each master entry (bytes), the non-native slot count of
+// each bead (ints), the slot words (pair-list bits cleared) and, for explicit
+// master lists, their master indices.
+
+namespace {
+struct CkptHeader {
+  uint32_t magic, version;
+  int32_t nbead, natt, rep_explicit, pad;
+  int64_t nslots;
+  double istep;
+  int32_t inlup, iup;
+  uint64_t noise_base;
+};
+constexpr uint32_t kCkptMagic = 0x43504f53u;  // "SOPC"
+
+int ckpt_lists(sopg_ctx* c, int chain, std::vector<uint8_t>& nl, std::vector<int>& cnt, std::vector<uint32_t>& w,
+               std::vector<int>& k) {
+  HostChain& h = c->chains[chain];
+  const ChainP& P = c->cp[chain];
+  const int n = P.n, cap = c->D.cap;
+  const size_t nq = 2 * (size_t)P.natt, tq = h.at[P_TINC];
+  std::vector<uint8_t> f(nq);
+  std::vector<uint32_t> sl((size_t)n * cap);
+  std::vector<int> sk;
+  cnt.resize(n);
+  if (nq) HIP_OK(c, hipMemcpyAsync(f.data(), c->D.att_flag + tq, nq, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(c, hipMemcpyAsync(cnt.data(), c->D.cnt + P.off, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(c, hipMemcpyAsync(sl.data(), c->D.slot + (size_t)P.off * cap, sizeof(uint32_t) * n * cap,
+                           hipMemcpyDeviceToHost, c->stream));
+  if (h.rep_explicit) {
+    sk.resize((size_t)n * cap);
+    HIP_OK(c, hipMemcpyAsync(sk.data(), c->D.slot_k + (size_t)P.off * cap, sizeof(int) * n * cap,
+                             hipMemcpyDeviceToHost, c->stream));
+  }
+  HIP_OK(c, hipStreamSynchronize(c->stream));
+  nl.resize(P.natt);
+  for (int a = 0; a < P.natt; a++) nl[a] = f[(size_t)h.q0[a] - tq] & 1;
+  w.clear();
+  k.clear();
+  for (int b = 0; b < n; b++)
+    for (int q = 0; q < cnt[b]; q++) {
+      w.push_back(sl[(size_t)b * cap + q] & ~(3u << 30));
+      if (h.rep_explicit) k.push_back(sk[(size_t)b * cap + q]);
+    }
+  return SOPG_OK;
+}
+}  // namespace
+
+int sopg_checkpoint(sopg_ctx* c, int chain, void* buf, long cap, long* size) {
+  HostChain* h;
+  ChainP* P;
+  int rc = get_chain(c, chain, &h, &P);
+  if (rc) return rc;
+  std::vector<uint8_t> nl;
+  std::vector<int> cnt, k;
+  std::vector<uint32_t> w;
+  if ((rc = ckpt_lists(c, chain, nl, cnt, w, k))) return rc;
+  const size_t n3 = 3 * (size_t)P->n;
+  const size_t need = sizeof(CkptHeader) + 4 * n3 * sizeof(double) + nl.size() + cnt.size() * sizeof(int) +
+                      w.size() * sizeof(uint32_t) + k.size() * sizeof(int);
+  if (size) *size = (long)need;
+  if (!buf) return SOPG_OK;
+  if (cap < (long)need) return fail(c, SOPG_ERR_ARG, "checkpoint buffer too small (query the size with buf = NULL)");
+  CkptHeader hd{kCkptMagic, 1, P->n, P->natt, h->rep_explicit ? 1 : 0, 0, (int64_t)w.size(),
+                h->istep, h->inlup, h->iup, h->noise_base};
+  char* o = (char*)buf;
+  std::memcpy(o, &hd, sizeof hd);
+  o += sizeof hd;
+  double* st = (double*)o;
+  if ((rc = sopg_get_state(c, chain, st, st + n3, st + 2 * n3, st + 3 * n3))) return rc;
+  o += 4 * n3 * sizeof(double);
+  auto put = [&](const void* p, size_t bytes) {
+    if (bytes) std::memcpy(o, p, bytes);
+    o += bytes;
+  };
+  put(nl.data(), nl.size());
+  put(cnt.data(), cnt.size() * sizeof(int));
+  put(w.data(), w.size() * sizeof(uint32_t));
+  put(k.data(), k.size() * sizeof(int));
+  return SOPG_OK;
+}
+
+int sopg_restore(sopg_ctx* c, int chain, const void* buf, long size) {
+  HostChain* h;
+  ChainP* P;
+  int rc = get_chain(c, chain, &h, &P);
+  if (rc) return rc;
+  if (!buf || size < (long)sizeof(CkptHeader)) return fail(c, SOPG_ERR_ARG, "checkpoint too short");
+  CkptHeader hd;
+  std::memcpy(&hd, buf, sizeof hd);
+  if (hd.magic != kCkptMagic || hd.version != 1) return fail(c, SOPG_ERR_ARG, "not a chain checkpoint");
+  if (hd.nbead != P->n || hd.natt != P->natt || hd.rep_explicit != (h->rep_explicit ? 1 : 0))
+    return fail(c, SOPG_ERR_ARG, "checkpoint was taken from a chain of another topology");
+  if (hd.inlup < 1 || hd.inlup > h->p.nnlup || hd.iup < 1 || hd.iup > h->p.nup)
+    return fail(c, SOPG_ERR_ARG, "checkpoint step counters do not fit this chain's nnlup / nup");
+  const size_t n = P->n, n3 = 3 * n, cap = c->D.cap;
+  const size_t need = sizeof(CkptHeader) + 4 * n3 * sizeof(double) + (size_t)hd.natt + n * sizeof(int) +
+                      (size_t)hd.nslots * sizeof(uint32_t) + (hd.rep_explicit ? (size_t)hd.nslots * sizeof(int) : 0);
+  if ((size_t)size < need) return fail(c, SOPG_ERR_ARG, "checkpoint truncated");
+  const char* in = (const char*)buf + sizeof hd;
+  const double* st = (const double*)in;
+  in += 4 * n3 * sizeof(double);
+  const uint8_t* nl = (const uint8_t*)in;
+  in += hd.natt;
+  std::vector<int> cnt(n);
+  std::memcpy(cnt.data(), in, n * sizeof(int));
+  in += n * sizeof(int);
+  const uint32_t* w = (const uint32_t*)in;
+  in += (size_t)hd.nslots * sizeof(uint32_t);
+  const int* kk = (const int*)in;
+  // per-bead slots, validated against this chain and the context's capacity
+  std::vector<uint32_t> sl(n * cap, 0u);
+  std::vector<int> sk(h->rep_explicit ? n * cap : 0, 0);
+  size_t q = 0;
+  for (size_t b = 0; b < n; b++) {
+    if (cnt[b] < 0 || (size_t)cnt[b] > cap || q + cnt[b] > (size_t)hd.nslots)
+      return fail(c, SOPG_ERR_ARG, "checkpoint lists do not fit this context's per-bead capacity");
+    for (int e = 0; e < cnt[b]; e++, q++) {
+      if ((w[q] & kPartnerMask) >= n) return fail(c, SOPG_ERR_ARG, "checkpoint list entry out of range");
+      sl[b * cap + e] = w[q] & ~(3u << 30);
+      if (h->rep_explicit) sk[b * cap + e] = kk[q];
+    }
+  }
+  if (q != (size_t)hd.nslots) return fail(c, SOPG_ERR_ARG, "checkpoint list counts inconsistent");
+  if ((rc = sopg_put_state(c, chain, st, st + n3, st + 2 * n3, st + 3 * n3))) return rc;
+  const size_t nq = 2 * (size_t)P->natt, tq = h->at[P_TINC];
+  std::vector<uint8_t> f(nq, 0);
+  for (int a = 0; a < P->natt; a++) {
+    f[(size_t)h->q0[a] - tq] = nl[a] ? 1 : 0;
+    f[(size_t)h->q1[a] - tq] = nl[a] ? 1 : 0;
+  }
+  if ((rc = upload_at(c, c->D.att_flag, tq, f)) || (rc = upload_at(c, c->D.cnt, (size_t)P->off, cnt)) ||
+      (rc = upload_at(c, c->D.slot, (size_t)P->off * cap, sl)))
+    return rc;
+  if (h->rep_explicit && (rc = upload_at(c, c->D.slot_k, (size_t)P->off * cap, sk))) return rc;
+  h->istep = hd.istep;
+  h->inlup = hd.inlup;
+  h->iup = hd.iup;
+  h->noise_base = hd.noise_base;
+  h->pl_fresh = false;  // the pair list is refiltered from the restored lists before the next step
   return upload_chainp(c, chain);
 }
 

@@ -90,6 +90,7 @@ EXPORTS = [
     "sopgen_saw", "sopgen_native", "sopgen_maxwell", "sopgen_check", "sopgen_write_topology",
     "sopgen_write_xyz", "sopgen_natives", "sopg_force_terms", "sopg_set_binned",
     "sopg_get_lanes", "sopg_autotune", "sopg_reserve", "sopg_remove_chain", "sopg_get_step",
+    "sopg_checkpoint", "sopg_restore",
     "sopg_server_create", "sopg_server_destroy", "sopg_server_last_error", "sopg_server_now",
     "sopg_server_footprint", "sopg_server_submit", "sopg_server_poll", "sopg_server_job",
     "sopg_server_job_records", "sopg_server_job_state", "sopg_server_release", "sopg_server_stats",
@@ -120,6 +121,8 @@ def lib():
         L.sopg_reserve.argtypes = [vp, C.c_long, C.c_int]
         L.sopg_remove_chain.argtypes = [vp, C.c_int]
         L.sopg_get_step.argtypes = [vp, C.c_int, _DP, _IP, _IP]
+        L.sopg_checkpoint.argtypes = [vp, C.c_int, vp, C.c_long, C.POINTER(C.c_long)]
+        L.sopg_restore.argtypes = [vp, C.c_int, vp, C.c_long]
         L.sopg_force_terms.argtypes = [vp, C.c_int, C.c_int, vp]
         L.sopg_overdamped_iteration.argtypes = [vp, C.c_int, vp]
         L.sopg_add_chain.argtypes = [vp, C.POINTER(Topology), C.POINTER(ChainParams), _IP]
@@ -439,6 +442,18 @@ class Context:
 
     def set_step(self, chain, istep, inlup, iup):
         self._chk(self.L.sopg_set_step(self.h, chain, istep, inlup, iup))
+
+    def checkpoint(self, chain):
+        """Lossless checkpoint of one chain as bytes."""
+        n = C.c_long()
+        self._chk(self.L.sopg_checkpoint(self.h, chain, None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value)
+        self._chk(self.L.sopg_checkpoint(self.h, chain, buf, n.value, C.byref(n)))
+        return buf.raw[:n.value]
+
+    def restore(self, chain, blob):
+        buf = C.create_string_buffer(blob, len(blob))
+        self._chk(self.L.sopg_restore(self.h, chain, buf, len(blob)))
 
     def get_step(self, chain):
         istep, inlup, iup = C.c_double(), C.c_int(), C.c_int()

@@ -404,3 +404,31 @@ def test_autotune_keeps_results(inputs_root):
         a, b = ref.get_state(c), tuned.get_state(c)
         for k in ("unc", "vel", "force"):
             assert np.array_equal(a[k], b[k]), k
+
+
+def test_checkpoint_restore_lossless(inputs_root):
+    """A chain checkpointed mid-run (between list rebuilds) and restored into
+    another context -- at another offset, beside another chain -- continues
+    bit-identically to the uninterrupted run: state, records, noise stream."""
+    a_in = chain_inputs(inputs_root, 110, 81)[:2]
+    b_in = chain_inputs(inputs_root, 190, 82)[:2]
+    other = chain_inputs(inputs_root, 70, 83)[:2]
+    chains = [(a_in[0], a_in[1], 11), (b_in[0], b_in[1], 12)]
+    print("[gpu] checkpoint / restore", flush=True)
+    ctx = _philox_ctx(chains)
+    ctx.run(130)  # 130 is not a multiple of the rebuild period 20
+    blob = ctx.checkpoint(1)
+    tail = ctx.run(170)
+    fresh = _philox_ctx([(other[0], other[1], 13)])
+    cid = fresh.add_chain(sopg.chain_topology(b_in[0]),
+                          sopg.chain_params(cutoff=sopg.CUTOFF_NEIGHBOR, nnlup=20, nup=100,
+                                            noise_mode=sopg.NOISE_PHILOX, seed=12))
+    fresh.restore(cid, blob)
+    resumed = fresh.run(170)
+    assert fresh.get_step(cid) == ctx.get_step(1)
+    got, ref = fresh.get_state(cid), ctx.get_state(1)
+    for k in ("unc", "vel", "force", "pos"):
+        assert np.array_equal(got[k], ref[k]), k
+    assert [r for r in resumed if r["chain"] == cid] == [dict(r, chain=cid) for r in tail if r["chain"] == 1]
+    with pytest.raises(sopg.SopgError):
+        fresh.restore(0, blob)  # a chain of another topology
