@@ -71,6 +71,10 @@ int sopg_server_job(const sopg_server* srv, long job, sopg_job_info* info);
 int sopg_server_job_records(const sopg_server* srv, long job, sopg_record* rec, long cap, long* n);
 /* Final pos, unc_pos, vel, force of a finished job (3N each; any may be NULL). */
 int sopg_server_job_state(const sopg_server* srv, long job, double* pos, double* unc, double* vel, double* force);
+/* Preempt a resident job: checkpoint it losslessly (sopg_checkpoint), free its
+ * device memory and put it at the head of the queue.  When it is admitted
+ * again it continues exactly where it stopped. */
+int sopg_server_evict(sopg_server* srv, long job);
 /* Drop the host copy of a finished job's results. */
 int sopg_server_release(sopg_server* srv, long job);
 /* Totals: bead-steps advanced, rounds run, seconds spent inside rounds. */

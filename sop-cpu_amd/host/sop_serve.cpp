@@ -34,6 +34,8 @@ struct Job {
   std::vector<double> pos, vel;  // initial state, then final pos
   std::vector<double> unc, fvel, force;
   std::vector<sopg_record> rec;
+  std::vector<char> ckpt;  // set while the job is evicted: resumes from here
+  long evictions = 0;
   sopg_topology topo() const {
     sopg_topology t{};
     t.nbead = (int)nbead;
@@ -103,6 +105,18 @@ int admit(sopg_server* s, Job& j, long id) {
   int rc = sopg_add_chain(s->ctx, &t, &j.p, &chain);
   if (rc) return ctx_fail(s, rc, "sopg_add_chain");
   j.chain = chain;
+  if (!j.ckpt.empty()) {
+    // an evicted job continues from its checkpoint
+    if ((rc = sopg_restore(s->ctx, chain, j.ckpt.data(), (long)j.ckpt.size()))) return ctx_fail(s, rc, "sopg_restore");
+    j.ckpt.clear();
+    j.ckpt.shrink_to_fit();
+    if ((size_t)chain >= s->by_chain.size()) s->by_chain.resize(chain + 1, -1);
+    s->by_chain[chain] = id;
+    j.state = SOPG_JOB_ACTIVE;
+    s->used += j.footprint;
+    s->resident++;
+    return SOPG_OK;
+  }
   const std::vector<double> zero(3 * j.nbead, 0.0);
   if ((rc = sopg_put_state(s->ctx, chain, j.pos.data(), j.pos.data(), j.vel.data(), zero.data())))
     return ctx_fail(s, rc, "sopg_put_state");
@@ -348,6 +362,26 @@ int sopg_server_release(sopg_server* s, long id) {
   light.footprint = j.footprint;
   light.waited = j.waited;
   j = std::move(light);
+  return SOPG_OK;
+}
+
+int sopg_server_evict(sopg_server* s, long id) {
+  if (!find(s, id)) return SOPG_ERR_ARG;
+  Job& j = s->jobs[id];
+  if (j.state != SOPG_JOB_ACTIVE) return sfail(s, SOPG_ERR_ARG, "only a resident job can be evicted");
+  long n = 0;
+  int rc = sopg_checkpoint(s->ctx, j.chain, nullptr, 0, &n);
+  if (rc) return ctx_fail(s, rc, "sopg_checkpoint");
+  j.ckpt.resize((size_t)n);
+  if ((rc = sopg_checkpoint(s->ctx, j.chain, j.ckpt.data(), n, &n))) return ctx_fail(s, rc, "sopg_checkpoint");
+  if ((rc = sopg_remove_chain(s->ctx, j.chain))) return ctx_fail(s, rc, "sopg_remove_chain");
+  s->by_chain[j.chain] = -1;
+  j.chain = -1;
+  j.state = SOPG_JOB_QUEUED;
+  j.evictions++;
+  s->used -= j.footprint;
+  s->resident--;
+  s->queue.push_front(id);
   return SOPG_OK;
 }
 

@@ -94,6 +94,7 @@ EXPORTS = [
     "sopg_server_create", "sopg_server_destroy", "sopg_server_last_error", "sopg_server_now",
     "sopg_server_footprint", "sopg_server_submit", "sopg_server_poll", "sopg_server_job",
     "sopg_server_job_records", "sopg_server_job_state", "sopg_server_release", "sopg_server_stats",
+    "sopg_server_evict",
 ]
 
 _lib = None
@@ -159,6 +160,7 @@ def lib():
         L.sopg_server_job_records.argtypes = [vp, C.c_long, C.POINTER(Record), C.c_long, C.POINTER(C.c_long)]
         L.sopg_server_job_state.argtypes = [vp, C.c_long, vp, vp, vp, vp]
         L.sopg_server_release.argtypes = [vp, C.c_long]
+        L.sopg_server_evict.argtypes = [vp, C.c_long]
         L.sopg_server_stats.argtypes = [vp, _DP, C.POINTER(C.c_long), _DP]
         L.sopgen_saw.argtypes = [C.c_int, C.c_uint64, vp]
         L.sopgen_native.argtypes = [C.c_int, C.c_uint64, vp]
@@ -546,6 +548,9 @@ class Server:
 
     def release(self, job):
         self._chk(self.L.sopg_server_release(self.h, job))
+
+    def evict(self, job):
+        self._chk(self.L.sopg_server_evict(self.h, job))
 
     def stats(self):
         bs, busy = C.c_double(), C.c_double()

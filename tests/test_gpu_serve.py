@@ -72,3 +72,29 @@ def test_served_jobs_equal_alone(inputs_root):
         assert srv.records(j) == [dict(r, chain=chain) for r in recs]
         assert len(recs) == nsteps // 100
     srv.close()
+
+
+def test_evicted_job_resumes_exactly(inputs_root):
+    """A job preempted mid-run (checkpointed, its memory freed, requeued) and
+    later resumed ends bit-identically to the same job run alone."""
+    specs = [(80, 91, 900, 0.7), (150, 92, 1500, 1.0), (60, 93, 700, 0.8)]
+    data = [chain_inputs(inputs_root, n, seed)[:2] for (n, seed, _, _) in specs]
+    srv = sopg.Server(chunk=335)  # two rounds end between list rebuilds (670 % 20 != 0)
+    ids = [srv.submit(sopg.chain_topology(x), _params(kT, 700 + seed), x, v, nsteps)
+           for (n, seed, nsteps, kT), (x, v) in zip(specs, data)]
+    print("[gpu] evict and resume", flush=True)
+    srv.poll()
+    srv.poll()  # 670 steps done
+    srv.evict(ids[1])
+    assert srv.job(ids[1])["state"] == sopg.JOB_QUEUED and srv.job(ids[1])["steps_done"] == 670
+    for _ in range(100):
+        active, queued = srv.poll()
+        if active == 0 and queued == 0:
+            break
+    for (n, seed, nsteps, kT), (x, v), j in zip(specs, data, ids):
+        st, recs = _alone(x, v, kT, 700 + seed, nsteps)
+        got = srv.state(j)
+        for k in ("unc", "vel", "force", "pos"):
+            assert np.array_equal(got[k], st[k]), (j, k)
+        assert [dict(r, chain=0) for r in srv.records(j)] == recs
+    srv.close()
