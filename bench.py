@@ -24,8 +24,12 @@ timed steps of the batched form, inputs resident in HBM.  plain = the same
 chains advanced one chain per context (one chain per launch), K_plain steps
 each; speedup_vs_plain = value / plain.
 
+The timed run replays repeating step segments from captured HIP graphs (the
+library default; --no-graphs launches every kernel directly).
+
 roofline: the fused step kernel (SOPG_KERNEL_STEP), timed with HIP events on
-the library's stream over the timed region; algorithmic bytes per launch =
+the library's stream over a second run of the same batch (min(steps, 400)
+steps, direct launches); algorithmic bytes per launch =
 sum over chains of 192 N + 16 nbnd + 8 nang + 40 nnl_att + 8 nnl_rep, against
 8 TB/s HBM.
 
@@ -133,10 +137,15 @@ def timed_run(ctx, steps, warmup, profile, autotune=False):
     return t1 - t0, prof, recs
 
 
-def gpu_batched(chains, w, steps, warmup, lanes, list_lanes, topo_cache):
+def gpu_batched(chains, w, steps, warmup, lanes, list_lanes, topo_cache, graphs=True):
     ctx, topos = build_ctx(chains, w, lanes, list_lanes, topo_cache)
-    seconds, prof, recs = timed_run(ctx, steps, warmup, True, autotune=not lanes)
+    ctx.set_graphs(graphs)
+    # value: the production path (graph replay on, no per-kernel timing)
+    seconds, _, recs = timed_run(ctx, steps, warmup, False, autotune=not lanes)
     step_lanes = ctx.lanes()[0]
+    graph_stats = ctx.graph_stats()
+    # roofline: the same batch, each kernel bracketed by HIP events (direct launches)
+    _, prof, _ = timed_run(ctx, min(steps, 400), 0, True)
     bytes_step = 0
     counts = []
     for c, t in enumerate(topos):
@@ -145,7 +154,7 @@ def gpu_batched(chains, w, steps, warmup, lanes, list_lanes, topo_cache):
         bytes_step += (192 * t["n"] + 16 * len(t["ib"]) + 8 * len(t["ia"]) + 40 * k["nnl_att"] + 8 * k["nnl_rep"])
     ctx.close()
     return dict(seconds=seconds, prof=prof, bytes_step=bytes_step, counts=counts, records=len(recs),
-                lanes=step_lanes)
+                lanes=step_lanes, graphs=graph_stats)
 
 
 def gpu_plain(chains, w, steps, warmup, topo_cache, lanes=0, list_lanes=0):
@@ -329,6 +338,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true", help="launch every kernel directly (no graph replay)")
     ap.add_argument("--serve-jobs", type=int, default=100)
     ap.add_argument("--serve-step-scale", type=float, default=0.1,
                     help="job step counts are log-uniform in [1e4, 1e6] times this factor")
@@ -349,7 +359,7 @@ def main():
     sizes = [len(x) // 3 for x, _, _ in chains]
     nb = sum(sizes)
     topo_cache = {}
-    g = gpu_batched(chains, w, a.steps, a.warmup, a.lanes, a.list_lanes, topo_cache)
+    g = gpu_batched(chains, w, a.steps, a.warmup, a.lanes, a.list_lanes, topo_cache, graphs=not a.no_graphs)
     value = nb * a.steps / g["seconds"]
     print(f"[bench] batched {value:.4g} bead-steps/s", file=sys.stderr, flush=True)
     plain, plain_t = (None, None) if a.no_plain else gpu_plain(chains, w, a.plain_steps, 10, topo_cache, a.lanes,
@@ -397,7 +407,8 @@ def main():
                    "n_min": min(sizes), "n_max": max(sizes), "form": "batched (all chains per launch)",
                    "nnl_att": tot("nnl_att"), "nnl_rep": tot("nnl_rep"), "nil_att": tot("nil_att"),
                    "nil_rep": tot("nil_rep"), "records": g["records"],
-                   "step_lanes": g["lanes"], "lanes_from": "flag" if a.lanes else "autotuner"},
+                   "step_lanes": g["lanes"], "lanes_from": "flag" if a.lanes else "autotuner",
+                   "graph_segments": {"captured": g["graphs"][0], "replayed": g["graphs"][1]}},
         "plain": None if plain is None else {"value": plain, "unit": "bead-steps/s",
                                              "form": "plain (one chain per launch)", "steps_per_chain": a.plain_steps},
         "speedup_vs_plain": None if plain is None else value / plain,

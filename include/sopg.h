@@ -192,6 +192,14 @@ int sopg_get_step(const sopg_ctx* ctx, int chain, double* istep, int* inlup, int
  * the run exactly as if it had not stopped.  buf = NULL: *size = bytes needed. */
 int sopg_checkpoint(sopg_ctx* ctx, int chain, void* buf, long cap, long* size);
 int sopg_restore(sopg_ctx* ctx, int chain, const void* buf, long size);
+/* sopg_run replays repeating step segments (one list-rebuild period, no
+ * record, not the run's last step) from captured HIP graphs instead of
+ * launching their kernels one by one.  Results do not depend on it (default:
+ * enabled; segments with replayed noise or kernel timing always run direct).
+ * No counterpart in the reference. */
+int sopg_set_graphs(sopg_ctx* ctx, int enable);
+/* Segments captured and graph replays so far. */
+int sopg_graph_stats(const sopg_ctx* ctx, long* captures, long* launches);
 /* Block until all work queued on the context's stream has finished. */
 int sopg_sync(sopg_ctx* ctx);
 

@@ -34,6 +34,7 @@ void launch_posupdate(const Dev& D, int src, int dst, int g0, int g1, hipStream_
 void launch_velupdate(const Dev& D, int g0, int g1, hipStream_t s);
 void launch_observe(const Dev& D, int cur, int par, int rel, int all, int what, int g0, int g1, hipStream_t s);
 void launch_chain_reduce(const Dev& D, int c0, int nc, double* out, hipStream_t s);
+void launch_set_base(int* base, int value, hipStream_t s);
 enum { F_STORE = 1, F_VEL = 2, F_ENERGY = 4, F_ENERGY_ALL = 8, F_NOISE = 16, F_BOND = 32, F_ANGLE = 64,
        F_VDW = 128, F_ACCUM = 256, F_ALLTERMS = 16 | 32 | 64 | 128 };
 enum { OBS_PHYS = 1, OBS_COUNTS = 2 };
@@ -139,6 +140,17 @@ struct sopg_ctx {
   size_t noise_cap = 0;         // doubles
   double* d_rec = nullptr;      // per-record-step reductions
   size_t rec_cap_d = 0;
+  // graph replay of sopg_run's step segments
+  int* d_relb = nullptr;        // [0] = 0 (direct launches), [1] = step base of the replayed segment
+  bool graphs = true;
+  struct GraphEntry {
+    std::vector<char> key;
+    hipGraphExec_t exec;
+    long used;
+  };
+  std::vector<GraphEntry> gcache;
+  long gclock = 0;
+  long graph_launches = 0, graph_captures = 0;
   // kernel timing (sopg_profile)
   bool prof = false;
   std::vector<hipEvent_t> ev_pool;
@@ -587,8 +599,10 @@ int ensure_device(sopg_ctx* c) {
   const size_t init[NPOOL] = {beads, 2 * nbnd, 2 * nang, natt, 2 * natt, nrep, 2 * nrep, nch};
   for (int pi = 0; pi < NPOOL; pi++)
     if ((rc = grow_pool(c, pi, std::max<size_t>(init[pi], 1)))) return rc;
-  if ((rc = dalloc(c, &D.err, 1)) || (rc = dalloc(c, &c->d_T, 1))) return rc;
+  if ((rc = dalloc(c, &D.err, 1)) || (rc = dalloc(c, &c->d_T, 1)) || (rc = dalloc(c, &c->d_relb, 2))) return rc;
   HIP_OK(c, hipMemsetAsync(D.err, 0, sizeof(int), c->stream));
+  HIP_OK(c, hipMemsetAsync(c->d_relb, 0, 2 * sizeof(int), c->stream));
+  D.relb = c->d_relb;
   HIP_OK(c, hipMemcpyAsync(c->d_T, &c->T, sizeof(Tables), hipMemcpyHostToDevice, c->stream));
   c->device_ready = true;
   for (size_t id = 0; id < c->chains.size(); id++)
@@ -803,6 +817,7 @@ int sopg_destroy(sopg_ctx* c) {
       if (*a.p) (void)hipFree(*a.p);
   for (void* p : c->allocs) (void)hipFree(p);
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+  for (auto& ge : c->gcache) (void)hipGraphExecDestroy(ge.exec);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return SOPG_OK;
@@ -1377,6 +1392,19 @@ int sopg_list_counts(sopg_ctx* c, int chain, int* nnl_att, int* nnl_rep, int* ni
   return SOPG_OK;
 }
 
+int sopg_set_graphs(sopg_ctx* c, int enable) {
+  if (!c) return SOPG_ERR_ARG;
+  c->graphs = enable != 0;
+  return SOPG_OK;
+}
+
+int sopg_graph_stats(const sopg_ctx* c, long* captures, long* launches) {
+  if (!c) return SOPG_ERR_ARG;
+  if (captures) *captures = c->graph_captures;
+  if (launches) *launches = c->graph_launches;
+  return SOPG_OK;
+}
+
 int sopg_profile(sopg_ctx* c, int enable) {
   if (!c) return SOPG_ERR_ARG;
   c->prof = enable != 0;
@@ -1501,30 +1529,101 @@ int sopg_run(sopg_ctx* c, long nsteps, const double* noise, sopg_record* rec, lo
   }
   launch_posupdate(D, c->cur, c->cur, 0, nb, s);
   std::vector<long> rec_steps;
-  for (long r = 0; r < nsteps; r++) {
+  // The launches of run step r.  Kernels see step k + *Dx.relb: direct
+  // launches pass k = r with base 0; a captured segment passes k relative to
+  // its first step, whose run step the replay writes to the base.
+  auto one_step = [&](const Dev& Dx, long r, int k) {
     const bool last = (r == nsteps - 1);
     const double* nz = noise ? c->d_noise + (size_t)r * 3 * nb : nullptr;
-    pe = prof_begin(c);
-    launch_force(L, D, c->d_T, c->cur, c->par, (int)r, F_STORE | F_VEL | F_ENERGY | F_ALLTERMS, !last, nz, 0, nb, s);
+    int pe = prof_begin(c);
+    launch_force(L, Dx, c->d_T, c->cur, c->par, k, F_STORE | F_VEL | F_ENERGY | F_ALLTERMS, !last, nz, 0, nb, s);
     prof_end(c, SOPG_KERNEL_STEP, pe);
     if (records(r)) {
       pe = prof_begin(c);
-      launch_observe(D, c->cur, c->par, (int)r, 0, OBS_PHYS, 0, nb, s);
-      launch_chain_reduce(D, 0, nc, c->d_rec + rec_steps.size() * nc * kRed, s);
+      launch_observe(Dx, c->cur, c->par, k, 0, OBS_PHYS, 0, nb, s);
+      launch_chain_reduce(Dx, 0, nc, c->d_rec + rec_steps.size() * nc * kRed, s);
       prof_end(c, SOPG_KERNEL_OBSERVE, pe);
       rec_steps.push_back(r);
     }
     if (!last) {
       c->par ^= 1;
-      dr = due(r + 1);
-      if (dr.first) {
+      const auto d = due(r + 1);
+      if (d.first) {
         pe = prof_begin(c);
-        if (dr.second) launch_cellcoords(D, c->cur, 0, nb, s);
-        launch_rebuild(c->list_lanes, D, c->d_T, c->cur, c->par, (int)(r + 1), -1, 0, nb, 0, nc, c->binned, s);
+        if (d.second) launch_cellcoords(Dx, c->cur, 0, nb, s);
+        launch_rebuild(c->list_lanes, Dx, c->d_T, c->cur, c->par, k + 1, -1, 0, nb, 0, nc, c->binned, s);
         prof_end(c, SOPG_KERNEL_REBUILD, pe);
       }
       c->cur ^= 1;
     }
+  };
+  // Segments of S steps -- the rebuild period (the lcm of the chains' nnlup
+  // when small), so that the launch pattern repeats -- are captured once into
+  // a graph and replayed.  A segment holding a record step or the run's last
+  // step, replayed noise or kernel timing runs as direct launches.
+  long S = 1;
+  for (auto& h : c->chains)
+    if (h.active && h.p.cutoff != SOPG_CUTOFF_NONE) {
+      long a = S, b = h.p.nnlup;
+      while (b) { const long t = a % b; a = b; b = t; }
+      S = S / a * h.p.nnlup;
+      if (S > 64) break;
+    }
+  if (S > 64 || S < 8) S = S > 64 ? 32 : S * ((8 + S - 1) / S);
+  const bool use_graphs = c->graphs && !noise && !c->prof;
+  Dev Dg;
+  std::memcpy(&Dg, &D, sizeof Dg);  // bytewise: the struct is part of the graph key
+  Dg.relb = c->d_relb + 1;
+  long r = 0;
+  while (r < nsteps) {
+    const long seg = std::min(S, nsteps - r);
+    bool graphable = use_graphs && seg == S && r + S < nsteps;
+    for (long k = 0; graphable && k < S; k++) graphable = !records(r + k);
+    if (!graphable) {
+      for (long k = 0; k < seg; k++) one_step(D, r + k, (int)(r + k));
+      r += seg;
+      continue;
+    }
+    // key: launch shape, parities, the segment's rebuild pattern, device layout
+    std::vector<char> key;
+    auto add = [&](const void* p, size_t n) { key.insert(key.end(), (const char*)p, (const char*)p + n); };
+    const int hdr[6] = {(int)S, c->cur, c->par, L, c->list_lanes, c->binned};
+    add(hdr, sizeof hdr);
+    for (long k = 0; k < S; k++) {
+      const auto d = due(r + k + 1);
+      const char f = (char)((d.first ? 1 : 0) | (d.second ? 2 : 0));
+      add(&f, 1);
+    }
+    add(&Dg, sizeof Dg);
+    add(&c->d_T, sizeof c->d_T);
+    hipGraphExec_t exec = nullptr;
+    for (auto& ge : c->gcache)
+      if (ge.key == key) { exec = ge.exec; ge.used = ++c->gclock; break; }
+    if (exec) {
+      // replaying advances the parities as the captured steps did
+      if (S & 1) { c->cur ^= 1; c->par ^= 1; }
+    } else {
+      hipGraph_t graph = nullptr;
+      HIP_OK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      for (long k = 0; k < S; k++) one_step(Dg, r + k, (int)k);
+      HIP_OK(c, hipStreamEndCapture(s, &graph));
+      hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (e != hipSuccess) return fail(c, SOPG_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+      if (c->gcache.size() >= 16) {
+        auto lru = std::min_element(c->gcache.begin(), c->gcache.end(),
+                                    [](const sopg_ctx::GraphEntry& a, const sopg_ctx::GraphEntry& b) { return a.used < b.used; });
+        HIP_OK(c, hipStreamSynchronize(s));
+        (void)hipGraphExecDestroy(lru->exec);
+        c->gcache.erase(lru);
+      }
+      c->gcache.push_back({key, exec, ++c->gclock});
+      c->graph_captures++;
+    }
+    launch_set_base(c->d_relb + 1, (int)r, s);
+    HIP_OK(c, hipGraphLaunch(exec, s));
+    c->graph_launches++;
+    r += S;
   }
   HIP_OK(c, hipGetLastError());
   for (auto& h : c->chains) h.pl_fresh = false;

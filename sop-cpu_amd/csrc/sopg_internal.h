@@ -60,6 +60,10 @@ struct GridP {
 enum { R_EBND = 0, R_EANG, R_EATT, R_EREP, R_CONT, R_RG, R_V2, R_NLA, R_PLA, R_NLR, R_PLR, kRed = 12 };
 
 struct Dev {
+  // Added to every kernel's relative step argument: 0 for direct launches;
+  // for a replayed graph segment the run-relative step at which it starts
+  // (the segment's kernels are captured with steps relative to it).
+  const int* relb;
   int nb;    // bead high-water mark: beads [0, nb) are live or free (chain -1)
   int cap;   // non-native slots per bead
   double *ux[2], *uy[2], *uz[2];

@@ -264,6 +264,7 @@ __device__ __forceinline__ void rebuild_scan_bead(const Dev& D, const Tables* __
 template <int L>
 __global__ void __launch_bounds__(256) k_rebuild(Dev D, const Tables* __restrict__ T, int cur, int par, int rel,
                                                  int mode, int g0, int g1) {
+  rel += *D.relb;
   const int l = threadIdx.x & (L - 1);
   const int stride = (int)(gridDim.x * blockDim.x / L);
   for (int g = g0 + (xcd_block() * (int)blockDim.x + (int)threadIdx.x) / L; g < g1; g += stride)
@@ -289,6 +290,7 @@ __device__ __forceinline__ bool rebuild_due(const ChainP& P, int rel, int mode, 
 // Per-chain extent and grid; one block per chain.  Chains of more than maxn
 // beads (whose LDS bitmap would not fit) keep the full scan.
 __global__ void __launch_bounds__(256) k_grid(Dev D, int cur, int rel, int mode, int c0, int maxn) {
+  rel += *D.relb;
   const int c = c0 + blockIdx.x;
   const ChainP& P = D.cp[c];
   GridP& G = D.grid[c];
@@ -345,6 +347,7 @@ __device__ __forceinline__ int grid_coord(double v, double o, double inv, int n)
 
 // Counting sort of beads into cells: count, scan, fill.
 __global__ void k_bin_count(Dev D, int cur, int rel, int mode, int g0, int g1) {
+  rel += *D.relb;
   const int g = g0 + blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= g1) return;
   const int c = D.chain[g];
@@ -364,6 +367,7 @@ __global__ void k_bin_count(Dev D, int cur, int rel, int mode, int g0, int g1) {
 }
 
 __global__ void __launch_bounds__(256) k_bin_scan(Dev D, int rel, int mode, int c0) {
+  rel += *D.relb;
   const int c = c0 + blockIdx.x;
   const ChainP& P = D.cp[c];
   const GridP& G = D.grid[c];
@@ -398,6 +402,7 @@ __global__ void __launch_bounds__(256) k_bin_scan(Dev D, int rel, int mode, int 
 }
 
 __global__ void k_bin_fill(Dev D, int rel, int mode, int g0, int g1) {
+  rel += *D.relb;
   const int g = g0 + blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= g1) return;
   const int c = D.chain[g];
@@ -418,6 +423,7 @@ __global__ void k_bin_fill(Dev D, int rel, int mode, int g0, int g1) {
 template <int LG>
 __global__ void __launch_bounds__(256) k_rebuild_grid(Dev D, const Tables* __restrict__ T, int cur, int par, int rel,
                                                       int mode, int g0, int g1) {
+  rel += *D.relb;
   constexpr int BPB = 256 / LG;
   extern __shared__ uint32_t bitmaps[];
   __shared__ double cut2[9][2];  // per type pair (it * 3 + jt): (3.2 sigma)^2, (2.5 sigma)^2
@@ -653,6 +659,7 @@ enum { TP_FREP = 0, TP_S6, TP_S12, TP_PLRC2, TP_CREP, TP_FATT, TP_CATT, TP_N };
 template <int L, bool NEXT>
 __global__ void __launch_bounds__(256) k_force(Dev D, const Tables* __restrict__ T, int cur, int pr, int rel, int flags,
                                                const double* __restrict__ noise, int g0, int g1) {
+  rel += *D.relb;
   __shared__ double tp[9][TP_N];
   if (threadIdx.x < 9) {
     const int it = threadIdx.x / 3, jt = threadIdx.x % 3;
@@ -914,6 +921,7 @@ __global__ void __launch_bounds__(256) k_force(Dev D, const Tables* __restrict__
 // Observables (sop.cpp:200-280) and list counts, per bead.
 enum { OBS_PHYS = 1, OBS_COUNTS = 2 };
 __global__ void k_observe(Dev D, int cur, int par, int rel, int all, int what, int g0, int g1) {
+  rel += *D.relb;
   const int g = g0 + blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= g1 || D.chain[g] < 0) return;  // past the end, or a free bead
   const ChainP& P = D.cp[D.chain[g]];
@@ -987,6 +995,9 @@ __global__ void __launch_bounds__(256) k_chain_reduce(Dev D, int c0, double* out
     __syncthreads();
   }
 }
+
+// Step base of a replayed graph segment (see Dev::relb).
+__global__ void k_set_base(int* base, int value) { *base = value; }
 
 // ------------------------------------------------------------------ launchers
 
@@ -1070,6 +1081,9 @@ void launch_velupdate(const Dev& D, int g0, int g1, hipStream_t s) {
 }
 void launch_observe(const Dev& D, int cur, int par, int rel, int all, int what, int g0, int g1, hipStream_t s) {
   hipLaunchKernelGGL(k_observe, dim3(blocks_for(g1 - g0)), dim3(256), 0, s, D, cur, par, rel, all, what, g0, g1);
+}
+void launch_set_base(int* base, int value, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_base, dim3(1), dim3(1), 0, s, base, value);
 }
 void launch_chain_reduce(const Dev& D, int c0, int nc, double* out, hipStream_t s) {
   hipLaunchKernelGGL(k_chain_reduce, dim3(nc), dim3(256), 0, s, D, c0, out);
