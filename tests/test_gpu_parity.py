@@ -476,3 +476,41 @@ def test_graph_replay_equals_direct(inputs_root, cutoff):
             for k in ("unc", "vel", "force", "pos"):
                 assert np.array_equal(sa[k], sg[k]), k
     assert ra == rg
+
+
+def test_long_run_statistics_match_reference(inputs_root):
+    """The device generator (Philox) is not the reference's stream, so its
+    trajectories differ from the reference's; long-run averages of Q and Rg^2
+    must agree within their statistical error bars.  16 independent replicas
+    on each side: the device with 16 Philox keys, the oracle (bit-identical to
+    the reference) with its own generator at 16 `run` seeds."""
+    n, kT, nnlup, nup, equil, steps, reps = 76, 0.9, 50, 100, 2000, 10000, 16
+    x, v, d = chain_inputs(inputs_root, n, 5)
+    topo = sopg.chain_topology(x)
+    ctx = sopg.Context()
+    for r in range(reps):
+        ctx.add_chain(topo, sopg.chain_params(T=kT, cutoff=sopg.CUTOFF_NEIGHBOR, nnlup=nnlup, nup=nup,
+                                              noise_mode=sopg.NOISE_PHILOX, seed=5000 + r))
+    for r in range(reps):
+        _setup_chain(ctx, r, x, v)
+    print("[gpu] long-run statistics", flush=True)
+    recs = ctx.run(equil + steps)
+    dev = {k: np.array([np.mean([rec[k] for rec in recs if rec["chain"] == r and rec["istep"] > equil])
+                        for r in range(reps)]) for k in ("Q", "rgsq")}
+    ref = {"Q": [], "rgsq": []}
+    for r in range(reps):
+        orc = Oracle()
+        setup_system(orc, d, std_cmds(temp=kT, nnlup=nnlup, nup=nup, extra=[("run", str(r + 1))]))
+        orc.set_coords(2, v)
+        orc.call("update_neighbor_list"); orc.call("update_pair_list")
+        orc.call("energy_eval"); orc.call("force_eval")
+        orecs = run_oracle_ctrl(orc, equil + steps, nnlup, nup)
+        for k in ("Q", "rgsq"):
+            ref[k].append(np.mean([rec[k] for rec in orecs if rec["istep"] > equil]))
+    for k in ("Q", "rgsq"):
+        a, b = dev[k], np.array(ref[k])
+        se = np.sqrt(a.var(ddof=1) / reps + b.var(ddof=1) / reps)
+        z = abs(a.mean() - b.mean()) / max(se, 1e-12)
+        print(f"{k}: device {a.mean():.5f} +- {a.std(ddof=1) / np.sqrt(reps):.5f}, "
+              f"reference {b.mean():.5f} +- {b.std(ddof=1) / np.sqrt(reps):.5f}, z = {z:.2f}")
+        assert z < 4.0, (k, a.mean(), b.mean(), se)
