@@ -294,8 +294,7 @@ void register_arrays(sopg_ctx* c) {
   for (double** p : {&D.am_r2, &D.am_nlrc2}) add(P_ATT, p, 8, 1);
   add(P_ATT, &D.am_rf, 4, 1);
   add(P_TINC, &D.att_code, 4, 1); add(P_TINC, &D.att_partner, 4, 1); add(P_TINC, &D.att_flag, 1, 1);
-  add(P_TINC, &D.att_t, 1, 1);
-  for (double** p : {&D.att_r6, &D.att_r12, &D.att_plrc2}) add(P_TINC, p, 8, 1);
+  add(P_TINC, &D.att_k, sizeof(AttK), 1);
   add(P_REP, &D.rm_i, 4, 1); add(P_REP, &D.rm_j, 4, 1); add(P_REP, &D.rm_t, 1, 1);
   add(P_RINC, &D.rep_code, 4, 1);
   add(P_CHAIN, &D.cp, sizeof(ChainP), 1);
@@ -503,24 +502,23 @@ int materialize(sopg_ctx* c, int id) {
   // per incidence: partner, and the pair's constants in the bead's own order
   const size_t nq = code.size();
   std::vector<int> partner(nq), q0(nt), q1(nt);
-  std::vector<int8_t> qt(nq);
-  std::vector<double> qr6(nq), qr12(nq), qpl(nq);
+  std::vector<AttK> qk(nq);
   for (size_t q = 0; q < nq; q++) {
     const size_t k = (size_t)(code[q] >> 1) - h.at[P_ATT];
     partner[q] = off + ((code[q] & 1) ? h.ai[k] : h.aj[k]);
     ((code[q] & 1) ? q1 : q0)[k] = (int)(h.at[P_TINC] + q);
-    qt[q] = (int8_t)(at[k] | ((code[q] & 1) << 4));
-    qr6[q] = r6[k];
-    qr12[q] = r12[k];
-    qpl[q] = plrc2[k];
+    qk[q].r6 = r6[k];
+    qk[q].r12 = r12[k];
+    qk[q].plrc2 = plrc2[k];
+    qk[q].t = at[k] | ((code[q] & 1) << 4);
+    qk[q].pad = 0;
   }
   h.q0.assign(q0.begin(), q0.end());
   h.q1.assign(q1.begin(), q1.end());
   const size_t ta = h.at[P_ATT], tq = h.at[P_TINC];
   if ((rc = upload_at(c, D.att_beg, off, beg)) || (rc = upload_at(c, D.att_end, off, end)) ||
       (rc = upload_at(c, D.att_code, tq, code)) || (rc = upload_at(c, D.att_partner, tq, partner)) ||
-      (rc = upload_at(c, D.att_t, tq, qt)) || (rc = upload_at(c, D.att_r6, tq, qr6)) ||
-      (rc = upload_at(c, D.att_r12, tq, qr12)) || (rc = upload_at(c, D.att_plrc2, tq, qpl)) ||
+      (rc = upload_at(c, D.att_k, tq, qk)) ||
       (rc = upload_at(c, D.am_i, ta, glob(h.ai))) || (rc = upload_at(c, D.am_j, ta, glob(h.aj))) ||
       (rc = upload_at(c, D.am_q0, ta, q0)) || (rc = upload_at(c, D.am_t, ta, at)) ||
       (rc = upload_at(c, D.am_r2, ta, r2)) || (rc = upload_at(c, D.am_nlrc2, ta, nlrc2)) ||

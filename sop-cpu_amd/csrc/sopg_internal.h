@@ -56,6 +56,14 @@ struct GridP {
   int use;             // 1: binned rebuild, 0: full candidate scan
 };
 
+// Constants of one native-contact incidence that the step kernel reads, in
+// one 32-byte record (two 16-byte loads): sigma^6, sigma^12, (2.5 sigma)^2 and
+// t = itype | jtype << 2 | side << 4.
+struct alignas(32) AttK {
+  double r6, r12, plrc2;
+  int32_t t, pad;
+};
+
 // Per-bead reduction fields (red[g * kRed + field]).
 enum { R_EBND = 0, R_EANG, R_EATT, R_EREP, R_CONT, R_RG, R_V2, R_NLA, R_PLA, R_NLR, R_PLR, kRed = 12 };
 
@@ -76,11 +84,10 @@ struct Dev {
   // angles (i, k) pairs, per incidence: partner << 1 | side (0: bead is i, 1: bead is k)
   int *ang_beg, *ang_end, *ang_pq;
   // natives: code = master << 1 | side; partner = the other bead (global).
-  // Per incidence also the pair's constants the step kernel reads, so that a
-  // bead's lanes load them contiguously: att_t = itype | jtype << 2 | side << 4,
-  // sigma^6, sigma^12, (2.5 sigma)^2.
+  // Per incidence also the pair's constants the step kernel reads (AttK), so
+  // that a bead's lanes load them contiguously.
   int *att_beg, *att_end, *att_code, *att_partner; uint8_t* att_flag;  // flag: bit0 NL, bit1+p PL(parity p)
-  int8_t* att_t; double *att_r6, *att_r12, *att_plrc2;
+  AttK* att_k;
   // master native entries (am_t = itype | jtype << 2; am_q0 = incidence of the first bead)
   int *am_i, *am_j, *am_q0; int8_t* am_t;
   double *am_r2, *am_nlrc2; float* am_rf;

@@ -178,8 +178,8 @@ __device__ __forceinline__ void rebuild_scan_bead(const Dev& D, const Tables* __
     const double dz = mimg(uz[j] - uz[i], boxl, qbox);
     const double d2 = dx * dx + dy * dy + dz * dz;
     bool nl, pl;
-    if (crit == SOPG_CUTOFF_NEIGHBOR) { nl = d2 < D.am_nlrc2[k]; pl = nl && d2 < D.att_plrc2[q]; }
-    else if (crit == SOPG_CUTOFF_CELL) { nl = cell_adjacent(D, P, i, j); pl = nl && d2 < D.att_plrc2[q]; }
+    if (crit == SOPG_CUTOFF_NEIGHBOR) { nl = d2 < D.am_nlrc2[k]; pl = nl && d2 < D.att_k[q].plrc2; }
+    else if (crit == SOPG_CUTOFF_CELL) { nl = cell_adjacent(D, P, i, j); pl = nl && d2 < D.att_k[q].plrc2; }
     else { nl = true; pl = true; }
     D.att_flag[q] = (uint8_t)((nl ? 1 : 0) | (pl ? (2 << par) : 0));
   }
@@ -464,8 +464,8 @@ __global__ void __launch_bounds__(256) k_rebuild_grid(Dev D, const Tables* __res
       const double dz = mimg(uz[j] - uz[i], boxl, qbox);
       const double d2 = dx * dx + dy * dy + dz * dz;
       bool nl, pl;
-      if (crit == SOPG_CUTOFF_NEIGHBOR) { nl = d2 < D.am_nlrc2[k]; pl = nl && d2 < D.att_plrc2[q]; }
-      else { nl = cell_adjacent(D, P, i, j); pl = nl && d2 < D.att_plrc2[q]; }
+      if (crit == SOPG_CUTOFF_NEIGHBOR) { nl = d2 < D.am_nlrc2[k]; pl = nl && d2 < D.att_k[q].plrc2; }
+      else { nl = cell_adjacent(D, P, i, j); pl = nl && d2 < D.att_k[q].plrc2; }
       D.att_flag[q] = (uint8_t)((nl ? 1 : 0) | (pl ? (2 << par) : 0));
     }
     for (int w = l; w < words; w += LG) { nlb[w] = 0u; plb[w] = 0u; }
@@ -589,7 +589,7 @@ __global__ void __launch_bounds__(256) k_pairfilter(Dev D, const Tables* __restr
       const double dy = mimg(uy[j] - uy[i], boxl, qbox);
       const double dz = mimg(uz[j] - uz[i], boxl, qbox);
       const double d2 = dx * dx + dy * dy + dz * dz;
-      pl = (P.cutoff == SOPG_CUTOFF_NONE) ? true : d2 < D.att_plrc2[q];
+      pl = (P.cutoff == SOPG_CUTOFF_NONE) ? true : d2 < D.att_k[q].plrc2;
     }
     D.att_flag[q] = (uint8_t)((f & ~pbit) | (pl ? pbit : 0));
   }
@@ -657,7 +657,7 @@ enum { F_STORE = 1, F_VEL = 2, F_ENERGY = 4, F_ENERGY_ALL = 8, F_NOISE = 16, F_B
 enum { TP_FREP = 0, TP_S6, TP_S12, TP_PLRC2, TP_CREP, TP_FATT, TP_CATT, TP_N };
 
 template <int L, bool NEXT>
-__global__ void __launch_bounds__(256) k_force(Dev D, const Tables* __restrict__ T, int cur, int pr, int rel, int flags,
+__global__ void __launch_bounds__(256, 5) k_force(Dev D, const Tables* __restrict__ T, int cur, int pr, int rel, int flags,
                                                const double* __restrict__ noise, int g0, int g1) {
   rel += *D.relb;
   __shared__ double tp[9][TP_N];
@@ -793,12 +793,13 @@ __global__ void __launch_bounds__(256) k_force(Dev D, const Tables* __restrict__
       double fx = 0.0, fy = 0.0, fz = 0.0;
       if (q < p1) {
         if (f & 1) {
-          const int tt = D.att_t[q], side = (tt >> 4) & 1, tpi = (tt & 3) * 3 + ((tt >> 2) & 3);
+          const AttK K = D.att_k[q];
+          const int tt = K.t, side = (tt >> 4) & 1, tpi = (tt & 3) * 3 + ((tt >> 2) & 3);
           double dx, dy, dz;
           pair_d3(side, cx, cy, cz, px, py, pz, boxl, qbox, dx, dy, dz);
           const double d2 = dx * dx + dy * dy + dz * dz;
           if ((f >> (1 + pr)) & 1) {
-            const double r6 = D.att_r6[q], r12 = D.att_r12[q];
+            const double r6 = K.r6, r12 = K.r12;
             const double d6 = d2 * d2 * d2;
             const double d12 = d6 * d6;
             const bool skip = (d2 < thr_att) &&
@@ -812,7 +813,7 @@ __global__ void __launch_bounds__(256) k_force(Dev D, const Tables* __restrict__
             if (energy && side == 0) eatt += tp[tpi][TP_CATT] * (r12 / d12) - 2.0 * (r6 / d6);
           }
           if constexpr (NEXT) {
-            const bool npl = no_cut || d2 < D.att_plrc2[q];
+            const bool npl = no_cut || d2 < K.plrc2;
             const uint8_t nf = (uint8_t)((f & ~(2 << pw)) | (npl ? (2 << pw) : 0));
             if (nf != f) D.att_flag[q] = nf;
           }
