@@ -485,32 +485,59 @@ __global__ void __launch_bounds__(256) k_rebuild_grid(Dev D, const Tables* __res
     const double ox = ux[g], oy = uy[g], oz = uz[g];
     const int gt = D.type[g];
     const bool nlcrit = crit == SOPG_CUTOFF_NEIGHBOR;
-    for (int dz = -1; dz <= 1; dz++) {
-      const int z = czi + dz;
-      if (z < 0 || z >= G.nz) continue;
-      for (int dy = -1; dy <= 1; dy++) {
-        const int y = cyi + dy;
-        if (y < 0 || y >= G.ny) continue;
+    // the 27 neighbouring cells as up to 9 x-runs, contiguous in the sorted
+    // array, walked as one candidate sequence (LG candidates per chunk, the
+    // next chunk loaded before the current one is tested)
+    int rs[9], pre[10];
+    pre[0] = 0;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      const int z = czi + k / 3 - 1, y = cyi + k % 3 - 1;
+      int s0 = 0, len = 0;
+      if (z >= 0 && z < G.nz && y >= 0 && y < G.ny) {
         const int row = (z * G.ny + y) * G.nx;
-        const int s0 = D.cell_start[cbase + row + x0];
-        const int s1 = D.cell_start[cbase + row + x1] + D.cell_fill[cbase + row + x1];
-        for (int s = s0 + l; s < s1; s += LG) {
-          const int cc = cb[s];
-          if (!(cc - b > 2 || b - cc > 2)) continue;
-          const int gc = off + cc;
-          const int side = gc < g ? 1 : 0;  // 1: the partner is the pair's first bead
-          const int ct = D.type[gc];
-          const int tpi = side ? ct * 3 + gt : gt * 3 + ct;
-          double ddx, ddy, ddz;
-          pair_d3(side, ox, oy, oz, ux[gc], uy[gc], uz[gc], boxl, qbox, ddx, ddy, ddz);
-          const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-          const bool keep = nlcrit ? d2 < cut2[tpi][0] : cell_adjacent(D, P, side ? gc : g, side ? g : gc);
-          if (keep) {
-            const uint32_t bit = 1u << (cc & 31);
-            atomicOr(&nlb[cc >> 5], bit);
-            if (d2 < cut2[tpi][1]) atomicOr(&plb[cc >> 5], bit);
-          }
-        }
+        s0 = D.cell_start[cbase + row + x0];
+        len = D.cell_start[cbase + row + x1] + D.cell_fill[cbase + row + x1] - s0;
+      }
+      rs[k] = s0;
+      pre[k + 1] = pre[k] + len;
+    }
+    const int total = pre[9];
+    // position in the sorted array of candidate t: the last run starting at or before t
+    auto locate = [&](int t) {
+      int at = rs[0] + t;
+#pragma unroll
+      for (int k = 1; k < 9; k++) at = (pre[k] <= t) ? rs[k] + (t - pre[k]) : at;
+      return at;
+    };
+    int ccn = -1;
+    double pxn = 0.0, pyn = 0.0, pzn = 0.0;
+    if (l < total) {
+      ccn = cb[locate(l)];
+      const int gc = off + ccn;
+      pxn = ux[gc]; pyn = uy[gc]; pzn = uz[gc];
+    }
+    for (int base = 0; base < total; base += LG) {
+      const int cc = ccn;
+      const double px = pxn, py = pyn, pz = pzn;
+      if (base + LG + l < total) {
+        ccn = cb[locate(base + LG + l)];
+        const int gc = off + ccn;
+        pxn = ux[gc]; pyn = uy[gc]; pzn = uz[gc];
+      }
+      if (base + l >= total || !(cc - b > 2 || b - cc > 2)) continue;
+      const int gc = off + cc;
+      const int side = gc < g ? 1 : 0;  // 1: the partner is the pair's first bead
+      const int ct = D.type[gc];
+      const int tpi = side ? ct * 3 + gt : gt * 3 + ct;
+      double ddx, ddy, ddz;
+      pair_d3(side, ox, oy, oz, px, py, pz, boxl, qbox, ddx, ddy, ddz);
+      const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+      const bool keep = nlcrit ? d2 < cut2[tpi][0] : cell_adjacent(D, P, side ? gc : g, side ? g : gc);
+      if (keep) {
+        const uint32_t bit = 1u << (cc & 31);
+        atomicOr(&nlb[cc >> 5], bit);
+        if (d2 < cut2[tpi][1]) atomicOr(&plb[cc >> 5], bit);
       }
     }
   }
