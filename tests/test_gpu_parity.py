@@ -514,3 +514,32 @@ def test_long_run_statistics_match_reference(inputs_root):
         print(f"{k}: device {a.mean():.5f} +- {a.std(ddof=1) / np.sqrt(reps):.5f}, "
               f"reference {b.mean():.5f} +- {b.std(ddof=1) / np.sqrt(reps):.5f}, z = {z:.2f}")
         assert z < 4.0, (k, a.mean(), b.mean(), se)
+
+
+def test_list_capacity_overflow_is_an_error(inputs_root):
+    """A per-bead non-native list longer than the context's capacity is a
+    clean error (the kernels never write past the capacity), not a fault."""
+    x, v, _ = chain_inputs(inputs_root, 300, 111)
+    ctx = sopg.Context(capacity=8)
+    ctx.add_chain(sopg.chain_topology(x), sopg.chain_params(cutoff=sopg.CUTOFF_NEIGHBOR, nnlup=20, nup=100))
+    ctx.put_state(0, pos=x, unc=x, vel=v, force=np.zeros_like(x))
+    with pytest.raises(sopg.SopgError, match="capacity"):
+        ctx.update_neighbor_list(0)
+    ctx.close()
+
+
+def test_empty_runs(inputs_root):
+    """Zero steps and a context without chains are no-ops."""
+    ctx = sopg.Context()
+    assert ctx.run(10) == []
+    x, v, _ = chain_inputs(inputs_root, 50, 112)
+    ctx.add_chain(sopg.chain_topology(x), sopg.chain_params(cutoff=sopg.CUTOFF_NEIGHBOR, nnlup=20, nup=100))
+    _setup_chain(ctx, 0, x, v)
+    before = ctx.get_state(0)
+    assert ctx.run(0) == []
+    after = ctx.get_state(0)
+    for k in ("unc", "vel", "force"):
+        assert np.array_equal(before[k], after[k])
+    ctx.remove_chain(0)
+    assert ctx.nchains() == 0 and ctx.run(5) == []
+    ctx.close()
