@@ -104,8 +104,8 @@ const char* sopg_last_error(const sopg_ctx* ctx);
 int sopg_default_tables(sopg_tables* t);
 int sopg_set_tables(sopg_ctx* ctx, const sopg_tables* t);
 /* Lanes (1..64, power of two; 0 = default) that cooperate on one bead in the
- * step (force) kernels and in the list-rebuild kernels.  Results do not depend
- * on either. */
+ * step (force) kernels and in the list-rebuild kernels.  Results (states,
+ * lists, records) do not depend on either. */
 int sopg_set_lanes(sopg_ctx* ctx, int step_lanes, int list_lanes);
 int sopg_get_lanes(const sopg_ctx* ctx, int* step_lanes, int* list_lanes);
 /* Time the step kernel at every lane count on the context's current chains
@@ -192,6 +192,13 @@ int sopg_get_step(const sopg_ctx* ctx, int chain, double* istep, int* inlup, int
  * the run exactly as if it had not stopped.  buf = NULL: *size = bytes needed. */
 int sopg_checkpoint(sopg_ctx* ctx, int chain, void* buf, long cap, long* size);
 int sopg_restore(sopg_ctx* ctx, int chain, const void* buf, long size);
+/* sopg_run advances every chain of at most max_beads beads (0: none, the
+ * default) in one persistent kernel: a block per chain runs all of the run's
+ * steps -- force, integration, records, list rebuilds -- with block barriers
+ * only, concurrently with the per-step kernels of the other chains.  lanes:
+ * lanes per bead in it (0: automatic); max_beads * lanes <= 512.  Results do
+ * not depend on it.  No counterpart in the reference. */
+int sopg_set_resident(sopg_ctx* ctx, int max_beads, int lanes);
 /* sopg_run replays repeating step segments (one list-rebuild period, no
  * record, not the run's last step) from captured HIP graphs instead of
  * launching their kernels one by one.  Results do not depend on it (default:

@@ -35,6 +35,8 @@ void launch_velupdate(const Dev& D, int g0, int g1, hipStream_t s);
 void launch_observe(const Dev& D, int cur, int par, int rel, int all, int what, int g0, int g1, hipStream_t s);
 void launch_chain_reduce(const Dev& D, int c0, int nc, double* out, hipStream_t s);
 void launch_set_base(int* base, int value, hipStream_t s);
+void launch_resident(int L, const Dev& D, const Tables* T, const int* chains, int nchains, int threads, int cur,
+                     int par, int nsteps, const int* rec_slot, double* rec, int nc, hipStream_t s);
 enum { F_STORE = 1, F_VEL = 2, F_ENERGY = 4, F_ENERGY_ALL = 8, F_NOISE = 16, F_BOND = 32, F_ANGLE = 64,
        F_VDW = 128, F_ACCUM = 256, F_ALLTERMS = 16 | 32 | 64 | 128 };
 enum { OBS_PHYS = 1, OBS_COUNTS = 2 };
@@ -110,6 +112,7 @@ struct HostChain {
   // explicit list update since the last position change): sopg_run may skip
   // its first pair-list filter for this chain
   bool pl_fresh = false;
+  bool resident = false;  // advanced by the resident kernel in the current sopg_run
   size_t at[NPOOL] = {0};  // range starts in each pool
 };
 
@@ -151,6 +154,15 @@ struct sopg_ctx {
   std::vector<GraphEntry> gcache;
   long gclock = 0;
   long graph_launches = 0, graph_captures = 0;
+  // resident (persistent) kernel for small chains
+  int resident_max = 0;         // chains of at most this many beads (0: off)
+  int resident_lanes = 0;       // lanes per bead in it (0: automatic)
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_res = nullptr;
+  int* d_rlist = nullptr;
+  size_t rlist_cap = 0;
+  int* d_rslot = nullptr;
+  size_t rslot_cap = 0;
   // kernel timing (sopg_profile)
   bool prof = false;
   std::vector<hipEvent_t> ev_pool;
@@ -391,6 +403,7 @@ void fill_chainp(const HostChain& h, const Tables& T, ChainP& P) {
     for (int b = 0; b < 3; b++)
       if (present[a] && present[b]) rc2 = std::max(rc2, T.nlrc2[a][b]);
   P.gcell = std::sqrt(rc2) * 1.001 + 1e-9;
+  P.resident = h.resident ? 1 : 0;
 }
 
 // Device copy of the per-chain parameters: chain `only`, or all (only < 0).
@@ -815,6 +828,8 @@ int sopg_destroy(sopg_ctx* c) {
       if (*a.p) (void)hipFree(*a.p);
   for (void* p : c->allocs) (void)hipFree(p);
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+  if (c->ev_res) (void)hipEventDestroy(c->ev_res);
+  if (c->stream2) (void)hipStreamDestroy(c->stream2);
   for (auto& ge : c->gcache) (void)hipGraphExecDestroy(ge.exec);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -1390,6 +1405,16 @@ int sopg_list_counts(sopg_ctx* c, int chain, int* nnl_att, int* nnl_rep, int* ni
   return SOPG_OK;
 }
 
+int sopg_set_resident(sopg_ctx* c, int max_beads, int lanes) {
+  if (!c || max_beads < 0 || lanes < 0 || lanes > 8 || (lanes & (lanes - 1)))
+    return c ? fail(c, SOPG_ERR_ARG, "resident: max_beads >= 0, lanes 0 or a power of two up to 8") : SOPG_ERR_ARG;
+  if (max_beads * (lanes ? lanes : 1) > 512)
+    return fail(c, SOPG_ERR_ARG, "resident chains must fit one 512-thread block (max_beads * lanes <= 512)");
+  c->resident_max = max_beads;
+  c->resident_lanes = lanes;
+  return SOPG_OK;
+}
+
 int sopg_set_graphs(sopg_ctx* c, int enable) {
   if (!c) return SOPG_ERR_ARG;
   c->graphs = enable != 0;
@@ -1463,6 +1488,18 @@ int sopg_run(sopg_ctx* c, long nsteps, const double* noise, sopg_record* rec, lo
   const bool replay = any_replay(c);
   if (replay && !noise) return fail(c, SOPG_ERR_ARG, "replayed-noise chains need the run's noise");
   if (!replay && noise) return fail(c, SOPG_ERR_ARG, "all chains use the device generator: noise must be NULL");
+  // small chains run in the resident kernel (not with replayed noise or kernel timing)
+  std::vector<int> rlist;
+  int rmaxn = 0;
+  if (c->resident_max > 0 && !noise && !c->prof)
+    for (int id = 0; id < (int)c->chains.size(); id++) {
+      HostChain& h = c->chains[id];
+      if (h.active && h.n <= c->resident_max) {
+        h.resident = true;
+        rlist.push_back(id);
+        rmaxn = std::max(rmaxn, h.n);
+      }
+    }
   if ((rc = upload_chainp(c))) return rc;
   const int nb = c->D.nb, nc = (int)c->pool[P_CHAIN].cap;
   hipStream_t s = c->stream;
@@ -1480,10 +1517,13 @@ int sopg_run(sopg_ctx* c, long nsteps, const double* noise, sopg_record* rec, lo
     HIP_OK(c, hipMemcpyAsync(c->d_noise, lay.data(), sizeof(double) * lay.size(), hipMemcpyHostToDevice, s));
     HIP_OK(c, hipStreamSynchronize(s));
   }
+  // the per-step launches below serve the chains the resident kernel does not
+  bool stepped = false;
+  for (auto& h : c->chains) stepped |= h.active && !h.resident;
   auto due = [&](long r) {
     bool any = false, cell = false;
     for (auto& h : c->chains)
-      if (h.active && h.p.cutoff != SOPG_CUTOFF_NONE && (h.inlup + r) % h.p.nnlup == 0) {
+      if (h.active && !h.resident && h.p.cutoff != SOPG_CUTOFF_NONE && (h.inlup + r) % h.p.nnlup == 0) {
         any = true;
         cell |= h.p.cutoff == SOPG_CUTOFF_CELL;
       }
@@ -1491,11 +1531,20 @@ int sopg_run(sopg_ctx* c, long nsteps, const double* noise, sopg_record* rec, lo
   };
   auto records = [&](long r) {
     for (auto& h : c->chains)
-      if (h.active && (h.iup + r) % h.p.nup == 0) return true;
+      if (h.active && !h.resident && (h.iup + r) % h.p.nup == 0) return true;
     return false;
   };
-  long nrs = 0;
-  for (long r = 0; r < nsteps; r++) nrs += records(r);
+  // record buffer rows: one per run step at which any chain records
+  std::vector<long> rec_steps;
+  std::vector<int> row_of(nsteps, -1);
+  for (long r = 0; r < nsteps; r++)
+    for (auto& h : c->chains)
+      if (h.active && (h.iup + r) % h.p.nup == 0) {
+        row_of[r] = (int)rec_steps.size();
+        rec_steps.push_back(r);
+        break;
+      }
+  const long nrs = (long)rec_steps.size();
   if ((size_t)nrs * nc * kRed > c->rec_cap_d) {
     HIP_OK(c, hipStreamSynchronize(s));
     dfree(c, c->d_rec);
@@ -1506,10 +1555,45 @@ int sopg_run(sopg_ctx* c, long nsteps, const double* noise, sopg_record* rec, lo
   }
   const int L = c->lanes;
   Dev& D = c->D;
+  if (!rlist.empty()) {
+    if (!c->stream2) {
+      HIP_OK(c, hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+      HIP_OK(c, hipEventCreateWithFlags(&c->ev_res, hipEventDisableTiming));
+    }
+    if (rlist.size() > c->rlist_cap) {
+      HIP_OK(c, hipStreamSynchronize(c->stream2));
+      dfree(c, c->d_rlist);
+      c->d_rlist = nullptr;
+      if ((rc = dalloc(c, &c->d_rlist, rlist.size()))) return rc;
+      c->rlist_cap = rlist.size();
+    }
+    if ((size_t)nsteps > c->rslot_cap) {
+      HIP_OK(c, hipStreamSynchronize(c->stream2));
+      dfree(c, c->d_rslot);
+      c->d_rslot = nullptr;
+      if ((rc = dalloc(c, &c->d_rslot, (size_t)nsteps))) return rc;
+      c->rslot_cap = (size_t)nsteps;
+    }
+    if ((rc = upload_at(c, c->d_rlist, 0, rlist)) || (rc = upload_at(c, c->d_rslot, 0, row_of))) return rc;
+    int rl = c->resident_lanes;
+    if (rl <= 0)
+      for (rl = 8; rl > 1 && rmaxn * rl > 512; rl >>= 1) {}
+    const int threads = (rmaxn * rl + 63) / 64 * 64;
+    launch_resident(rl, D, c->d_T, c->d_rlist, (int)rlist.size(), threads, c->cur, c->par, (int)nsteps, c->d_rslot,
+                    c->d_rec, nc, c->stream2);
+    HIP_OK(c, hipGetLastError());
+    HIP_OK(c, hipEventRecord(c->ev_res, c->stream2));
+  }
   // step 1 of the run: rebuild if due, pair list, position half
   auto dr = due(0);
   int pe;
-  if (dr.first) {
+  if (!stepped) {
+    // every chain is resident: no per-step launches; the parities advance as they would
+    if ((nsteps - 1) & 1) {
+      c->cur ^= 1;
+      c->par ^= 1;
+    }
+  } else if (dr.first) {
     pe = prof_begin(c);
     if (dr.second) launch_cellcoords(D, c->cur, 0, nb, s);
     launch_rebuild(c->list_lanes, D, c->d_T, c->cur, c->par, 0, -1, 0, nb, 0, nc, c->binned, s);
@@ -1519,14 +1603,13 @@ int sopg_run(sopg_ctx* c, long nsteps, const double* noise, sopg_record* rec, lo
   // list is fresh reproduces it, so one launch covers every chain
   bool fresh = true;
   for (auto& h : c->chains)
-    if (h.active && !h.pl_fresh) fresh = false;
+    if (h.active && !h.resident && !h.pl_fresh) fresh = false;
   if (!fresh) {
     pe = prof_begin(c);
     launch_pairfilter(c->list_lanes, D, c->d_T, c->cur, c->par, 0, nb, s);
     prof_end(c, SOPG_KERNEL_PAIRLIST, pe);
   }
-  launch_posupdate(D, c->cur, c->cur, 0, nb, s);
-  std::vector<long> rec_steps;
+  if (stepped) launch_posupdate(D, c->cur, c->cur, 0, nb, s);
   // The launches of run step r.  Kernels see step k + *Dx.relb: direct
   // launches pass k = r with base 0; a captured segment passes k relative to
   // its first step, whose run step the replay writes to the base.
@@ -1539,9 +1622,8 @@ int sopg_run(sopg_ctx* c, long nsteps, const double* noise, sopg_record* rec, lo
     if (records(r)) {
       pe = prof_begin(c);
       launch_observe(Dx, c->cur, c->par, k, 0, OBS_PHYS, 0, nb, s);
-      launch_chain_reduce(Dx, 0, nc, c->d_rec + rec_steps.size() * nc * kRed, s);
+      launch_chain_reduce(Dx, 0, nc, c->d_rec + (size_t)row_of[r] * nc * kRed, s);
       prof_end(c, SOPG_KERNEL_OBSERVE, pe);
-      rec_steps.push_back(r);
     }
     if (!last) {
       c->par ^= 1;
@@ -1561,7 +1643,7 @@ int sopg_run(sopg_ctx* c, long nsteps, const double* noise, sopg_record* rec, lo
   // step, replayed noise or kernel timing runs as direct launches.
   long S = 1;
   for (auto& h : c->chains)
-    if (h.active && h.p.cutoff != SOPG_CUTOFF_NONE) {
+    if (h.active && !h.resident && h.p.cutoff != SOPG_CUTOFF_NONE) {
       long a = S, b = h.p.nnlup;
       while (b) { const long t = a % b; a = b; b = t; }
       S = S / a * h.p.nnlup;
@@ -1572,7 +1654,7 @@ int sopg_run(sopg_ctx* c, long nsteps, const double* noise, sopg_record* rec, lo
   Dev Dg;
   std::memcpy(&Dg, &D, sizeof Dg);  // bytewise: the struct is part of the graph key
   Dg.relb = c->d_relb + 1;
-  long r = 0;
+  long r = stepped ? 0 : nsteps;
   while (r < nsteps) {
     const long seg = std::min(S, nsteps - r);
     bool graphable = use_graphs && seg == S && r + S < nsteps;
@@ -1624,7 +1706,11 @@ int sopg_run(sopg_ctx* c, long nsteps, const double* noise, sopg_record* rec, lo
     r += S;
   }
   HIP_OK(c, hipGetLastError());
-  for (auto& h : c->chains) h.pl_fresh = false;
+  if (!rlist.empty()) HIP_OK(c, hipStreamWaitEvent(s, c->ev_res, 0));
+  for (auto& h : c->chains) {
+    h.pl_fresh = false;
+    h.resident = false;
+  }
   std::vector<double> hr(rec_steps.size() * nc * kRed);
   if (!hr.empty())
     HIP_OK(c, hipMemcpyAsync(hr.data(), c->d_rec, sizeof(double) * hr.size(), hipMemcpyDeviceToHost, s));

@@ -41,6 +41,8 @@ struct ChainP {
   uint64_t seed, noise_base;  // Philox key, force evaluations done so far
   int sim_type;        // 1 underdamped, 2 overdamped
   double gcell;        // binned neighbor-list rebuild: grid edge >= largest 3.2 sigma_rep of the chain
+  int resident;        // 1 while sopg_run advances the chain in the resident kernel (the
+                       // per-step kernels then skip it)
 };
 
 // Per-chain grid of the binned rebuild, recomputed at every rebuild.  A chain

@@ -53,12 +53,20 @@ __device__ __forceinline__ void fold3(double& Fx, double& Fy, double& Fz, bool c
   }
 }
 
-// Deterministic group sum (fixed butterfly), used for energy partials only.
+// fold3 for one value (energy terms): the same list order for every L, so
+// a bead's energy sums do not depend on the lane count either.
 template <int L>
-__device__ __forceinline__ double group_sum(double v) {
-#pragma unroll
-  for (int s = L / 2; s >= 1; s >>= 1) v += __shfl_xor(v, s, L);
-  return v;
+__device__ __forceinline__ void fold1(double& E, bool c, double e) {
+  if constexpr (L == 1) {
+    if (c) E += e;
+  } else {
+    uint64_t gm = group_mask<L>(c);
+    while (gm) {
+      const int k = __builtin_ctzll(gm);
+      gm &= gm - 1ull;
+      E += __shfl(e, k, L);
+    }
+  }
 }
 
 // Workgroups are dispatched round-robin over the chip's 8 XCDs, each with its
@@ -138,13 +146,17 @@ __device__ __forceinline__ bool cell_adjacent(const Dev& D, const ChainP& P, int
 // ------------------------------------------------------------------ kernels
 
 // Cell index per bead: floor(unc_pos / lcell) (cell_list.cpp:24-31).
-__global__ void k_cellcoords(Dev D, int cur, int g0, int g1) {
-  const int g = g0 + blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= g1 || D.chain[g] < 0) return;  // past the end, or a free bead
+__device__ __forceinline__ void cellcoords_bead(const Dev& D, int cur, int g) {
   const ChainP& P = D.cp[D.chain[g]];
   D.cx[g] = floor((cur ? D.ux[1] : D.ux[0])[g] / P.lcell);
   D.cy[g] = floor((cur ? D.uy[1] : D.uy[0])[g] / P.lcell);
   D.cz[g] = floor((cur ? D.uz[1] : D.uz[0])[g] / P.lcell);
+}
+__global__ void k_cellcoords(Dev D, int cur, int g0, int g1) {
+  const int g = g0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= g1 || D.chain[g] < 0) return;  // past the end, or a free bead
+  if (D.cp[D.chain[g]].resident) return;  // the resident kernel keeps its own
+  cellcoords_bead(D, cur, g);
 }
 
 // Neighbor-list (distance) or cell-list rebuild plus the pair-list filter at
@@ -152,7 +164,8 @@ __global__ void k_cellcoords(Dev D, int cur, int g0, int g1) {
 // binned kernel below covers compact chains).  mode < 0: each chain's own
 // cutoff, only chains whose rebuild is due at relative step rel; mode >= 0:
 // that criterion, all chains.
-template <int L>
+// RES: called from the resident kernel (which owns its chain and uses no grid).
+template <int L, bool RES = false>
 __device__ __forceinline__ void rebuild_scan_bead(const Dev& D, const Tables* __restrict__ T, int cur, int par,
                                                   int rel, int mode, int g, int l) {
   if (D.chain[g] < 0) return;  // a free bead
@@ -160,9 +173,10 @@ __device__ __forceinline__ void rebuild_scan_bead(const Dev& D, const Tables* __
   int crit = mode;
   if (mode < 0) {
     if (P.cutoff == SOPG_CUTOFF_NONE || (P.phase + rel) % P.nnlup != 0) return;
+    if (!RES && P.resident) return;
     crit = P.cutoff;
   }
-  if (D.grid[D.chain[g]].use) return;  // rebuilt by k_rebuild_grid
+  if (!RES && D.grid[D.chain[g]].use) return;  // rebuilt by k_rebuild_grid
   const double* ux = cur ? D.ux[1] : D.ux[0];
   const double* uy = cur ? D.uy[1] : D.uy[0];
   const double* uz = cur ? D.uz[1] : D.uz[0];
@@ -284,7 +298,7 @@ __device__ __forceinline__ bool rebuild_due(const ChainP& P, int rel, int mode, 
   crit = mode;
   if (mode >= 0) return true;
   crit = P.cutoff;
-  return P.cutoff != SOPG_CUTOFF_NONE && (P.phase + rel) % P.nnlup == 0;
+  return P.cutoff != SOPG_CUTOFF_NONE && !P.resident && (P.phase + rel) % P.nnlup == 0;
 }
 
 // Per-chain extent and grid; one block per chain.  Chains of more than maxn
@@ -594,11 +608,8 @@ __global__ void __launch_bounds__(256) k_rebuild_grid(Dev D, const Tables* __res
 
 // Pair-list filter at the current positions (sop.cpp:473-553), parity par.
 template <int L>
-__global__ void __launch_bounds__(256) k_pairfilter(Dev D, const Tables* __restrict__ T, int cur, int par, int g0, int g1) {
-  const int t = xcd_block() * blockDim.x + threadIdx.x;
-  const int g = g0 + t / L;
-  const int l = t & (L - 1);
-  if (g >= g1 || D.chain[g] < 0) return;  // past the end, or a free bead
+__device__ __forceinline__ void pairfilter_bead(const Dev& D, const Tables* __restrict__ T, int cur, int par, int g,
+                                                int l) {
   const ChainP& P = D.cp[D.chain[g]];
   const double* ux = cur ? D.ux[1] : D.ux[0];
   const double* uy = cur ? D.uy[1] : D.uy[0];
@@ -636,10 +647,17 @@ __global__ void __launch_bounds__(256) k_pairfilter(Dev D, const Tables* __restr
   }
 }
 
-// Position half of underdamped_iteration (sop.cpp:288-310): incr, pos, unc_pos.
-__global__ void k_posupdate(Dev D, int src, int dst, int g0, int g1) {
-  const int g = g0 + blockIdx.x * blockDim.x + threadIdx.x;
+template <int L>
+__global__ void __launch_bounds__(256) k_pairfilter(Dev D, const Tables* __restrict__ T, int cur, int par, int g0, int g1) {
+  const int t = xcd_block() * blockDim.x + threadIdx.x;
+  const int g = g0 + t / L;
   if (g >= g1 || D.chain[g] < 0) return;  // past the end, or a free bead
+  if (D.cp[D.chain[g]].resident) return;  // the resident kernel filters its own
+  pairfilter_bead<L>(D, T, cur, par, g, t & (L - 1));
+}
+
+// Position half of underdamped_iteration (sop.cpp:288-310): incr, pos, unc_pos.
+__device__ __forceinline__ void posupdate_bead(const Dev& D, int src, int dst, int g) {
   const ChainP& P = D.cp[D.chain[g]];
   // underdamped: incr = a1 v + a2 F (sop.cpp:292-294); overdamped: incr = a5 F (sop.cpp:339-341)
   const bool od = P.sim_type == 2;
@@ -653,6 +671,13 @@ __global__ void k_posupdate(Dev D, int src, int dst, int g0, int g1) {
   (dst ? D.ux[1] : D.ux[0])[g] = (src ? D.ux[1] : D.ux[0])[g] + ix;
   (dst ? D.uy[1] : D.uy[0])[g] = (src ? D.uy[1] : D.uy[0])[g] + iy;
   (dst ? D.uz[1] : D.uz[0])[g] = (src ? D.uz[1] : D.uz[0])[g] + iz;
+}
+
+__global__ void k_posupdate(Dev D, int src, int dst, int g0, int g1) {
+  const int g = g0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= g1 || D.chain[g] < 0) return;  // past the end, or a free bead
+  if (D.cp[D.chain[g]].resident) return;  // the resident kernel moves its own
+  posupdate_bead(D, src, dst, g);
 }
 
 // Velocity half of underdamped_iteration (sop.cpp:316-328).
@@ -683,11 +708,7 @@ enum { F_STORE = 1, F_VEL = 2, F_ENERGY = 4, F_ENERGY_ALL = 8, F_NOISE = 16, F_B
 // Type-pair tables the pair loops read, staged in LDS (index it * 3 + jt).
 enum { TP_FREP = 0, TP_S6, TP_S12, TP_PLRC2, TP_CREP, TP_FATT, TP_CATT, TP_N };
 
-template <int L, bool NEXT>
-__global__ void __launch_bounds__(256, 5) k_force(Dev D, const Tables* __restrict__ T, int cur, int pr, int rel, int flags,
-                                               const double* __restrict__ noise, int g0, int g1) {
-  rel += *D.relb;
-  __shared__ double tp[9][TP_N];
+__device__ __forceinline__ void load_type_tables(const Tables* __restrict__ T, double (*tp)[TP_N]) {
   if (threadIdx.x < 9) {
     const int it = threadIdx.x / 3, jt = threadIdx.x % 3;
     tp[threadIdx.x][TP_FREP] = T->frep[it][jt];
@@ -698,11 +719,14 @@ __global__ void __launch_bounds__(256, 5) k_force(Dev D, const Tables* __restric
     tp[threadIdx.x][TP_FATT] = T->fatt[it][jt];
     tp[threadIdx.x][TP_CATT] = T->catt[it][jt];
   }
-  __syncthreads();
-  const int t = xcd_block() * blockDim.x + threadIdx.x;
-  const int g = g0 + t / L;
-  const int l = t & (L - 1);
-  if (g >= g1) return;
+}
+
+// The step for bead g, computed by the L lanes of its group (lane l); the
+// body of k_force and of the resident kernel.
+template <int L, bool NEXT>
+__device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restrict__ T, const double (*tp)[TP_N],
+                                           int cur, int pr, int rel, int flags, const double* __restrict__ noise,
+                                           int g, int l) {
   const int chain = D.chain[g];
   if (chain < 0) return;  // a free bead
   const ChainP* Pp = D.cp + chain;
@@ -759,7 +783,8 @@ __global__ void __launch_bounds__(256, 5) k_force(Dev D, const Tables* __restric
     for (int base = p0; base < p1; base += L) {
       const int q = base + l;
       const bool has = q < p1;
-      double fx = 0.0, fy = 0.0, fz = 0.0;
+      double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
+      bool ec = false;
       if (has) {
         const int pq = D.bond_pq[q], o = pq >> 1, side = pq & 1;
         double dx, dy, dz;
@@ -770,9 +795,10 @@ __global__ void __launch_bounds__(256, 5) k_force(Dev D, const Tables* __restric
         const double temp = -T->k_bnd * dev / d / (1.0 - dev2 / T->R0sq);
         fx = temp * dx; fy = temp * dy; fz = temp * dz;
         if (side == 0) { fx = -fx; fy = -fy; fz = -fz; }
-        if (energy && side == 0) eb += log1p(-dev * dev / T->R0sq);
+        if (energy && side == 0) { e = log1p(-dev * dev / T->R0sq); ec = true; }
       }
       fold3<L>(Fx, Fy, Fz, has, fx, fy, fz);
+      if (energy) fold1<L>(eb, ec, e);
     }
   }
   // soft-sphere (i, i+2) repulsion (energy.cpp:354-400): F_i -= f, F_k += f
@@ -781,7 +807,8 @@ __global__ void __launch_bounds__(256, 5) k_force(Dev D, const Tables* __restric
     for (int base = p0; base < p1; base += L) {
       const int q = base + l;
       const bool has = q < p1;
-      double fx = 0.0, fy = 0.0, fz = 0.0;
+      double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
+      bool ec = false;
       if (has) {
         const int pq = D.ang_pq[q], o = pq >> 1, side = pq & 1;
         double dx, dy, dz;
@@ -791,9 +818,10 @@ __global__ void __launch_bounds__(256, 5) k_force(Dev D, const Tables* __restric
         const double co1 = T->f_ang_ss_coeff / d8;
         fx = co1 * dx; fy = co1 * dy; fz = co1 * dz;
         if (side == 0) { fx = -fx; fy = -fy; fz = -fz; }
-        if (energy && side == 0) ea += T->e_ang_ss_coeff / pow6_cr(d);
+        if (energy && side == 0) { e = T->e_ang_ss_coeff / pow6_cr(d); ec = true; }
       }
       fold3<L>(Fx, Fy, Fz, has, fx, fy, fz);
+      if (energy) fold1<L>(ea, ec, e);
     }
   }
   // native LJ 12-6 over the pair list (energy.cpp:274-311): F_i += f, F_j -= f
@@ -816,8 +844,8 @@ __global__ void __launch_bounds__(256, 5) k_force(Dev D, const Tables* __restric
         const int o = D.att_partner[q + L];
         pxn = ux[o]; pyn = uy[o]; pzn = uz[o];
       }
-      bool contrib = false;
-      double fx = 0.0, fy = 0.0, fz = 0.0;
+      bool contrib = false, ec = false;
+      double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
       if (q < p1) {
         if (f & 1) {
           const AttK K = D.att_k[q];
@@ -837,7 +865,7 @@ __global__ void __launch_bounds__(256, 5) k_force(Dev D, const Tables* __restric
               if (side == 1) { fx = -fx; fy = -fy; fz = -fz; }
               contrib = true;
             }
-            if (energy && side == 0) eatt += tp[tpi][TP_CATT] * (r12 / d12) - 2.0 * (r6 / d6);
+            if (energy && side == 0) { e = tp[tpi][TP_CATT] * (r12 / d12) - 2.0 * (r6 / d6); ec = true; }
           }
           if constexpr (NEXT) {
             const bool npl = no_cut || d2 < K.plrc2;
@@ -847,6 +875,7 @@ __global__ void __launch_bounds__(256, 5) k_force(Dev D, const Tables* __restric
         }
       }
       fold3<L>(Fx, Fy, Fz, contrib, fx, fy, fz);
+      if (energy) fold1<L>(eatt, ec, e);
     }
   }
   // non-native repulsion over the pair list (energy.cpp:313-350): F_i += f, F_j -= f
@@ -873,8 +902,8 @@ __global__ void __launch_bounds__(256, 5) k_force(Dev D, const Tables* __restric
         const int o = off + (int)(wn & kPartnerMask);
         pxn = ux[o]; pyn = uy[o]; pzn = uz[o];
       }
-      bool contrib = false;
-      double fx = 0.0, fy = 0.0, fz = 0.0;
+      bool contrib = false, ec = false;
+      double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
       if (q < cnt) {
         const int side = (w >> 25) & 1;
         const int tpi = (int)((w >> 26) & 3) * 3 + (int)((w >> 28) & 3);
@@ -893,7 +922,7 @@ __global__ void __launch_bounds__(256, 5) k_force(Dev D, const Tables* __restric
             if (side == 1) { fx = -fx; fy = -fy; fz = -fz; }
             contrib = true;
           }
-          if (energy && side == 0) erep += tp[tpi][TP_CREP] * (s12 / d12 + s6 / d6);
+          if (energy && side == 0) { e = tp[tpi][TP_CREP] * (s12 / d12 + s6 / d6); ec = true; }
         }
         if constexpr (NEXT) {
           const bool npl = no_cut || d2 < tp[tpi][TP_PLRC2];
@@ -902,12 +931,10 @@ __global__ void __launch_bounds__(256, 5) k_force(Dev D, const Tables* __restric
         }
       }
       fold3<L>(Fx, Fy, Fz, contrib, fx, fy, fz);
+      if (energy) fold1<L>(erep, ec, e);
     }
   }
 
-  if (energy) {
-    eb = group_sum<L>(eb); ea = group_sum<L>(ea); eatt = group_sum<L>(eatt); erep = group_sum<L>(erep);
-  }
   if (l != 0) return;
   if (energy) {
     double* r = D.red + (size_t)g * kRed;
@@ -946,12 +973,24 @@ __global__ void __launch_bounds__(256, 5) k_force(Dev D, const Tables* __restric
   }
 }
 
+template <int L, bool NEXT>
+__global__ void __launch_bounds__(256, 5) k_force(Dev D, const Tables* __restrict__ T, int cur, int pr, int rel, int flags,
+                                               const double* __restrict__ noise, int g0, int g1) {
+  rel += *D.relb;
+  __shared__ double tp[9][TP_N];
+  load_type_tables(T, tp);
+  __syncthreads();
+  const int t = xcd_block() * blockDim.x + threadIdx.x;
+  const int g = g0 + t / L;
+  if (g >= g1) return;
+  const int chain = D.chain[g];
+  if (chain < 0 || D.cp[chain].resident) return;  // a free bead, or a chain the resident kernel advances
+  force_bead<L, NEXT>(D, T, tp, cur, pr, rel, flags, noise, g, t & (L - 1));
+}
+
 // Observables (sop.cpp:200-280) and list counts, per bead.
 enum { OBS_PHYS = 1, OBS_COUNTS = 2 };
-__global__ void k_observe(Dev D, int cur, int par, int rel, int all, int what, int g0, int g1) {
-  rel += *D.relb;
-  const int g = g0 + blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= g1 || D.chain[g] < 0) return;  // past the end, or a free bead
+__device__ __forceinline__ void observe_bead(const Dev& D, int cur, int par, int rel, int all, int what, int g) {
   const ChainP& P = D.cp[D.chain[g]];
   if (!all && (P.iup0 + rel) % P.nup != 0) return;
   const double* ux = cur ? D.ux[1] : D.ux[0];
@@ -1005,22 +1044,103 @@ __global__ void k_observe(Dev D, int cur, int par, int rel, int all, int what, i
   r[R_CONT] = cont; r[R_RG] = rg; r[R_V2] = v2;
 }
 
-// Per-chain sums of the per-bead fields: one block per chain, fixed order.
-__global__ void __launch_bounds__(256) k_chain_reduce(Dev D, int c0, double* out) {
-  const int c = c0 + blockIdx.x;
+__global__ void k_observe(Dev D, int cur, int par, int rel, int all, int what, int g0, int g1) {
+  rel += *D.relb;
+  const int g = g0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= g1 || D.chain[g] < 0) return;  // past the end, or a free bead
+  if (!all && D.cp[D.chain[g]].resident) return;
+  observe_bead(D, cur, par, rel, all, what, g);
+}
+
+// Sums of chain c's per-bead fields into out[kRed], in a fixed order: 256
+// strided partial sums, then a tree -- the same order for any block size (a
+// thread takes entries t, t + blockDim.x, ... of each level).  sh: 256
+// doubles of LDS.
+__device__ __forceinline__ void chain_reduce_block(const Dev& D, int c, double* out, double* sh) {
   const ChainP& P = D.cp[c];
-  __shared__ double sh[256];
+  const int t = threadIdx.x, nt = blockDim.x;
   for (int f = 0; f < kRed; f++) {
-    double s = 0.0;
-    for (int b = threadIdx.x; b < P.n; b += blockDim.x) s += D.red[(size_t)(P.off + b) * kRed + f];
-    sh[threadIdx.x] = s;
+    for (int u = t; u < 256; u += nt) {
+      double s = 0.0;
+      for (int b = u; b < P.n; b += 256) s += D.red[(size_t)(P.off + b) * kRed + f];
+      sh[u] = s;
+    }
     __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-      if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    for (int w = 128; w > 0; w >>= 1) {
+      for (int u = t; u < w; u += nt) sh[u] += sh[u + w];
       __syncthreads();
     }
-    if (threadIdx.x == 0) out[(size_t)blockIdx.x * kRed + f] = sh[0];
+    if (t == 0) out[f] = sh[0];
     __syncthreads();
+  }
+}
+
+// Per-chain sums of the per-bead fields: one block of 256 per chain.
+// Chains the resident kernel advances write their own sums during sopg_run.
+__global__ void __launch_bounds__(256) k_chain_reduce(Dev D, int c0, int skip_resident, double* out) {
+  const int c = c0 + blockIdx.x;
+  if (skip_resident && D.cp[c].resident) return;
+  __shared__ double sh[256];
+  chain_reduce_block(D, c, out + (size_t)blockIdx.x * kRed, sh);
+}
+
+// ------------------------------------------------------------ resident kernel
+//
+// The persistent form of sopg_run for small chains: one block per chain (the
+// chains[] list) advances that chain through every step of the run -- the
+// run's first list rebuild, pair-list filter and position half, then per
+// step the fused force / velocity / next-position pass, the record when due
+// and the list rebuild when due -- with block barriers between the phases
+// and no other launch.  It calls the per-bead code of the per-step kernels
+// (force_bead, rebuild_scan_bead, pairfilter_bead, posupdate_bead,
+// observe_bead, chain_reduce_block), so its results are the same bits; the
+// rebuild is the full candidate scan, whose lists equal the binned ones.
+// rec_slot[s] is the record buffer row of run step s (-1: no record).
+template <int L>
+__global__ void __launch_bounds__(512) k_resident(Dev D, const Tables* __restrict__ T, const int* __restrict__ chains,
+                                                   int cur, int par, int nsteps, const int* __restrict__ rec_slot,
+                                                   double* rec, int nc) {
+  __shared__ double tp[9][TP_N];
+  __shared__ double sh[256];
+  load_type_tables(T, tp);
+  const int c = chains[blockIdx.x];
+  const ChainP& P = D.cp[c];
+  const int n = P.n, off = P.off;
+  const int groups = (int)blockDim.x / L, grp = (int)threadIdx.x / L, l = (int)threadIdx.x & (L - 1);
+  const int flags = F_STORE | F_VEL | F_ENERGY | F_NOISE | F_BOND | F_ANGLE | F_VDW;
+  auto rebuild = [&](int step) {
+    if (P.cutoff == SOPG_CUTOFF_NONE || (P.phase + step) % P.nnlup != 0) return;
+    if (P.cutoff == SOPG_CUTOFF_CELL) {
+      for (int b = threadIdx.x; b < n; b += blockDim.x) cellcoords_bead(D, cur, off + b);
+      __syncthreads();
+    }
+    for (int b = grp; b < n; b += groups) rebuild_scan_bead<L, true>(D, T, cur, par, step, -1, off + b, l);
+    __syncthreads();
+  };
+  __syncthreads();
+  // the run's first step: rebuild if due, pair list, position half
+  rebuild(0);
+  for (int b = grp; b < n; b += groups) pairfilter_bead<L>(D, T, cur, par, off + b, l);
+  __syncthreads();
+  for (int b = threadIdx.x; b < n; b += blockDim.x) posupdate_bead(D, cur, cur, off + b);
+  __syncthreads();
+  for (int s = 0; s < nsteps; s++) {
+    const bool last = s == nsteps - 1;
+    for (int b = grp; b < n; b += groups) {
+      if (last) force_bead<L, false>(D, T, tp, cur, par, s, flags, nullptr, off + b, l);
+      else force_bead<L, true>(D, T, tp, cur, par, s, flags, nullptr, off + b, l);
+    }
+    __syncthreads();
+    if (rec_slot[s] >= 0 && (P.iup0 + s) % P.nup == 0) {
+      for (int b = threadIdx.x; b < n; b += blockDim.x) observe_bead(D, cur, par, s, 0, OBS_PHYS, off + b);
+      __syncthreads();
+      chain_reduce_block(D, c, rec + ((size_t)rec_slot[s] * nc + c) * kRed, sh);
+    }
+    if (!last) {
+      par ^= 1;
+      rebuild(s + 1);
+      cur ^= 1;
+    }
   }
 }
 
@@ -1110,11 +1230,18 @@ void launch_velupdate(const Dev& D, int g0, int g1, hipStream_t s) {
 void launch_observe(const Dev& D, int cur, int par, int rel, int all, int what, int g0, int g1, hipStream_t s) {
   hipLaunchKernelGGL(k_observe, dim3(blocks_for(g1 - g0)), dim3(256), 0, s, D, cur, par, rel, all, what, g0, g1);
 }
+void launch_resident(int L, const Dev& D, const Tables* T, const int* chains, int nchains, int threads, int cur,
+                     int par, int nsteps, const int* rec_slot, double* rec, int nc, hipStream_t s) {
+#define C_(X) hipLaunchKernelGGL(k_resident<X>, dim3(nchains), dim3(threads), 0, s, D, T, chains, cur, par, nsteps, \
+                                 rec_slot, rec, nc)
+  SOPG_DISPATCH_L(L, C_)
+#undef C_
+}
 void launch_set_base(int* base, int value, hipStream_t s) {
   hipLaunchKernelGGL(k_set_base, dim3(1), dim3(1), 0, s, base, value);
 }
 void launch_chain_reduce(const Dev& D, int c0, int nc, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_chain_reduce, dim3(nc), dim3(256), 0, s, D, c0, out);
+  hipLaunchKernelGGL(k_chain_reduce, dim3(nc), dim3(256), 0, s, D, c0, 1, out);
 }
 
 }  // namespace sopg

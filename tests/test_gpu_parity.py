@@ -478,6 +478,72 @@ def test_graph_replay_equals_direct(inputs_root, cutoff):
     assert ra == rg
 
 
+@pytest.mark.parametrize("cutoff,lanes", [(sopg.CUTOFF_NEIGHBOR, 0), (sopg.CUTOFF_CELL, 0),
+                                          (sopg.CUTOFF_NEIGHBOR, 1), (sopg.CUTOFF_NEIGHBOR, 4)])
+def test_resident_kernel_equals_per_step(inputs_root, cutoff, lanes):
+    """Small chains advanced by the persistent resident kernel (all steps of a
+    run in one launch, next to the per-step kernels of the large chain) end
+    with the same bits, and give the same records, as the per-step path --
+    over several runs, with their own rebuild and record periods."""
+    chains = []
+    for n, seed, nnlup, nup in ((50, 111, 20, 100), (76, 112, 15, 50), (120, 113, 20, 100), (300, 114, 20, 75)):
+        x, v, _ = chain_inputs(inputs_root, n, seed)
+        chains.append((x, v, 60 + seed, nnlup, nup))
+    results = []
+    for resident in (0, 128):
+        ctx = sopg.Context()
+        for (x, v, seed, nnlup, nup) in chains:
+            ctx.add_chain(sopg.chain_topology(x),
+                          sopg.chain_params(boxl=504.0, ncell=42.0, cutoff=cutoff, nnlup=nnlup, nup=nup,
+                                            noise_mode=sopg.NOISE_PHILOX, seed=seed))
+        ctx.set_resident(resident, lanes)
+        for c, (x, v, *_rest) in enumerate(chains):
+            ctx.put_state(c, pos=x, unc=x, vel=v, force=np.zeros_like(x))
+            (ctx.update_cell_list if cutoff == sopg.CUTOFF_CELL else ctx.update_neighbor_list)(c)
+            ctx.update_pair_list(c)
+            ctx.force_eval(c)
+        print(f"[gpu] resident={resident} lanes={lanes} cutoff={cutoff}", flush=True)
+        per_run, recs = [], []
+        for nsteps in (250, 251, 170):
+            recs += ctx.run(nsteps)
+            ctx.energy_eval(1)  # eager work between runs
+            per_run.append([ctx.get_state(c) for c in range(len(chains))])
+        results.append((per_run, recs))
+        ctx.close()
+    (da, ra), (ga, rg) = results
+    for run_a, run_g in zip(da, ga):
+        for c, (sa, sg) in enumerate(zip(run_a, run_g)):
+            for k in ("unc", "vel", "force", "pos"):
+                assert np.array_equal(sa[k], sg[k]), (c, k)
+    assert {r["chain"] for r in ra} == {0, 1, 2, 3}
+    assert sorted(ra, key=lambda r: (r["istep"], r["chain"])) == sorted(rg, key=lambda r: (r["istep"], r["chain"]))
+
+
+def test_resident_only_run_all_chains_resident(inputs_root):
+    """A run whose chains are all resident launches no per-step kernels; the
+    result equals the per-step path's, and a following per-step run (resident
+    off) continues from it exactly."""
+    chains = []
+    for n, seed in ((60, 121), (76, 122), (90, 123)):
+        x, v, _ = chain_inputs(inputs_root, n, seed)
+        chains.append((x, v, 80 + seed))
+    results = []
+    for resident in (0, 100):
+        ctx = _philox_ctx(chains)
+        ctx.set_resident(resident)
+        print(f"[gpu] all-resident={resident}", flush=True)
+        recs = ctx.run(333)
+        ctx.set_resident(0)
+        recs += ctx.run(120)
+        results.append(([ctx.get_state(c) for c in range(len(chains))], recs))
+        ctx.close()
+    (sa, ra), (sg, rg) = results
+    for a, g in zip(sa, sg):
+        for k in ("unc", "vel", "force", "pos"):
+            assert np.array_equal(a[k], g[k]), k
+    assert ra == rg
+
+
 def test_long_run_statistics_match_reference(inputs_root):
     """The device generator (Philox) is not the reference's stream, so its
     trajectories differ from the reference's; long-run averages of Q and Rg^2
