@@ -34,11 +34,34 @@ __device__ __forceinline__ uint64_t group_mask(bool c) {
   }
 }
 
+// Lane K of each L-lane group (L = 2 or 4) to all lanes of the group: a DPP
+// quad permutation (no LDS round trip), 32 bits at a time.
+template <int K, int L>
+__device__ __forceinline__ double group_bcast(double v) {
+  constexpr int s01 = K, s23 = (L == 4) ? K : 2 + K;
+  constexpr int ctrl = s01 | s01 << 2 | s23 << 4 | s23 << 6;
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)b, ctrl, 0xF, 0xF, false);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(b >> 32), ctrl, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, (uint64_t)lo | ((uint64_t)hi << 32));
+}
+
 // F += contributions of the group's lanes, in lane order (= list order).
+// For L = 2 and 4 every lane is added in turn, a non-contributing lane as
+// -0.0: x + (-0.0) is x for every x (both zeros included), so the sums are
+// the same bits as adding the contributing lanes only.
 template <int L>
 __device__ __forceinline__ void fold3(double& Fx, double& Fy, double& Fz, bool c, double fx, double fy, double fz) {
   if constexpr (L == 1) {
     if (c) { Fx += fx; Fy += fy; Fz += fz; }
+  } else if constexpr (L == 2 || L == 4) {
+    const double ax = c ? fx : -0.0, ay = c ? fy : -0.0, az = c ? fz : -0.0;
+    Fx += group_bcast<0, L>(ax); Fy += group_bcast<0, L>(ay); Fz += group_bcast<0, L>(az);
+    Fx += group_bcast<1, L>(ax); Fy += group_bcast<1, L>(ay); Fz += group_bcast<1, L>(az);
+    if constexpr (L == 4) {
+      Fx += group_bcast<2, L>(ax); Fy += group_bcast<2, L>(ay); Fz += group_bcast<2, L>(az);
+      Fx += group_bcast<3, L>(ax); Fy += group_bcast<3, L>(ay); Fz += group_bcast<3, L>(az);
+    }
   } else {
     uint64_t gm = group_mask<L>(c);
     // group-uniform loop over the contributing lanes, lowest lane (= earliest
@@ -59,6 +82,14 @@ template <int L>
 __device__ __forceinline__ void fold1(double& E, bool c, double e) {
   if constexpr (L == 1) {
     if (c) E += e;
+  } else if constexpr (L == 2 || L == 4) {
+    const double a = c ? e : -0.0;
+    E += group_bcast<0, L>(a);
+    E += group_bcast<1, L>(a);
+    if constexpr (L == 4) {
+      E += group_bcast<2, L>(a);
+      E += group_bcast<3, L>(a);
+    }
   } else {
     uint64_t gm = group_mask<L>(c);
     while (gm) {
@@ -762,9 +793,15 @@ __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restric
     } else {
       double a, c;
       philox_gauss2(Pp->seed, ctr, bead, (uint32_t)(l & 1), a, c);
-      gx = __shfl(a, 0, L);
-      gy = __shfl(c, 0, L);
-      gz = __shfl(a, 1, L);
+      if constexpr (L == 2 || L == 4) {
+        gx = group_bcast<0, L>(a);
+        gy = group_bcast<0, L>(c);
+        gz = group_bcast<1, L>(a);
+      } else {
+        gx = __shfl(a, 0, L);
+        gy = __shfl(c, 0, L);
+        gz = __shfl(a, 1, L);
+      }
     }
   }
   double Fx = (flags & F_ACCUM) ? D.fx[g] : 0.0;

@@ -98,7 +98,7 @@ def compare_records(drecs, orecs):
             assert rel(a[k], b[k]) <= 1e-12, (k, a[k], b[k])
 
 
-@pytest.mark.parametrize("lanes", [1, 8])
+@pytest.mark.parametrize("lanes", [1, 4, 8])
 def test_lists_forces_energy_nl76(inputs_root, pow_cr, lanes):
     n = 76
     params = sopg.chain_params(cutoff=sopg.CUTOFF_NEIGHBOR, nnlup=50, nup=100, noise_mode=sopg.NOISE_REPLAY)
@@ -285,7 +285,7 @@ def test_batch_equals_alone(inputs_root):
         assert [r for r in rm if r["chain"] == c] == [dict(r, chain=c) for r in ra]
 
 
-@pytest.mark.parametrize("lanes", [1, 2, 16, 64])
+@pytest.mark.parametrize("lanes", [1, 2, 4, 16, 64])
 def test_lanes_invariance(inputs_root, lanes):
     x, v, _ = chain_inputs(inputs_root, 150, 31)
     base = _philox_ctx([(x, v, 5)], lanes=8)
