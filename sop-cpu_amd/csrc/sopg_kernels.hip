@@ -814,51 +814,43 @@ __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restric
     Fz += var * gz;
   }
 
-  // FENE bonds (energy.cpp:402-447): F_i -= f, F_j += f
-  if (flags & F_BOND) {
-    const int p0 = D.bond_beg[g], p1 = D.bond_end[g];
-    for (int base = p0; base < p1; base += L) {
+  // FENE bonds (energy.cpp:402-447): F_i -= f, F_j += f; then soft-sphere
+  // (i, i+2) repulsion (energy.cpp:354-400): F_i -= f, F_k += f.  One loop
+  // over the bead's bond incidences followed by its angle incidences, so a
+  // chunk of L lanes covers both (about two of each per bead) and the fold
+  // still adds every bond before every angle, each in list order.
+  if (flags & (F_BOND | F_ANGLE)) {
+    const int b0 = (flags & F_BOND) ? D.bond_beg[g] : 0, nbq = (flags & F_BOND) ? D.bond_end[g] - b0 : 0;
+    const int a0 = (flags & F_ANGLE) ? D.ang_beg[g] : 0, naq = (flags & F_ANGLE) ? D.ang_end[g] - a0 : 0;
+    const int ntot = nbq + naq;
+    for (int base = 0; base < ntot; base += L) {
       const int q = base + l;
-      const bool has = q < p1;
+      const bool has = q < ntot, isb = q < nbq;
       double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
       bool ec = false;
       if (has) {
-        const int pq = D.bond_pq[q], o = pq >> 1, side = pq & 1;
+        const int pq = isb ? D.bond_pq[b0 + q] : D.ang_pq[a0 + q - nbq], o = pq >> 1, side = pq & 1;
         double dx, dy, dz;
         pair_d3(side, cx, cy, cz, ux[o], uy[o], uz[o], boxl, qbox, dx, dy, dz);
         const double d = sqrt(dx * dx + dy * dy + dz * dz);
-        const double dev = d - D.bond_r0[q];
-        const double dev2 = dev * dev;
-        const double temp = -T->k_bnd * dev / d / (1.0 - dev2 / T->R0sq);
-        fx = temp * dx; fy = temp * dy; fz = temp * dz;
-        if (side == 0) { fx = -fx; fy = -fy; fz = -fz; }
-        if (energy && side == 0) { e = log1p(-dev * dev / T->R0sq); ec = true; }
-      }
-      fold3<L>(Fx, Fy, Fz, has, fx, fy, fz);
-      if (energy) fold1<L>(eb, ec, e);
-    }
-  }
-  // soft-sphere (i, i+2) repulsion (energy.cpp:354-400): F_i -= f, F_k += f
-  if (flags & F_ANGLE) {
-    const int p0 = D.ang_beg[g], p1 = D.ang_end[g];
-    for (int base = p0; base < p1; base += L) {
-      const int q = base + l;
-      const bool has = q < p1;
-      double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
-      bool ec = false;
-      if (has) {
-        const int pq = D.ang_pq[q], o = pq >> 1, side = pq & 1;
-        double dx, dy, dz;
-        pair_d3(side, cx, cy, cz, ux[o], uy[o], uz[o], boxl, qbox, dx, dy, dz);
-        const double d = sqrt(dx * dx + dy * dy + dz * dz);
-        const double d8 = pow8_cr(d);
-        const double co1 = T->f_ang_ss_coeff / d8;
+        double co1;
+        if (isb) {
+          const double dev = d - D.bond_r0[b0 + q];
+          const double dev2 = dev * dev;
+          co1 = -T->k_bnd * dev / d / (1.0 - dev2 / T->R0sq);
+          if (energy && side == 0) { e = log1p(-dev * dev / T->R0sq); ec = true; }
+        } else {
+          co1 = T->f_ang_ss_coeff / pow8_cr(d);
+          if (energy && side == 0) { e = T->e_ang_ss_coeff / pow6_cr(d); ec = true; }
+        }
         fx = co1 * dx; fy = co1 * dy; fz = co1 * dz;
         if (side == 0) { fx = -fx; fy = -fy; fz = -fz; }
-        if (energy && side == 0) { e = T->e_ang_ss_coeff / pow6_cr(d); ec = true; }
       }
       fold3<L>(Fx, Fy, Fz, has, fx, fy, fz);
-      if (energy) fold1<L>(ea, ec, e);
+      if (energy) {
+        fold1<L>(eb, ec && isb, e);
+        fold1<L>(ea, ec && !isb, e);
+      }
     }
   }
   // native LJ 12-6 over the pair list (energy.cpp:274-311): F_i += f, F_j -= f
