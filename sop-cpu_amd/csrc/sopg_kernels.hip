@@ -857,22 +857,22 @@ __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restric
   if (flags & F_VDW) {
     const int p0 = D.att_beg[g], p1 = D.att_end[g];
     // software pipeline as in the non-native loop below
-    uint8_t fn = 0;
+    uint8_t fn = 0, fn2 = 0;
+    int on2 = 0;
     double pxn = 0.0, pyn = 0.0, pzn = 0.0;
     if (p0 + l < p1) {
       fn = D.att_flag[p0 + l];
       const int o = D.att_partner[p0 + l];
       pxn = ux[o]; pyn = uy[o]; pzn = uz[o];
     }
+    if (p0 + l + L < p1) { fn2 = D.att_flag[p0 + l + L]; on2 = D.att_partner[p0 + l + L]; }
     for (int base = p0; base < p1; base += L) {
       const int q = base + l;
       const uint8_t f = fn;
       const double px = pxn, py = pyn, pz = pzn;
-      if (q + L < p1) {
-        fn = D.att_flag[q + L];
-        const int o = D.att_partner[q + L];
-        pxn = ux[o]; pyn = uy[o]; pzn = uz[o];
-      }
+      if (q + L < p1) { pxn = ux[on2]; pyn = uy[on2]; pzn = uz[on2]; }
+      fn = fn2;
+      if (q + 2 * L < p1) { fn2 = D.att_flag[q + 2 * L]; on2 = D.att_partner[q + 2 * L]; }
       bool contrib = false, ec = false;
       double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
       if (q < p1) {
@@ -913,24 +913,27 @@ __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restric
     uint32_t* S = D.slot + (size_t)g * D.cap;
     const uint32_t rbit = 1u << (30 + pr), wbit = 1u << (30 + pw);
     const double thr_rep = T->thr_rep;
-    // software pipeline: the next chunk's slot word and partner coordinates
-    // are loaded before the current chunk is evaluated and folded
-    uint32_t wn = 0u;
+    // software pipeline: the partner coordinates of the next chunk and the
+    // slot words of the one after it are loaded before the current chunk is
+    // evaluated and folded, so no load waits on one issued in the same pass
+    uint32_t wn = 0u, wn2 = 0u;
     double pxn = 0.0, pyn = 0.0, pzn = 0.0;
     if (l < cnt) {
       wn = S[l];
       const int o = off + (int)(wn & kPartnerMask);
       pxn = ux[o]; pyn = uy[o]; pzn = uz[o];
     }
+    if (l + L < cnt) wn2 = S[l + L];
     for (int base = 0; base < cnt; base += L) {
       const int q = base + l;
       const uint32_t w = wn;
       const double px = pxn, py = pyn, pz = pzn;
       if (q + L < cnt) {
-        wn = S[q + L];
-        const int o = off + (int)(wn & kPartnerMask);
+        const int o = off + (int)(wn2 & kPartnerMask);
         pxn = ux[o]; pyn = uy[o]; pzn = uz[o];
       }
+      wn = wn2;
+      if (q + 2 * L < cnt) wn2 = S[q + 2 * L];
       bool contrib = false, ec = false;
       double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
       if (q < cnt) {
