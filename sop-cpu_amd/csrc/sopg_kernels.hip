@@ -973,15 +973,26 @@ __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restric
     r[R_EBND] = eb; r[R_EANG] = ea; r[R_EATT] = eatt; r[R_EREP] = erep;
   }
   if (!(flags & F_STORE)) return;
-  const bool vel = (flags & F_VEL) && Pp->vel_on && Pp->sim_type != 2;
-  const bool od = Pp->sim_type == 2;
+  // every load of the integration first (one memory round trip), then the
+  // arithmetic and the stores
+  const int vel_on = Pp->vel_on, sim = Pp->sim_type;
   const double a1 = Pp->a1, a2 = Pp->a2, a3 = Pp->a3, a4 = Pp->a4, a5 = Pp->a5;
+  const double vx0 = D.vx[g], vy0 = D.vy[g], vz0 = D.vz[g];
+  const double ix0 = D.ix[g], iy0 = D.iy[g], iz0 = D.iz[g];
+  double psx = 0.0, psy = 0.0, psz = 0.0;
+  if constexpr (NEXT) {
+    psx = (cur ? D.px[1] : D.px[0])[g];
+    psy = (cur ? D.py[1] : D.py[0])[g];
+    psz = (cur ? D.pz[1] : D.pz[0])[g];
+  }
+  const bool vel = (flags & F_VEL) && vel_on && sim != 2;
+  const bool od = sim == 2;
   D.fx[g] = Fx; D.fy[g] = Fy; D.fz[g] = Fz;
-  double vx = D.vx[g], vy = D.vy[g], vz = D.vz[g];
+  double vx = vx0, vy = vy0, vz = vz0;
   if (vel) {
-    vx = a3 * D.ix[g] + a4 * Fx;
-    vy = a3 * D.iy[g] + a4 * Fy;
-    vz = a3 * D.iz[g] + a4 * Fz;
+    vx = a3 * ix0 + a4 * Fx;
+    vy = a3 * iy0 + a4 * Fy;
+    vz = a3 * iz0 + a4 * Fz;
     D.vx[g] = vx; D.vy[g] = vy; D.vz[g] = vz;
   }
   if constexpr (NEXT) {
@@ -990,15 +1001,9 @@ __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restric
     const double iy = od ? a5 * Fy : a1 * vy + a2 * Fy;
     const double iz = od ? a5 * Fz : a1 * vz + a2 * Fz;
     D.ix[g] = ix; D.iy[g] = iy; D.iz[g] = iz;
-    const double* pxs = cur ? D.px[1] : D.px[0];
-    const double* pys = cur ? D.py[1] : D.py[0];
-    const double* pzs = cur ? D.pz[1] : D.pz[0];
-    double* pxd = cur ? D.px[0] : D.px[1];
-    double* pyd = cur ? D.py[0] : D.py[1];
-    double* pzd = cur ? D.pz[0] : D.pz[1];
-    pxd[g] = mimg(pxs[g] + ix, boxl, qbox);
-    pyd[g] = mimg(pys[g] + iy, boxl, qbox);
-    pzd[g] = mimg(pzs[g] + iz, boxl, qbox);
+    (cur ? D.px[0] : D.px[1])[g] = mimg(psx + ix, boxl, qbox);
+    (cur ? D.py[0] : D.py[1])[g] = mimg(psy + iy, boxl, qbox);
+    (cur ? D.pz[0] : D.pz[1])[g] = mimg(psz + iz, boxl, qbox);
     (cur ? D.ux[0] : D.ux[1])[g] = cx + ix;
     (cur ? D.uy[0] : D.uy[1])[g] = cy + iy;
     (cur ? D.uz[0] : D.uz[1])[g] = cz + iz;
