@@ -753,24 +753,33 @@ __device__ __forceinline__ void load_type_tables(const Tables* __restrict__ T, d
 }
 
 // The step for bead g, computed by the L lanes of its group (lane l); the
-// body of k_force and of the resident kernel.
-template <int L, bool NEXT>
+// body of k_force (SKIP_RES: chains the resident kernel advances are left
+// alone) and of the resident kernel.
+template <int L, bool NEXT, bool SKIP_RES = false>
 __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restrict__ T, const double (*tp)[TP_N],
                                            int cur, int pr, int rel, int flags, const double* __restrict__ noise,
                                            int g, int l) {
+  // loads in two batches: bead-indexed ones (any bead below the high-water
+  // mark is valid memory), then the chain's fields
   const int chain = D.chain[g];
-  if (chain < 0) return;  // a free bead
-  const ChainP* Pp = D.cp + chain;
-  // chain fields the loops use, in registers (the loops' stores could alias them)
-  const int off = Pp->off, att_off = Pp->att_off, natt = Pp->natt;
-  const bool no_cut = Pp->cutoff == SOPG_CUTOFF_NONE;
-  const double boxl = Pp->boxl, qbox = Pp->qbox, thr_att = Pp->thr_att;
   const double* __restrict__ ux = cur ? D.ux[1] : D.ux[0];
   const double* __restrict__ uy = cur ? D.uy[1] : D.uy[0];
   const double* __restrict__ uz = cur ? D.uz[1] : D.uz[0];
   const double cx = ux[g], cy = uy[g], cz = uz[g];
+  const int bond0 = D.bond_beg[g], bond1 = D.bond_end[g], ang0 = D.ang_beg[g], ang1 = D.ang_end[g];
+  const int att0 = D.att_beg[g], att1 = D.att_end[g], nslot = D.cnt[g];
+  if (chain < 0) return;  // a free bead
+  const ChainP* Pp = D.cp + chain;
+  // chain fields the loops use, in registers (the loops' stores could alias them)
+  const int off = Pp->off, att_off = Pp->att_off, natt = Pp->natt;
+  const int cutoff = Pp->cutoff, noise_mode = Pp->noise_mode, iup0 = Pp->iup0, nup = Pp->nup;
+  const int resident = Pp->resident;
+  const double boxl = Pp->boxl, qbox = Pp->qbox, thr_att = Pp->thr_att, var = Pp->var;
+  const uint64_t seed = Pp->seed, noise_base = Pp->noise_base;
+  if (SKIP_RES && resident) return;
+  const bool no_cut = cutoff == SOPG_CUTOFF_NONE;
   const int pw = pr ^ 1;
-  const bool energy = (flags & F_ENERGY_ALL) || ((flags & F_ENERGY) && ((Pp->iup0 + rel) % Pp->nup == 0));
+  const bool energy = (flags & F_ENERGY_ALL) || ((flags & F_ENERGY) && ((iup0 + rel) % nup == 0));
   double eb = 0.0, ea = 0.0, eatt = 0.0, erep = 0.0;
 
   // clear_forces, then random_force (energy.cpp:35-45, 449-465):
@@ -779,20 +788,20 @@ __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restric
   const bool with_noise = (flags & F_STORE) && (flags & F_NOISE);
   if (!with_noise) {
     // energy only, or the random term not requested
-  } else if (Pp->noise_mode == SOPG_NOISE_REPLAY) {
+  } else if (noise_mode == SOPG_NOISE_REPLAY) {
     gx = noise[3 * (size_t)g];
     gy = noise[3 * (size_t)g + 1];
     gz = noise[3 * (size_t)g + 2];
   } else {
-    const uint64_t ctr = Pp->noise_base + (uint64_t)rel;
+    const uint64_t ctr = noise_base + (uint64_t)rel;
     const uint32_t bead = (uint32_t)(g - off);
     if constexpr (L == 1) {
       double u;
-      philox_gauss2(Pp->seed, ctr, bead, 0, gx, gy);
-      philox_gauss2(Pp->seed, ctr, bead, 1, gz, u);
+      philox_gauss2(seed, ctr, bead, 0, gx, gy);
+      philox_gauss2(seed, ctr, bead, 1, gz, u);
     } else {
       double a, c;
-      philox_gauss2(Pp->seed, ctr, bead, (uint32_t)(l & 1), a, c);
+      philox_gauss2(seed, ctr, bead, (uint32_t)(l & 1), a, c);
       if constexpr (L == 2 || L == 4) {
         gx = group_bcast<0, L>(a);
         gy = group_bcast<0, L>(c);
@@ -808,7 +817,6 @@ __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restric
   double Fy = (flags & F_ACCUM) ? D.fy[g] : 0.0;
   double Fz = (flags & F_ACCUM) ? D.fz[g] : 0.0;
   if (with_noise) {
-    const double var = Pp->var;
     Fx += var * gx;
     Fy += var * gy;
     Fz += var * gz;
@@ -820,8 +828,8 @@ __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restric
   // chunk of L lanes covers both (about two of each per bead) and the fold
   // still adds every bond before every angle, each in list order.
   if (flags & (F_BOND | F_ANGLE)) {
-    const int b0 = (flags & F_BOND) ? D.bond_beg[g] : 0, nbq = (flags & F_BOND) ? D.bond_end[g] - b0 : 0;
-    const int a0 = (flags & F_ANGLE) ? D.ang_beg[g] : 0, naq = (flags & F_ANGLE) ? D.ang_end[g] - a0 : 0;
+    const int b0 = (flags & F_BOND) ? bond0 : 0, nbq = (flags & F_BOND) ? bond1 - b0 : 0;
+    const int a0 = (flags & F_ANGLE) ? ang0 : 0, naq = (flags & F_ANGLE) ? ang1 - a0 : 0;
     const int ntot = nbq + naq;
     for (int base = 0; base < ntot; base += L) {
       const int q = base + l;
@@ -855,7 +863,7 @@ __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restric
   }
   // native LJ 12-6 over the pair list (energy.cpp:274-311): F_i += f, F_j -= f
   if (flags & F_VDW) {
-    const int p0 = D.att_beg[g], p1 = D.att_end[g];
+    const int p0 = att0, p1 = att1;
     // software pipeline as in the non-native loop below
     uint8_t fn = 0, fn2 = 0;
     int on2 = 0;
@@ -909,7 +917,7 @@ __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restric
   }
   // non-native repulsion over the pair list (energy.cpp:313-350): F_i += f, F_j -= f
   if (flags & F_VDW) {
-    const int cnt = D.cnt[g];
+    const int cnt = nslot;
     uint32_t* S = D.slot + (size_t)g * D.cap;
     const uint32_t rbit = 1u << (30 + pr), wbit = 1u << (30 + pw);
     const double thr_rep = T->thr_rep;
@@ -1020,9 +1028,7 @@ __global__ void __launch_bounds__(256, 5) k_force(Dev D, const Tables* __restric
   const int t = xcd_block() * blockDim.x + threadIdx.x;
   const int g = g0 + t / L;
   if (g >= g1) return;
-  const int chain = D.chain[g];
-  if (chain < 0 || D.cp[chain].resident) return;  // a free bead, or a chain the resident kernel advances
-  force_bead<L, NEXT>(D, T, tp, cur, pr, rel, flags, noise, g, t & (L - 1));
+  force_bead<L, NEXT, true>(D, T, tp, cur, pr, rel, flags, noise, g, t & (L - 1));
 }
 
 // Observables (sop.cpp:200-280) and list counts, per bead.
