@@ -737,7 +737,9 @@ enum { F_STORE = 1, F_VEL = 2, F_ENERGY = 4, F_ENERGY_ALL = 8, F_NOISE = 16, F_B
        F_VDW = 128, F_ACCUM = 256 };
 
 // Type-pair tables the pair loops read, staged in LDS (index it * 3 + jt).
-enum { TP_FREP = 0, TP_S6, TP_S12, TP_PLRC2, TP_CREP, TP_FATT, TP_CATT, TP_N };
+// Rows of 8 doubles (64 bytes): the non-native loop reads entries 0-3 of a
+// row as two 16-byte LDS loads.
+enum { TP_FREP = 0, TP_S6, TP_S12, TP_PLRC2, TP_CREP, TP_FATT, TP_CATT, TP_PAD, TP_N };
 
 __device__ __forceinline__ void load_type_tables(const Tables* __restrict__ T, double (*tp)[TP_N]) {
   if (threadIdx.x < 9) {
@@ -950,14 +952,16 @@ __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restric
         double dx, dy, dz;
         pair_d3(side, cx, cy, cz, px, py, pz, boxl, qbox, dx, dy, dz);
         const double d2 = dx * dx + dy * dy + dz * dz;
+        // the type pair's constants, read together (one LDS round trip)
+        const double frep = tp[tpi][TP_FREP], s6 = tp[tpi][TP_S6], s12 = tp[tpi][TP_S12];
+        const double plrc2 = tp[tpi][TP_PLRC2];
         if (w & rbit) {
-          const double s6 = tp[tpi][TP_S6], s12 = tp[tpi][TP_S12];
           const double d6 = d2 * d2 * d2;
           const double d12 = d6 * d6;
           const bool skip =
               (d2 < thr_rep) && rep_tol_skip(D.att_flag, D.am_q0, D.am_t, T, att_off, natt, pr, d2);
           if (!skip) {
-            const double co1 = tp[tpi][TP_FREP] / d2 * (2.0 * s12 / d12 + s6 / d6);
+            const double co1 = frep / d2 * (2.0 * s12 / d12 + s6 / d6);
             fx = co1 * dx; fy = co1 * dy; fz = co1 * dz;
             if (side == 1) { fx = -fx; fy = -fy; fz = -fz; }
             contrib = true;
@@ -965,7 +969,7 @@ __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restric
           if (energy && side == 0) { e = tp[tpi][TP_CREP] * (s12 / d12 + s6 / d6); ec = true; }
         }
         if constexpr (NEXT) {
-          const bool npl = no_cut || d2 < tp[tpi][TP_PLRC2];
+          const bool npl = no_cut || d2 < plrc2;
           const uint32_t nw = (w & ~wbit) | (npl ? wbit : 0u);
           if (nw != w) S[q] = nw;
         }
@@ -1022,7 +1026,7 @@ template <int L, bool NEXT>
 __global__ void __launch_bounds__(256, 5) k_force(Dev D, const Tables* __restrict__ T, int cur, int pr, int rel, int flags,
                                                const double* __restrict__ noise, int g0, int g1) {
   rel += *D.relb;
-  __shared__ double tp[9][TP_N];
+  __shared__ __align__(16) double tp[9][TP_N];
   load_type_tables(T, tp);
   __syncthreads();
   const int t = xcd_block() * blockDim.x + threadIdx.x;
@@ -1143,7 +1147,7 @@ template <int L>
 __global__ void __launch_bounds__(512) k_resident(Dev D, const Tables* __restrict__ T, const int* __restrict__ chains,
                                                    int cur, int par, int nsteps, const int* __restrict__ rec_slot,
                                                    double* rec, int nc) {
-  __shared__ double tp[9][TP_N];
+  __shared__ __align__(16) double tp[9][TP_N];
   __shared__ double sh[256];
   load_type_tables(T, tp);
   const int c = chains[blockIdx.x];
