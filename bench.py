@@ -386,7 +386,13 @@ def main():
                             f"steps log-uniform [1e4, 1e6] x {a.serve_step_scale:g}, rounded to a multiple of 1000; "
                             f"kT uniform [0.5, 1.3]; half unfolding (from native), half folding (from a SAW); "
                             f"neighbor list every 20 steps; Q/Rg/energy record every {NUP} steps",
-                     hbm_budget_gib=a.serve_budget_gib, chunk=a.serve_chunk)
+                     hbm_budget_gib=a.serve_budget_gib, chunk=a.serve_chunk,
+                     # saturated: work offered faster than the batched throughput of this
+                     # mixture (the server's capacity) can complete it
+                     capacity_estimate=value,
+                     saturated=bool(serve["offered_load"] > value),
+                     offered_load_at_full_step_counts=serve["offered_load"] / a.serve_step_scale,
+                     saturated_at_full_step_counts=bool(serve["offered_load"] / a.serve_step_scale > value))
         print(f"[bench] serve {serve['value']:.4g} bead-steps/s p50 {serve['p50_latency_s']:.3f} s "
               f"p99 {serve['p99_latency_s']:.3f} s", file=sys.stderr, flush=True)
     cpu = None if a.no_cpu else cpu_baseline(chains, w, a.cpu_workers)
