@@ -47,20 +47,20 @@ __device__ __forceinline__ double group_bcast(double v) {
 }
 
 // F += contributions of the group's lanes, in lane order (= list order).
-// For L = 2 and 4 every lane is added in turn, a non-contributing lane as
-// -0.0: x + (-0.0) is x for every x (both zeros included), so the sums are
-// the same bits as adding the contributing lanes only.
+// For L = 2 and 4 every lane is added in turn: callers leave -0.0 in the
+// values of a lane that does not contribute (c false), and x + (-0.0) is x
+// for every x (both zeros included), so the sums are the same bits as adding
+// the contributing lanes only.
 template <int L>
 __device__ __forceinline__ void fold3(double& Fx, double& Fy, double& Fz, bool c, double fx, double fy, double fz) {
   if constexpr (L == 1) {
     if (c) { Fx += fx; Fy += fy; Fz += fz; }
   } else if constexpr (L == 2 || L == 4) {
-    const double ax = c ? fx : -0.0, ay = c ? fy : -0.0, az = c ? fz : -0.0;
-    Fx += group_bcast<0, L>(ax); Fy += group_bcast<0, L>(ay); Fz += group_bcast<0, L>(az);
-    Fx += group_bcast<1, L>(ax); Fy += group_bcast<1, L>(ay); Fz += group_bcast<1, L>(az);
+    Fx += group_bcast<0, L>(fx); Fy += group_bcast<0, L>(fy); Fz += group_bcast<0, L>(fz);
+    Fx += group_bcast<1, L>(fx); Fy += group_bcast<1, L>(fy); Fz += group_bcast<1, L>(fz);
     if constexpr (L == 4) {
-      Fx += group_bcast<2, L>(ax); Fy += group_bcast<2, L>(ay); Fz += group_bcast<2, L>(az);
-      Fx += group_bcast<3, L>(ax); Fy += group_bcast<3, L>(ay); Fz += group_bcast<3, L>(az);
+      Fx += group_bcast<2, L>(fx); Fy += group_bcast<2, L>(fy); Fz += group_bcast<2, L>(fz);
+      Fx += group_bcast<3, L>(fx); Fy += group_bcast<3, L>(fy); Fz += group_bcast<3, L>(fz);
     }
   } else {
     uint64_t gm = group_mask<L>(c);
@@ -77,18 +77,18 @@ __device__ __forceinline__ void fold3(double& Fx, double& Fy, double& Fz, bool c
 }
 
 // fold3 for one value (energy terms): the same list order for every L, so
-// a bead's energy sums do not depend on the lane count either.
+// a bead's energy sums do not depend on the lane count either.  Same
+// convention: e is -0.0 in a lane that does not contribute.
 template <int L>
 __device__ __forceinline__ void fold1(double& E, bool c, double e) {
   if constexpr (L == 1) {
     if (c) E += e;
   } else if constexpr (L == 2 || L == 4) {
-    const double a = c ? e : -0.0;
-    E += group_bcast<0, L>(a);
-    E += group_bcast<1, L>(a);
+    E += group_bcast<0, L>(e);
+    E += group_bcast<1, L>(e);
     if constexpr (L == 4) {
-      E += group_bcast<2, L>(a);
-      E += group_bcast<3, L>(a);
+      E += group_bcast<2, L>(e);
+      E += group_bcast<3, L>(e);
     }
   } else {
     uint64_t gm = group_mask<L>(c);
@@ -836,7 +836,8 @@ __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restric
     for (int base = 0; base < ntot; base += L) {
       const int q = base + l;
       const bool has = q < ntot, isb = q < nbq;
-      double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
+      // -0.0: what a lane without a term adds in the folds
+      double fx = -0.0, fy = -0.0, fz = -0.0, e_b = -0.0, e_a = -0.0;
       bool ec = false;
       if (has) {
         const int pq = isb ? D.bond_pq[b0 + q] : D.ang_pq[a0 + q - nbq], o = pq >> 1, side = pq & 1;
@@ -848,18 +849,18 @@ __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restric
           const double dev = d - D.bond_r0[b0 + q];
           const double dev2 = dev * dev;
           co1 = -T->k_bnd * dev / d / (1.0 - dev2 / T->R0sq);
-          if (energy && side == 0) { e = log1p(-dev * dev / T->R0sq); ec = true; }
+          if (energy && side == 0) { e_b = log1p(-dev * dev / T->R0sq); ec = true; }
         } else {
           co1 = T->f_ang_ss_coeff / pow8_cr(d);
-          if (energy && side == 0) { e = T->e_ang_ss_coeff / pow6_cr(d); ec = true; }
+          if (energy && side == 0) { e_a = T->e_ang_ss_coeff / pow6_cr(d); ec = true; }
         }
         fx = co1 * dx; fy = co1 * dy; fz = co1 * dz;
         if (side == 0) { fx = -fx; fy = -fy; fz = -fz; }
       }
       fold3<L>(Fx, Fy, Fz, has, fx, fy, fz);
       if (energy) {
-        fold1<L>(eb, ec && isb, e);
-        fold1<L>(ea, ec && !isb, e);
+        fold1<L>(eb, ec && isb, e_b);
+        fold1<L>(ea, ec && !isb, e_a);
       }
     }
   }
@@ -884,7 +885,7 @@ __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restric
       fn = fn2;
       if (q + 2 * L < p1) { fn2 = D.att_flag[q + 2 * L]; on2 = D.att_partner[q + 2 * L]; }
       bool contrib = false, ec = false;
-      double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
+      double fx = -0.0, fy = -0.0, fz = -0.0, e = -0.0;  // -0.0: adds nothing in the folds
       if (q < p1) {
         if (f & 1) {
           const AttK K = D.att_k[q];
@@ -945,7 +946,7 @@ __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restric
       wn = wn2;
       if (q + 2 * L < cnt) wn2 = S[q + 2 * L];
       bool contrib = false, ec = false;
-      double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
+      double fx = -0.0, fy = -0.0, fz = -0.0, e = -0.0;  // -0.0: adds nothing in the folds
       if (q < cnt) {
         const int side = (w >> 25) & 1;
         const int tpi = (int)((w >> 26) & 3) * 3 + (int)((w >> 28) & 3);
