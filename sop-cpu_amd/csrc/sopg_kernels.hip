@@ -153,12 +153,17 @@ __device__ __forceinline__ double pair_d(int side, double c, double o, double bo
 }
 
 // All three components at once, with one branch for the common case in which
-// every component takes mimg's exact fast path (|d| <= boxl/4).
+// every component takes mimg's exact fast path (|d| <= boxl/4).  Side 1 uses
+// (o - c) * -1.0 for c - o: rounding to nearest is symmetric, so the two are
+// the same bits except for c == o, where they are -0 and +0; mimg maps both
+// zeros to +0 (d + 0.0 on the fast path; d - boxl*rnd(d/boxl) = +0 on the full
+// form), so the result is pair_d's, with one subtraction per component.
 __device__ __forceinline__ void pair_d3(int side, double cx, double cy, double cz, double ox, double oy, double oz,
                                         double boxl, double qbox, double& dx, double& dy, double& dz) {
-  const double rx = side ? cx - ox : ox - cx;
-  const double ry = side ? cy - oy : oy - cy;
-  const double rz = side ? cz - oz : oz - cz;
+  const double sg = side ? -1.0 : 1.0;
+  const double rx = (ox - cx) * sg;
+  const double ry = (oy - cy) * sg;
+  const double rz = (oz - cz) * sg;
   if (fabs(rx) <= qbox && fabs(ry) <= qbox && fabs(rz) <= qbox) {
     dx = rx + 0.0; dy = ry + 0.0; dz = rz + 0.0;
   } else {
