@@ -60,13 +60,13 @@ NUP = 1000
 
 WORKLOADS = {
     "mixed256": dict(desc="256 chains, N log-uniform in [50, 5000] (seed 4), each its own native structure; "
-                          "neighbor list every 20 steps; kT 0.6, h 0.05, zeta 50; Philox noise",
+                          "neighbor list every 20 steps; kT {kT}, h 0.05, zeta 50; Philox noise",
                      cutoff=sopg.CUTOFF_NEIGHBOR, nnlup=20, boxl=500.0, ncell=55.0),
     "plain3000": dict(desc="one chain N=3000; cell list 12 A (boxl 504, ncell 42) rebuilt every 20 steps; "
-                           "kT 0.6, h 0.05, zeta 50; Philox noise",
+                           "kT {kT}, h 0.05, zeta 50; Philox noise",
                       cutoff=sopg.CUTOFF_CELL, nnlup=20, boxl=504.0, ncell=42.0),
     "replicas1024": dict(desc="1024 replicas of one N=76 chain, own seeds and Maxwell velocities; "
-                              "neighbor list every 50 steps; kT 0.6, h 0.05, zeta 50; Philox noise",
+                              "neighbor list every 50 steps; kT {kT}, h 0.05, zeta 50; Philox noise",
                          cutoff=sopg.CUTOFF_NEIGHBOR, nnlup=50, boxl=500.0, ncell=55.0),
 }
 
@@ -80,15 +80,15 @@ def chain_sizes(workload):
     return [76] * 1024
 
 
-def make_chains(workload, threads):
+def make_chains(workload, threads, kt=KT):
     sizes = chain_sizes(workload)
     if workload == "replicas1024":
         x = sopg.gen_native(76, 76)
-        return [(x, sopg.gen_maxwell(76, KT, 10_000 + r), 10_000 + r) for r in range(len(sizes))]
+        return [(x, sopg.gen_maxwell(76, kt, 10_000 + r), 10_000 + r) for r in range(len(sizes))]
 
     def one(c):
         n = sizes[c]
-        return (sopg.gen_native(n, 1000 + c), sopg.gen_maxwell(n, KT, 1000 + c), 1000 + c)
+        return (sopg.gen_native(n, 1000 + c), sopg.gen_maxwell(n, kt, 1000 + c), 1000 + c)
     with ThreadPoolExecutor(max_workers=threads) as ex:
         return list(ex.map(one, range(len(sizes))))
 
@@ -104,7 +104,7 @@ def build_ctx(chains, w, lanes=0, list_lanes=0, topo_cache=None):
             if topo_cache is not None:
                 topo_cache[key] = t
         topos.append(t)
-        ctx.add_chain(t, sopg.chain_params(T=KT, boxl=w["boxl"], ncell=w["ncell"], cutoff=w["cutoff"],
+        ctx.add_chain(t, sopg.chain_params(T=w.get("kT", KT), boxl=w["boxl"], ncell=w["ncell"], cutoff=w["cutoff"],
                                            nnlup=w["nnlup"], nup=NUP, noise_mode=sopg.NOISE_PHILOX, seed=seed))
     for c, (x, v, seed) in enumerate(chains):
         ctx.put_state(c, pos=x, unc=x, vel=v, force=np.zeros_like(x))
@@ -241,7 +241,7 @@ def _cpu_worker(job):
     """One reference process: set up one chain from its files and run the
     underdamped_ctrl loop body (sop.cpp:153-189) until the time budget is
     spent (at least one rebuild period).  Returns (N, steps, seconds)."""
-    d, n, v, cut, nnlup, extra, budget = job
+    d, n, v, cut, nnlup, extra, budget, kt = job
     # the reference reports its loading on stdout; keep this process's stdout
     # (shared with the benchmark's single JSON line) clean
     devnull = os.open(os.devnull, os.O_WRONLY)
@@ -249,7 +249,7 @@ def _cpu_worker(job):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from refs import Ref, setup_system, std_cmds
     r = Ref()
-    setup_system(r, d, std_cmds(temp=KT, cutoff=cut, nnlup=nnlup, nup=NUP, extra=extra))
+    setup_system(r, d, std_cmds(temp=kt, cutoff=cut, nnlup=nnlup, nup=NUP, extra=extra))
     r.set_coords(2, v)
     rebuild = "update_cell_list" if cut == "celllist" else "update_neighbor_list"
     r.call(rebuild)
@@ -289,7 +289,8 @@ def cpu_baseline(chains, w, workers, budget_s=15.0, max_n=3200):
         jobs = []
         for c in picks:
             x, v, _ = chains[c]
-            jobs.append((os.path.join(tmp, f"c{c}"), len(x) // 3, v, cut, w["nnlup"], extra, budget_s))
+            jobs.append((os.path.join(tmp, f"c{c}"), len(x) // 3, v, cut, w["nnlup"], extra, budget_s,
+                         w.get("kT", KT)))
         with ThreadPoolExecutor(max_workers=len(jobs)) as ex:
             list(ex.map(lambda jc: sopg.write_inputs(chains[jc[1]][0], jc[0][0], vel=chains[jc[1]][1]),
                         zip(jobs, picks)))
@@ -330,6 +331,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--workload", default="mixed256", choices=sorted(WORKLOADS))
+    ap.add_argument("--kt", type=float, default=KT, help="temperature (kcal/mol) of the workload's chains")
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--list-lanes", type=int, default=0)
     ap.add_argument("--plain-steps", type=int, default=100)
@@ -356,9 +358,10 @@ def main():
         print(f"bench.py: this build runs on one GPU only (independent chains shard across GPUs with no "
               f"per-step exchange); --gpus {a.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
-    w = WORKLOADS[a.workload]
+    w = dict(WORKLOADS[a.workload], kT=a.kt)
+    w["desc"] = w["desc"].format(kT=f"{a.kt:g}")
     t0 = time.perf_counter()
-    chains = make_chains(a.workload, a.threads)
+    chains = make_chains(a.workload, a.threads, a.kt)
     print(f"[bench] generated {len(chains)} chains in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     sizes = [len(x) // 3 for x, _, _ in chains]
     nb = sum(sizes)

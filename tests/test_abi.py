@@ -83,3 +83,11 @@ def test_written_files_roundtrip(tmp_path):
     assert np.array_equal(ai, t["att_i"]) and np.array_equal(aj, t["att_j"])
     assert np.array_equal(o.coords(1), x)
     assert o.geti("ncon_rep") == (60 - 2) * (60 - 3) // 2 - natt
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; asking for more exits with a message before
+    any device work."""
+    r = subprocess.run(["python3", os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2 and "one GPU only" in r.stderr and r.stdout == ""
