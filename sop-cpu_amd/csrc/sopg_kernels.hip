@@ -145,19 +145,15 @@ __device__ __noinline__ bool rep_tol_skip(const uint8_t* flag, const int* __rest
   return false;
 }
 
-// Pair displacement u[j] - u[i] seen from bead g: c is g's own coordinate, o
-// the partner's.  side 0 (g is i): o - c; side 1 (g is j): c - o -- the same
-// subtraction the reference performs, so the same bits.
-__device__ __forceinline__ double pair_d(int side, double c, double o, double boxl, double qbox) {
-  return mimg(side ? c - o : o - c, boxl, qbox);
-}
-
-// All three components at once, with one branch for the common case in which
-// every component takes mimg's exact fast path (|d| <= boxl/4).  Side 1 uses
-// (o - c) * -1.0 for c - o: rounding to nearest is symmetric, so the two are
-// the same bits except for c == o, where they are -0 and +0; mimg maps both
-// zeros to +0 (d + 0.0 on the fast path; d - boxl*rnd(d/boxl) = +0 on the full
-// form), so the result is pair_d's, with one subtraction per component.
+// Pair displacement u[j] - u[i] seen from bead g, min-imaged, all three
+// components at once: c is g's own position, o the partner's.  The reference
+// computes o - c when g is the pair's i bead (side 0) and c - o when it is j
+// (side 1).  Side 1 here uses (o - c) * -1.0: rounding to nearest is
+// symmetric, so the two are the same bits except for c == o, where they are
+// -0 and +0; mimg maps both zeros to +0 (d + 0.0 on the fast path; d -
+// boxl*rnd(d/boxl) = +0 on the full form), so the result is the reference's,
+// with one subtraction per component.  One branch covers the common case in
+// which every component takes mimg's exact fast path (|d| <= boxl/4).
 __device__ __forceinline__ void pair_d3(int side, double cx, double cy, double cz, double ox, double oy, double oz,
                                         double boxl, double qbox, double& dx, double& dy, double& dz) {
   const double sg = side ? -1.0 : 1.0;
