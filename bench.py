@@ -316,13 +316,15 @@ def profiled_traffic(workload):
         return None
     vals = {}
     for line in open(path):
-        if ":" in line and line.split(":")[0] in ("traffic_bytes_per_launch", "avg_launch_us", "valu_issue_frac"):
+        if ":" in line and line.split(":")[0] in ("traffic_bytes_per_launch", "avg_launch_us", "valu_issue_frac",
+                                                  "fp64_tflops"):
             k, v = line.split(":", 1)
             vals[k] = float(v)
     if "traffic_bytes_per_launch" not in vals:
         return None
     return dict(bytes=vals["traffic_bytes_per_launch"], avg_launch_us=vals.get("avg_launch_us"),
-                valu_issue_frac=vals.get("valu_issue_frac"), source=os.path.relpath(path, ROOT))
+                valu_issue_frac=vals.get("valu_issue_frac"), fp64_tflops=vals.get("fp64_tflops"),
+                source=os.path.relpath(path, ROOT))
 
 
 def main():
@@ -436,6 +438,8 @@ def main():
                      f"{prof_t['source']} (rocprofv3 FETCH_SIZE + WRITE_SIZE per k_force launch, scaled by a "
                      f"1 GiB copy; that profile's k_force average {prof_t['avg_launch_us']} us)",
                      "valu_issue_frac": None if prof_t is None else prof_t["valu_issue_frac"],
+                     "fp64_tflops_profiled": None if prof_t is None else prof_t["fp64_tflops"],
+                     "fp64_peak_tflops": 78.6,
                      "bytes_per_launch": g["bytes_step"], "avg_launch_us": avg_s * 1e6,
                      "launches": step_n, "rebuild_avg_us": reb_ms / max(reb_n, 1) * 1e3,
                      "rebuilds": reb_n, "observe_avg_us": obs_ms / max(obs_n, 1) * 1e3},
