@@ -50,13 +50,15 @@ __device__ __forceinline__ double group_bcast(double v) {
 // For L = 2 and 4 every lane is added in turn: callers leave -0.0 in the
 // values of a lane that does not contribute (c false), and x + (-0.0) is x
 // for every x (both zeros included), so the sums are the same bits as adding
-// the contributing lanes only.
+// the contributing lanes only.  There only lane 0's F is the bead's (lane 0
+// adds its own value first, then lanes 1.. in turn); the other lanes' F is
+// not used.
 template <int L>
 __device__ __forceinline__ void fold3(double& Fx, double& Fy, double& Fz, bool c, double fx, double fy, double fz) {
   if constexpr (L == 1) {
     if (c) { Fx += fx; Fy += fy; Fz += fz; }
   } else if constexpr (L == 2 || L == 4) {
-    Fx += group_bcast<0, L>(fx); Fy += group_bcast<0, L>(fy); Fz += group_bcast<0, L>(fz);
+    Fx += fx; Fy += fy; Fz += fz;
     Fx += group_bcast<1, L>(fx); Fy += group_bcast<1, L>(fy); Fz += group_bcast<1, L>(fz);
     if constexpr (L == 4) {
       Fx += group_bcast<2, L>(fx); Fy += group_bcast<2, L>(fy); Fz += group_bcast<2, L>(fz);
@@ -78,13 +80,14 @@ __device__ __forceinline__ void fold3(double& Fx, double& Fy, double& Fz, bool c
 
 // fold3 for one value (energy terms): the same list order for every L, so
 // a bead's energy sums do not depend on the lane count either.  Same
-// convention: e is -0.0 in a lane that does not contribute.
+// conventions: e is -0.0 in a lane that does not contribute, and at L = 2, 4
+// only lane 0's E is the bead's.
 template <int L>
 __device__ __forceinline__ void fold1(double& E, bool c, double e) {
   if constexpr (L == 1) {
     if (c) E += e;
   } else if constexpr (L == 2 || L == 4) {
-    E += group_bcast<0, L>(e);
+    E += e;
     E += group_bcast<1, L>(e);
     if constexpr (L == 4) {
       E += group_bcast<2, L>(e);
