@@ -203,15 +203,18 @@ __global__ void k_cellcoords(Dev D, int cur, int g0, int g1) {
 template <int L, bool RES = false>
 __device__ __forceinline__ void rebuild_scan_bead(const Dev& D, const Tables* __restrict__ T, int cur, int par,
                                                   int rel, int mode, int g, int l) {
-  if (D.chain[g] < 0) return;  // a free bead
-  const ChainP& P = D.cp[D.chain[g]];
+  const int c = D.chain[g];
+  if (c < 0) return;  // a free bead
+  // binned chains first (one load less for the common case): a chain whose
+  // grid is in use is either rebuilt by k_rebuild_grid now or not due
+  if (!RES && D.grid[c].use) return;
+  const ChainP& P = D.cp[c];
   int crit = mode;
   if (mode < 0) {
     if (P.cutoff == SOPG_CUTOFF_NONE || (P.phase + rel) % P.nnlup != 0) return;
     if (!RES && P.resident) return;
     crit = P.cutoff;
   }
-  if (!RES && D.grid[D.chain[g]].use) return;  // rebuilt by k_rebuild_grid
   const double* ux = cur ? D.ux[1] : D.ux[0];
   const double* uy = cur ? D.uy[1] : D.uy[0];
   const double* uz = cur ? D.uz[1] : D.uz[0];
