@@ -442,7 +442,7 @@ def main():
                      "fp64_peak_tflops": 78.6,
                      "bytes_per_launch": g["bytes_step"], "avg_launch_us": avg_s * 1e6,
                      "launches": step_n, "rebuild_avg_us": reb_ms / max(reb_n, 1) * 1e3,
-                     "rebuilds": reb_n, "observe_avg_us": obs_ms / max(obs_n, 1) * 1e3},
+                     "rebuilds": reb_n, "observe_avg_us": obs_ms / obs_n * 1e3 if obs_n else None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
