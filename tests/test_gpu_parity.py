@@ -609,3 +609,33 @@ def test_empty_runs(inputs_root):
     ctx.remove_chain(0)
     assert ctx.nchains() == 0 and ctx.run(5) == []
     ctx.close()
+
+
+def test_resident_kernel_overdamped(inputs_root):
+    """Overdamped chains (sim_type 2, first rebuild at step 1) in the resident
+    kernel give the per-step path's bits and records."""
+    chains = []
+    for n, seed in ((64, 141), (100, 142)):
+        x, v, _ = chain_inputs(inputs_root, n, seed)
+        chains.append((x, v, 20 + seed))
+    results = []
+    for resident in (0, 128):
+        ctx = sopg.Context()
+        for (x, v, seed) in chains:
+            ctx.add_chain(sopg.chain_topology(x),
+                          sopg.chain_params(T=0.9, cutoff=sopg.CUTOFF_NEIGHBOR, nnlup=10, nup=40,
+                                            noise_mode=sopg.NOISE_PHILOX, seed=seed, sim_type=2))
+        ctx.set_resident(resident)
+        for c, (x, v, seed) in enumerate(chains):
+            ctx.put_state(c, pos=x, unc=x, vel=np.zeros_like(x), force=np.zeros_like(x))
+            ctx.set_step(c, 1.0, 0, 1)  # overdamped_ctrl: inlup = 0, the list rebuilt on the first step
+            ctx.force_eval(c)
+        print(f"[gpu] overdamped resident={resident}", flush=True)
+        recs = ctx.run(157) + ctx.run(43)
+        results.append(([ctx.get_state(c) for c in range(len(chains))], recs))
+        ctx.close()
+    (sa, ra), (sb, rb) = results
+    for a, b in zip(sa, sb):
+        for k in ("unc", "vel", "force", "pos"):
+            assert np.array_equal(a[k], b[k]), k
+    assert ra == rb and len(ra) == 2 * (200 // 40)
