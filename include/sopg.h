@@ -199,6 +199,13 @@ int sopg_restore(sopg_ctx* ctx, int chain, const void* buf, long size);
  * lanes per bead in it (0: automatic); max_beads * lanes <= 512.  Results do
  * not depend on it.  No counterpart in the reference. */
 int sopg_set_resident(sopg_ctx* ctx, int max_beads, int lanes);
+/* Skin of the candidate lists of neighbor-list rebuilds during sopg_run, in
+ * Angstrom (default 3; 0: off).  A candidate list holds a bead's non-native
+ * pairs within 3.2 sigma + skin of the positions it was built at; while no
+ * bead of the chain has moved more than skin/2 from them, a rebuild filters
+ * the candidates instead of scanning the grid, otherwise it rebuilds them.
+ * Lists and results do not depend on it.  No counterpart in the reference. */
+int sopg_set_skin(sopg_ctx* ctx, double skin);
 /* sopg_run replays repeating step segments (one list-rebuild period, no
  * record, not the run's last step) from captured HIP graphs instead of
  * launching their kernels one by one.  Results do not depend on it (default:

@@ -26,7 +26,7 @@
 namespace sopg {
 void launch_cellcoords(const Dev& D, int cur, int g0, int g1, hipStream_t s);
 void launch_rebuild(int L, const Dev& D, const Tables* T, int cur, int par, int rel, int mode, int g0, int g1,
-                    int c0, int nc, int binned, hipStream_t s);
+                    int c0, int nc, int binned, int cand, hipStream_t s);
 void launch_pairfilter(int L, const Dev& D, const Tables* T, int cur, int par, int g0, int g1, hipStream_t s);
 void launch_force(int L, const Dev& D, const Tables* T, int cur, int pr, int rel, int flags, bool next,
                   const double* noise, int g0, int g1, hipStream_t s);
@@ -128,6 +128,7 @@ struct sopg_ctx {
   int list_lanes = 64;  // lanes per bead, full-scan list rebuild
   int binned = 1;       // binned list rebuild where a chain's extent allows it
   int cap_user = 0;
+  double skin = 3.0;    // candidate lists of neighbor-list rebuilds (Angstrom; <= 0: off)
   size_t reserve_beads = 0, reserve_chains = 0;
   Tables* d_T = nullptr;
   std::vector<HostChain> chains;
@@ -291,13 +292,15 @@ void register_arrays(sopg_ctx* c) {
     add(P_BEAD, &D.ux[s], 8, 1); add(P_BEAD, &D.uy[s], 8, 1); add(P_BEAD, &D.uz[s], 8, 1);
     add(P_BEAD, &D.px[s], 8, 1); add(P_BEAD, &D.py[s], 8, 1); add(P_BEAD, &D.pz[s], 8, 1);
   }
-  for (double** p : {&D.vx, &D.vy, &D.vz, &D.fx, &D.fy, &D.fz, &D.ix, &D.iy, &D.iz, &D.cx, &D.cy, &D.cz})
+  for (double** p : {&D.vx, &D.vy, &D.vz, &D.fx, &D.fy, &D.fz, &D.ix, &D.iy, &D.iz, &D.cx, &D.cy, &D.cz, &D.u0x,
+                     &D.u0y, &D.u0z})
     add(P_BEAD, p, 8, 1);
   for (int** p : {&D.chain, &D.bond_beg, &D.bond_end, &D.ang_beg, &D.ang_end, &D.att_beg, &D.att_end, &D.rep_beg,
-                  &D.rep_end, &D.cnt, &D.bead_cell, &D.cell_beads})
+                  &D.rep_end, &D.cnt, &D.ccnt, &D.bead_cell, &D.cell_beads})
     add(P_BEAD, p, 4, 1);
   add(P_BEAD, &D.type, 1, 1);
   add(P_BEAD, &D.slot, 4, cap);
+  add(P_BEAD, &D.cand, 4, cap);
   add(P_BEAD, &D.slot_k, 4, cap);
   add(P_BEAD, &D.red, 8, kRed);
   add(P_BINC, &D.bond_pq, 4, 1); add(P_BINC, &D.bond_r0, 8, 1);
@@ -319,7 +322,8 @@ void register_arrays(sopg_ctx* c) {
 
 // ------------------------------------------------------------ parameters
 
-void derive_tables(const sopg_tables& t, Tables& T) {
+// skin <= 0: no candidate lists
+void derive_tables(const sopg_tables& t, double skin, Tables& T) {
   double thr = 0.0;
   for (int i = 0; i < 3; i++)
     for (int j = 0; j < 3; j++) {
@@ -336,8 +340,16 @@ void derive_tables(const sopg_tables& t, Tables& T) {
       const double rn = 3.2 * s, rp = 2.5 * s;
       T.nlrc2[i][j] = rn * rn;
       T.plrc2[i][j] = rp * rp;
+      const double rc = rn + skin;
+      T.candc2[i][j] = rc * rc;
       thr = std::max(thr, T.s2[i][j] * kTol);
     }
+  // A pair of the neighbor list at positions P (d < 3.2 sigma) was within
+  // 3.2 sigma + skin of u0 if neither bead has moved more than skin/2 since;
+  // the relative margin is far above the rounding of any distance here.
+  T.cand_on = skin > 0.0 ? 1 : 0;
+  const double half = 0.5 * skin * (1.0 - 1e-6);
+  T.cand_disp2 = skin > 0.0 ? half * half : -1.0;
   T.k_bnd = t.k_bnd;
   T.R0sq = t.R0sq;
   T.e_bnd_coeff = t.e_bnd_coeff;
@@ -403,6 +415,11 @@ void fill_chainp(const HostChain& h, const Tables& T, ChainP& P) {
     for (int b = 0; b < 3; b++)
       if (present[a] && present[b]) rc2 = std::max(rc2, T.nlrc2[a][b]);
   P.gcell = std::sqrt(rc2) * 1.001 + 1e-9;
+  double cc2 = 0.0;
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++)
+      if (present[a] && present[b]) cc2 = std::max(cc2, T.candc2[a][b]);
+  P.gcand = std::sqrt(cc2) * 1.001 + 1e-9;
   P.resident = h.resident ? 1 : 0;
 }
 
@@ -569,7 +586,7 @@ int materialize(sopg_ctx* c, int id) {
   // No-cutoff chains interact through their full master lists, as the loader
   // leaves them (params.cpp:210-230): build that list once.
   if (h.p.cutoff == SOPG_CUTOFF_NONE) {
-    launch_rebuild(c->list_lanes, D, c->d_T, c->cur, c->par, 0, SOPG_CUTOFF_NONE, off, off + n, id, 1, 0, c->stream);
+    launch_rebuild(c->list_lanes, D, c->d_T, c->cur, c->par, 0, SOPG_CUTOFF_NONE, off, off + n, id, 1, 0, 0, c->stream);
     HIP_OK(c, hipGetLastError());
     return check_err_flag(c);
   }
@@ -686,7 +703,7 @@ int rebuild_one(sopg_ctx* c, int chain, int mode, int* nnl_att, int* nnl_rep) {
   if (mode == SOPG_CUTOFF_CELL && (!(h->p.ncell > 0.0) || !(h->p.lcell > 0.0)))
     return fail(c, SOPG_ERR_ARG, "cell list needs ncell > 0 and lcell > 0");
   if (mode == SOPG_CUTOFF_CELL) launch_cellcoords(c->D, c->cur, P->off, P->off + P->n, c->stream);
-  launch_rebuild(c->list_lanes, c->D, c->d_T, c->cur, c->par, 0, mode, P->off, P->off + P->n, chain, 1, c->binned,
+  launch_rebuild(c->list_lanes, c->D, c->d_T, c->cur, c->par, 0, mode, P->off, P->off + P->n, chain, 1, c->binned, 0,
                  c->stream);
   HIP_OK(c, hipGetLastError());
   if ((rc = check_err_flag(c))) return rc;
@@ -798,6 +815,22 @@ int parse_topology(sopg_ctx* c, const sopg_topology* t, HostChain& h) {
 
 // =================================================================== C ABI
 
+namespace {
+
+// Candidate lists built with other cutoffs or another skin are rebuilt at
+// each chain's next neighbor-list rebuild.
+int invalidate_candidates(sopg_ctx* c) {
+  if (!c->device_ready) return SOPG_OK;
+  const int zero = 0;
+  for (int id = 0; id < (int)c->chains.size(); id++)
+    if (c->chains[id].active && c->chains[id].on_device)
+      HIP_OK(c, hipMemcpyAsync(&c->D.grid[id].cand_valid, &zero, sizeof zero, hipMemcpyHostToDevice, c->stream));
+  HIP_OK(c, hipStreamSynchronize(c->stream));
+  return SOPG_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int sopg_create(int device, sopg_ctx** out) {
@@ -814,7 +847,7 @@ int sopg_create(int device, sopg_ctx** out) {
     return SOPG_ERR_HIP;
   }
   sopg_default_tables(&c->tab);
-  derive_tables(c->tab, c->T);
+  derive_tables(c->tab, c->skin, c->T);
   *out = c;
   return SOPG_OK;
 }
@@ -867,11 +900,13 @@ int sopg_default_tables(sopg_tables* t) {
 int sopg_set_tables(sopg_ctx* c, const sopg_tables* t) {
   if (!c || !t) return SOPG_ERR_ARG;
   c->tab = *t;
-  derive_tables(c->tab, c->T);
+  derive_tables(c->tab, c->skin, c->T);
   if (c->device_ready) {
     HIP_OK(c, hipMemcpyAsync(c->d_T, &c->T, sizeof(Tables), hipMemcpyHostToDevice, c->stream));
     HIP_OK(c, hipStreamSynchronize(c->stream));
-    return upload_chainp(c);
+    int rc;
+    if ((rc = upload_chainp(c))) return rc;
+    return invalidate_candidates(c);
   }
   return SOPG_OK;
 }
@@ -1415,6 +1450,20 @@ int sopg_set_resident(sopg_ctx* c, int max_beads, int lanes) {
   return SOPG_OK;
 }
 
+int sopg_set_skin(sopg_ctx* c, double skin) {
+  if (!c) return SOPG_ERR_ARG;
+  if (!(skin >= 0.0) || skin > 100.0) return fail(c, SOPG_ERR_ARG, "skin must be in [0, 100] Angstrom");
+  c->skin = skin;
+  derive_tables(c->tab, c->skin, c->T);
+  if (c->device_ready) {
+    HIP_OK(c, hipMemcpyAsync(c->d_T, &c->T, sizeof(Tables), hipMemcpyHostToDevice, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    int rc;
+    if ((rc = upload_chainp(c)) || (rc = invalidate_candidates(c))) return rc;
+  }
+  return SOPG_OK;
+}
+
 int sopg_set_graphs(sopg_ctx* c, int enable) {
   if (!c) return SOPG_ERR_ARG;
   c->graphs = enable != 0;
@@ -1596,7 +1645,7 @@ int sopg_run(sopg_ctx* c, long nsteps, const double* noise, sopg_record* rec, lo
   } else if (dr.first) {
     pe = prof_begin(c);
     if (dr.second) launch_cellcoords(D, c->cur, 0, nb, s);
-    launch_rebuild(c->list_lanes, D, c->d_T, c->cur, c->par, 0, -1, 0, nb, 0, nc, c->binned, s);
+    launch_rebuild(c->list_lanes, D, c->d_T, c->cur, c->par, 0, -1, 0, nb, 0, nc, c->binned, c->skin > 0.0, s);
     prof_end(c, SOPG_KERNEL_REBUILD, pe);
   }
   // pair lists at the current positions; rerunning the filter on a chain whose
@@ -1631,7 +1680,8 @@ int sopg_run(sopg_ctx* c, long nsteps, const double* noise, sopg_record* rec, lo
       if (d.first) {
         pe = prof_begin(c);
         if (d.second) launch_cellcoords(Dx, c->cur, 0, nb, s);
-        launch_rebuild(c->list_lanes, Dx, c->d_T, c->cur, c->par, k + 1, -1, 0, nb, 0, nc, c->binned, s);
+        launch_rebuild(c->list_lanes, Dx, c->d_T, c->cur, c->par, k + 1, -1, 0, nb, 0, nc, c->binned, c->skin > 0.0,
+                       s);
         prof_end(c, SOPG_KERNEL_REBUILD, pe);
       }
       c->cur ^= 1;

@@ -24,6 +24,14 @@ struct Tables {
   double nlrc2[3][3], plrc2[3][3];  // (3.2 sigma_rep)^2, (2.5 sigma_rep)^2
   double k_bnd, R0sq, e_bnd_coeff, e_ang_ss_coeff, f_ang_ss_coeff;
   double thr_rep;                   // max over types of sigma_rep2 * tol
+  // Candidate lists (neighbor-list criterion, runs only): the non-native
+  // pairs within 3.2 sigma + skin of the positions u0 of their build.  While
+  // no bead of the chain has moved more than skin/2 from u0, every pair of
+  // the neighbor list is among them, and a rebuild filters them instead of
+  // scanning the grid.
+  int cand_on;                      // skin > 0
+  double candc2[3][3];              // (3.2 sigma_rep + skin)^2
+  double cand_disp2;                // (skin/2)^2, less a margin far above rounding
 };
 
 struct ChainP {
@@ -41,6 +49,7 @@ struct ChainP {
   uint64_t seed, noise_base;  // Philox key, force evaluations done so far
   int sim_type;        // 1 underdamped, 2 overdamped
   double gcell;        // binned neighbor-list rebuild: grid edge >= largest 3.2 sigma_rep of the chain
+  double gcand;        // the same for the candidate-list build: >= largest 3.2 sigma_rep + skin
   int resident;        // 1 while sopg_run advances the chain in the resident kernel (the
                        // per-step kernels then skip it)
 };
@@ -51,11 +60,19 @@ struct ChainP {
 // list): then every pair the criterion keeps lies in adjacent grid cells.
 constexpr int kGridDim = 24;                       // cells per axis, at most
 constexpr int kGridCells = kGridDim * kGridDim * kGridDim;
+// What a list rebuild does for the chain (GridP::work, set by k_grid for every
+// chain of the launch, 0 for a chain not due).
+enum { W_BIN = 1,      // beads binned into the grid, k_rebuild_grid runs
+       W_CAND = 2,     // ... and writes the candidate list instead of the neighbor list
+       W_FILTER = 4,   // the neighbor list is filtered from the candidate list
+       W_SCAN = 8 };   // the full scan of k_rebuild
 struct GridP {
   double ox, oy, oz;   // origin (bead coordinate, or cell index for the cell list)
   double inv;          // 1 / grid cell edge (1 for the cell list: cells are the reference's)
   int nx, ny, nz;
-  int use;             // 1: binned rebuild, 0: full candidate scan
+  int work;            // W_* of the current rebuild
+  int cand_valid;      // the candidate list holds the pairs within 3.2 sigma + skin of u0
+  int pad;
 };
 
 // Constants of one native-contact incidence that the step kernel reads, in
@@ -97,6 +114,10 @@ struct Dev {
   int *rep_beg, *rep_end, *rep_code, *rm_i, *rm_j; int8_t* rm_t;
   // non-native neighbor slots
   uint32_t* slot; int* slot_k; int* cnt;
+  // candidate lists (cap words per bead, slot format without pair-list bits)
+  // and the positions they were built at
+  uint32_t* cand; int* ccnt;
+  double *u0x, *u0y, *u0z;
   double *cx, *cy, *cz;  // cell coordinates (cell list)
   double* red;           // per-bead reduction fields
   int* err;              // bit0: slot overflow

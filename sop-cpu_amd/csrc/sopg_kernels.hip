@@ -205,9 +205,8 @@ __device__ __forceinline__ void rebuild_scan_bead(const Dev& D, const Tables* __
                                                   int rel, int mode, int g, int l) {
   const int c = D.chain[g];
   if (c < 0) return;  // a free bead
-  // binned chains first (one load less for the common case): a chain whose
-  // grid is in use is either rebuilt by k_rebuild_grid now or not due
-  if (!RES && D.grid[c].use) return;
+  // the plan of k_grid (due chains only): the full scan, or another path
+  if (!RES && !(D.grid[c].work & W_SCAN)) return;
   const ChainP& P = D.cp[c];
   int crit = mode;
   if (mode < 0) {
@@ -339,30 +338,69 @@ __device__ __forceinline__ bool rebuild_due(const ChainP& P, int rel, int mode, 
   return P.cutoff != SOPG_CUTOFF_NONE && !P.resident && (P.phase + rel) % P.nnlup == 0;
 }
 
-// Per-chain extent and grid; one block per chain.  Chains of more than maxn
-// beads (whose LDS bitmap would not fit) keep the full scan.
-__global__ void __launch_bounds__(256) k_grid(Dev D, int cur, int rel, int mode, int c0, int maxn) {
+// Per-chain plan of a list rebuild (GridP::work) and, for binned chains, the
+// extent and grid; one block per chain, every chain of the launch.  Chains of
+// more than maxn beads (whose LDS bitmap would not fit) keep the full scan.
+// Neighbor-list chains in a run use candidate lists: while the chain's
+// largest displacement from u0 stays within half the skin, the neighbor list
+// is filtered from the candidate list; otherwise the candidate list is
+// rebuilt (binned, cells >= 3.2 sigma + skin) at the current positions.
+__global__ void __launch_bounds__(256) k_grid(Dev D, const Tables* __restrict__ T, int cur, int rel, int mode,
+                                              int c0, int maxn) {
   rel += *D.relb;
   const int c = c0 + blockIdx.x;
   const ChainP& P = D.cp[c];
   GridP& G = D.grid[c];
   int crit;
-  if (!rebuild_due(P, rel, mode, crit)) return;
-  if (P.n > maxn || P.rep_explicit || crit == SOPG_CUTOFF_NONE) {
-    if (threadIdx.x == 0) G.use = 0;
+  if (!rebuild_due(P, rel, mode, crit)) {
+    if (threadIdx.x == 0) G.work = 0;
     return;
   }
+  if (P.n > maxn || P.rep_explicit || crit == SOPG_CUTOFF_NONE) {
+    if (threadIdx.x == 0) G.work = W_SCAN;
+    return;
+  }
+  const bool cand = mode < 0 && crit == SOPG_CUTOFF_NEIGHBOR && T->cand_on;
+  const double* px = cur ? D.ux[1] : D.ux[0];
+  const double* py = cur ? D.uy[1] : D.uy[0];
+  const double* pz = cur ? D.uz[1] : D.uz[0];
+  __shared__ double sh[6][256];
+  if (cand && G.cand_valid) {
+    // largest squared displacement from the candidate lists' positions
+    double dmax = 0.0;
+    for (int b = threadIdx.x; b < P.n; b += blockDim.x) {
+      const int g = P.off + b;
+      const double ex = px[g] - D.u0x[g], ey = py[g] - D.u0y[g], ez = pz[g] - D.u0z[g];
+      const double dd = ex * ex + ey * ey + ez * ez;
+      dmax = (dd > dmax || dd != dd) ? dd : dmax;  // NaN propagates (a rebuild)
+    }
+    sh[0][threadIdx.x] = dmax;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) {
+        const double o = sh[0][threadIdx.x + w];
+        if (o > sh[0][threadIdx.x] || o != o) sh[0][threadIdx.x] = o;
+      }
+      __syncthreads();
+    }
+    const bool keep = sh[0][0] <= T->cand_disp2;  // false for NaN
+    __syncthreads();
+    if (keep) {
+      if (threadIdx.x == 0) G.work = W_FILTER;
+      return;
+    }
+  }
   const bool cell = crit == SOPG_CUTOFF_CELL;
-  const double* ax = cell ? D.cx : (cur ? D.ux[1] : D.ux[0]);
-  const double* ay = cell ? D.cy : (cur ? D.uy[1] : D.uy[0]);
-  const double* az = cell ? D.cz : (cur ? D.uz[1] : D.uz[0]);
+  const double* ax = cell ? D.cx : px;
+  const double* ay = cell ? D.cy : py;
+  const double* az = cell ? D.cz : pz;
   double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   for (int b = threadIdx.x; b < P.n; b += blockDim.x) {
     const int g = P.off + b;
     const double v[3] = {ax[g], ay[g], az[g]};
     for (int a = 0; a < 3; a++) { lo[a] = fmin(lo[a], v[a]); hi[a] = fmax(hi[a], v[a]); }
+    if (cand) { D.u0x[g] = v[0]; D.u0y[g] = v[1]; D.u0z[g] = v[2]; }
   }
-  __shared__ double sh[6][256];
   for (int a = 0; a < 3; a++) { sh[a][threadIdx.x] = lo[a]; sh[3 + a][threadIdx.x] = hi[a]; }
   __syncthreads();
   for (int w = blockDim.x / 2; w > 0; w >>= 1) {
@@ -377,7 +415,7 @@ __global__ void __launch_bounds__(256) k_grid(Dev D, int cur, int rel, int mode,
   // every pair difference stays below half the period: the min image is the
   // plain difference, so all kept pairs are in adjacent cells of a local grid
   const double limit = 0.49 * (cell ? P.ncell : P.boxl);
-  const double inv = cell ? 1.0 : 1.0 / P.gcell;
+  const double inv = cell ? 1.0 : 1.0 / (cand ? P.gcand : P.gcell);
   int dims[3];
   bool use = true;
   for (int a = 0; a < 3; a++) {
@@ -389,7 +427,8 @@ __global__ void __launch_bounds__(256) k_grid(Dev D, int cur, int rel, int mode,
   G.ox = sh[0][0]; G.oy = sh[1][0]; G.oz = sh[2][0];
   G.inv = inv;
   G.nx = dims[0]; G.ny = dims[1]; G.nz = dims[2];
-  G.use = use ? 1 : 0;
+  G.work = !use ? W_SCAN : (cand ? (W_BIN | W_CAND | W_FILTER) : W_BIN);
+  if (cand) G.cand_valid = use ? 1 : 0;
 }
 
 __device__ __forceinline__ int grid_coord(double v, double o, double inv, int n) {
@@ -399,16 +438,13 @@ __device__ __forceinline__ int grid_coord(double v, double o, double inv, int n)
 
 // Counting sort of beads into cells: count, scan, fill.
 __global__ void k_bin_count(Dev D, int cur, int rel, int mode, int g0, int g1) {
-  rel += *D.relb;
   const int g = g0 + blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= g1) return;
   const int c = D.chain[g];
   if (c < 0) return;
-  const ChainP& P = D.cp[c];
   const GridP& G = D.grid[c];
-  int crit;
-  if (!rebuild_due(P, rel, mode, crit) || !G.use) return;
-  const bool cell = crit == SOPG_CUTOFF_CELL;
+  if (!(G.work & W_BIN)) return;
+  const bool cell = (mode >= 0 ? mode : D.cp[c].cutoff) == SOPG_CUTOFF_CELL;
   const double x = cell ? D.cx[g] : (cur ? D.ux[1] : D.ux[0])[g];
   const double y = cell ? D.cy[g] : (cur ? D.uy[1] : D.uy[0])[g];
   const double z = cell ? D.cz[g] : (cur ? D.uz[1] : D.uz[0])[g];
@@ -419,12 +455,9 @@ __global__ void k_bin_count(Dev D, int cur, int rel, int mode, int g0, int g1) {
 }
 
 __global__ void __launch_bounds__(256) k_bin_scan(Dev D, int rel, int mode, int c0) {
-  rel += *D.relb;
   const int c = c0 + blockIdx.x;
-  const ChainP& P = D.cp[c];
   const GridP& G = D.grid[c];
-  int crit;
-  if (!rebuild_due(P, rel, mode, crit) || !G.use) return;
+  if (!(G.work & W_BIN)) return;
   const int ncell = G.nx * G.ny * G.nz;
   int* cnt = D.cell_cnt + (size_t)c * kGridCells;
   int* st = D.cell_start + (size_t)c * kGridCells;
@@ -454,14 +487,12 @@ __global__ void __launch_bounds__(256) k_bin_scan(Dev D, int rel, int mode, int 
 }
 
 __global__ void k_bin_fill(Dev D, int rel, int mode, int g0, int g1) {
-  rel += *D.relb;
   const int g = g0 + blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= g1) return;
   const int c = D.chain[g];
   if (c < 0) return;
+  if (!(D.grid[c].work & W_BIN)) return;
   const ChainP& P = D.cp[c];
-  int crit;
-  if (!rebuild_due(P, rel, mode, crit) || !D.grid[c].use) return;
   const size_t cb = (size_t)c * kGridCells + D.bead_cell[g];
   const int at = D.cell_start[cb] + atomicAdd(&D.cell_fill[cb], 1);
   D.cell_beads[P.off + at] = g - P.off;
@@ -473,15 +504,16 @@ __global__ void k_bin_fill(Dev D, int rel, int mode, int g0, int g1) {
 // neighbouring grid cells are walked as 9 x-runs: cells (x-1..x+1, y, z) are
 // consecutive in the cell-sorted bead array.
 template <int LG>
-__global__ void __launch_bounds__(256) k_rebuild_grid(Dev D, const Tables* __restrict__ T, int cur, int par, int rel,
+__global__ void __launch_bounds__(256) k_rebuild_grid(Dev D, const Tables* __restrict__ T, int cur, int par,
                                                       int mode, int g0, int g1) {
-  rel += *D.relb;
   constexpr int BPB = 256 / LG;
   extern __shared__ uint32_t bitmaps[];
-  __shared__ double cut2[9][2];  // per type pair (it * 3 + jt): (3.2 sigma)^2, (2.5 sigma)^2
+  // per type pair (it * 3 + jt): (3.2 sigma)^2, (2.5 sigma)^2, (3.2 sigma + skin)^2
+  __shared__ double cut2[9][3];
   if (threadIdx.x < 9) {
     cut2[threadIdx.x][0] = T->nlrc2[threadIdx.x / 3][threadIdx.x % 3];
     cut2[threadIdx.x][1] = T->plrc2[threadIdx.x / 3][threadIdx.x % 3];
+    cut2[threadIdx.x][2] = T->candc2[threadIdx.x / 3][threadIdx.x % 3];
   }
   const int grp = (int)threadIdx.x / LG, l = (int)threadIdx.x & (LG - 1);
   const int g = g0 + xcd_block() * BPB + grp;
@@ -490,12 +522,19 @@ __global__ void __launch_bounds__(256) k_rebuild_grid(Dev D, const Tables* __res
   uint32_t* plb = nlb + W;
   bool active = g < g1;
   if (active) active = D.chain[g] >= 0;
-  int crit = 0;
+  int crit = 0, work = 0;
   const ChainP* Pp = nullptr;
   if (active) {
-    Pp = &D.cp[D.chain[g]];
-    active = rebuild_due(*Pp, rel, mode, crit) && D.grid[D.chain[g]].use;
+    work = D.grid[D.chain[g]].work;
+    active = (work & W_BIN) != 0;
   }
+  if (active) {
+    Pp = &D.cp[D.chain[g]];
+    crit = mode >= 0 ? mode : Pp->cutoff;
+  }
+  // candidate-list build: the wider cutoff, no pair-list bits, natives left
+  // to the filter
+  const bool cmode = (work & W_CAND) != 0;
   const double* ux = cur ? D.ux[1] : D.ux[0];
   const double* uy = cur ? D.uy[1] : D.uy[0];
   const double* uz = cur ? D.uz[1] : D.uz[0];
@@ -508,7 +547,7 @@ __global__ void __launch_bounds__(256) k_rebuild_grid(Dev D, const Tables* __res
     a1 = D.att_end[g];
     const double boxl = P.boxl, qbox = P.qbox;
     // natives: NL / PL flags, exactly as the full scan
-    for (int q = a0 + l; q < a1; q += LG) {
+    for (int q = a0 + l; q < a1 && !cmode; q += LG) {
       const int code = D.att_code[q], k = code >> 1, side = code & 1, o = D.att_partner[q];
       const int i = side ? o : g, j = side ? g : o;
       const double dx = mimg(ux[j] - ux[i], boxl, qbox);
@@ -585,11 +624,11 @@ __global__ void __launch_bounds__(256) k_rebuild_grid(Dev D, const Tables* __res
       double ddx, ddy, ddz;
       pair_d3(side, ox, oy, oz, px, py, pz, boxl, qbox, ddx, ddy, ddz);
       const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-      const bool keep = nlcrit ? d2 < cut2[tpi][0] : cell_adjacent(D, P, side ? gc : g, side ? g : gc);
+      const bool keep = nlcrit ? d2 < cut2[tpi][cmode ? 2 : 0] : cell_adjacent(D, P, side ? gc : g, side ? g : gc);
       if (keep) {
         const uint32_t bit = 1u << (cc & 31);
         atomicOr(&nlb[cc >> 5], bit);
-        if (d2 < cut2[tpi][1]) atomicOr(&plb[cc >> 5], bit);
+        if (!cmode && d2 < cut2[tpi][1]) atomicOr(&plb[cc >> 5], bit);
       }
     }
   }
@@ -609,8 +648,8 @@ __global__ void __launch_bounds__(256) k_rebuild_grid(Dev D, const Tables* __res
   const ChainP& P = *Pp;
   const int off = P.off;
   const uint32_t gt = (uint32_t)D.type[g];
-  uint32_t* S = D.slot + (size_t)g * D.cap;
-  const uint32_t pbit = 1u << (30 + par);
+  uint32_t* S = (cmode ? D.cand : D.slot) + (size_t)g * D.cap;
+  const uint32_t pbit = cmode ? 0u : 1u << (30 + par);
   int cnt = 0;
   for (int base = 0; base < words; base += LG) {
     const int w = base + l;
@@ -639,9 +678,66 @@ __global__ void __launch_bounds__(256) k_rebuild_grid(Dev D, const Tables* __res
     cnt += __shfl(incl, LG - 1, LG);
   }
   if (l == 0) {
-    D.cnt[g] = cnt < D.cap ? cnt : D.cap;
+    (cmode ? D.ccnt : D.cnt)[g] = cnt < D.cap ? cnt : D.cap;
     if (cnt > D.cap) atomicOr(D.err, 1);
   }
+}
+
+// Neighbor list from the candidate list (chains planned W_FILTER): the
+// candidates within the neighbor-list cutoff at the current positions, in
+// their (ascending, master) order, with the pair-list bit at the same d^2;
+// natives as in the full scan.  One group of LF lanes per bead.
+template <int LF>
+__global__ void __launch_bounds__(256) k_cand_filter(Dev D, const Tables* __restrict__ T, int cur, int par, int g0,
+                                                     int g1) {
+  const int t = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
+  const int g = g0 + t / LF, l = t & (LF - 1);
+  if (g >= g1) return;
+  const int c = D.chain[g];
+  if (c < 0 || !(D.grid[c].work & W_FILTER)) return;
+  const ChainP& P = D.cp[c];
+  const double* ux = cur ? D.ux[1] : D.ux[0];
+  const double* uy = cur ? D.uy[1] : D.uy[0];
+  const double* uz = cur ? D.uz[1] : D.uz[0];
+  const double boxl = P.boxl, qbox = P.qbox;
+  // natives (neighbor_list.cpp:17-67)
+  const int a0 = D.att_beg[g], a1 = D.att_end[g];
+  for (int q = a0 + l; q < a1; q += LF) {
+    const int code = D.att_code[q], k = code >> 1, side = code & 1, o = D.att_partner[q];
+    const int i = side ? o : g, j = side ? g : o;
+    const double dx = mimg(ux[j] - ux[i], boxl, qbox);
+    const double dy = mimg(uy[j] - uy[i], boxl, qbox);
+    const double dz = mimg(uz[j] - uz[i], boxl, qbox);
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    const bool nl = d2 < D.am_nlrc2[k], pl = nl && d2 < D.att_k[q].plrc2;
+    D.att_flag[q] = (uint8_t)((nl ? 1 : 0) | (pl ? (2 << par) : 0));
+  }
+  // non-natives (neighbor_list.cpp:97-103): stable compaction of the candidates
+  const double cx = ux[g], cy = uy[g], cz = uz[g];
+  const uint32_t* C = D.cand + (size_t)g * D.cap;
+  uint32_t* S = D.slot + (size_t)g * D.cap;
+  const int n = D.ccnt[g];
+  const uint32_t pbit = 1u << (30 + par);
+  int cnt = 0;
+  for (int base = 0; base < n; base += LF) {
+    const int q = base + l;
+    bool keep = false;
+    uint32_t w = 0u;
+    if (q < n) {
+      w = C[q];
+      const int o = P.off + (int)(w & kPartnerMask), side = (w >> 25) & 1;
+      const int it = (w >> 26) & 3, jt = (w >> 28) & 3;
+      double dx, dy, dz;
+      pair_d3(side, cx, cy, cz, ux[o], uy[o], uz[o], boxl, qbox, dx, dy, dz);
+      const double d2 = dx * dx + dy * dy + dz * dz;
+      keep = d2 < T->nlrc2[it][jt];
+      w = (w & ~(3u << 30)) | ((keep && d2 < T->plrc2[it][jt]) ? pbit : 0u);
+    }
+    const uint64_t m = group_mask<LF>(keep);
+    if (keep) S[cnt + __popcll(m & ((1ull << l) - 1ull))] = w;
+    cnt += __popcll(m);
+  }
+  if (l == 0) D.cnt[g] = cnt;  // <= the candidate count <= cap
 }
 
 // Pair-list filter at the current positions (sop.cpp:473-553), parity par.
@@ -1248,9 +1344,9 @@ constexpr int kGridLanes = 16;                       // lanes per bead in the bi
 constexpr int kGridBeads = 256 / kGridLanes;         // beads per block
 constexpr int kMaxBitmapWords = 512;                 // 16 beads x 2 bitmaps x 2 KiB = 64 KiB of LDS
 void launch_rebuild(int L, const Dev& D, const Tables* T, int cur, int par, int rel, int mode, int g0, int g1,
-                    int c0, int nc, int binned, hipStream_t s) {
+                    int c0, int nc, int binned, int cand, hipStream_t s) {
   const int words = D.bitmap_words < kMaxBitmapWords ? D.bitmap_words : kMaxBitmapWords;
-  hipLaunchKernelGGL(k_grid, dim3(nc), dim3(256), 0, s, D, cur, rel, mode, c0, binned ? 32 * words : 0);
+  hipLaunchKernelGGL(k_grid, dim3(nc), dim3(256), 0, s, D, T, cur, rel, mode, c0, binned ? 32 * words : 0);
   if (binned) {
     hipLaunchKernelGGL(k_bin_count, dim3(blocks_for(g1 - g0)), dim3(256), 0, s, D, cur, rel, mode, g0, g1);
     hipLaunchKernelGGL(k_bin_scan, dim3(nc), dim3(256), 0, s, D, rel, mode, c0);
@@ -1258,8 +1354,10 @@ void launch_rebuild(int L, const Dev& D, const Tables* T, int cur, int par, int 
     Dev Dg = D;
     Dg.bitmap_words = words;
     hipLaunchKernelGGL(k_rebuild_grid<kGridLanes>, dim3((unsigned)((g1 - g0 + kGridBeads - 1) / kGridBeads)),
-                       dim3(256), (size_t)kGridBeads * 2 * words * sizeof(uint32_t), s, Dg, T, cur, par, rel, mode,
-                       g0, g1);
+                       dim3(256), (size_t)kGridBeads * 2 * words * sizeof(uint32_t), s, Dg, T, cur, par, mode, g0, g1);
+    if (cand && mode < 0)
+      hipLaunchKernelGGL(k_cand_filter<kGridLanes>, dim3((unsigned)((g1 - g0 + kGridBeads - 1) / kGridBeads)),
+                         dim3(256), 0, s, D, T, cur, par, g0, g1);
   }
 #define C_(X) launch_rebuild_L<X>(D, T, cur, par, rel, mode, g0, g1, s)
   SOPG_DISPATCH_L(L, C_)

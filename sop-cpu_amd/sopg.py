@@ -90,7 +90,7 @@ EXPORTS = [
     "sopgen_saw", "sopgen_native", "sopgen_maxwell", "sopgen_check", "sopgen_write_topology",
     "sopgen_write_xyz", "sopgen_natives", "sopg_force_terms", "sopg_set_binned",
     "sopg_get_lanes", "sopg_autotune", "sopg_reserve", "sopg_remove_chain", "sopg_get_step",
-    "sopg_checkpoint", "sopg_restore", "sopg_set_graphs", "sopg_graph_stats", "sopg_set_resident",
+    "sopg_checkpoint", "sopg_restore", "sopg_set_graphs", "sopg_graph_stats", "sopg_set_resident", "sopg_set_skin",
     "sopg_server_create", "sopg_server_destroy", "sopg_server_last_error", "sopg_server_now",
     "sopg_server_footprint", "sopg_server_submit", "sopg_server_poll", "sopg_server_job",
     "sopg_server_job_records", "sopg_server_job_state", "sopg_server_release", "sopg_server_stats",
@@ -126,6 +126,7 @@ def lib():
         L.sopg_restore.argtypes = [vp, C.c_int, vp, C.c_long]
         L.sopg_set_graphs.argtypes = [vp, C.c_int]
         L.sopg_set_resident.argtypes = [vp, C.c_int, C.c_int]
+        L.sopg_set_skin.argtypes = [vp, C.c_double]
         L.sopg_graph_stats.argtypes = [vp, C.POINTER(C.c_long), C.POINTER(C.c_long)]
         L.sopg_force_terms.argtypes = [vp, C.c_int, C.c_int, vp]
         L.sopg_overdamped_iteration.argtypes = [vp, C.c_int, vp]
@@ -374,6 +375,9 @@ class Context:
         lanes = C.c_int()
         self._chk(self.L.sopg_autotune(self.h, C.byref(lanes)))
         return lanes.value
+
+    def set_skin(self, skin):
+        self._chk(self.L.sopg_set_skin(self.h, float(skin)))
 
     def set_resident(self, max_beads, lanes=0):
         self._chk(self.L.sopg_set_resident(self.h, max_beads, lanes))

@@ -639,3 +639,48 @@ def test_resident_kernel_overdamped(inputs_root):
         for k in ("unc", "vel", "force", "pos"):
             assert np.array_equal(a[k], b[k]), k
     assert ra == rb and len(ra) == 2 * (200 // 40)
+
+
+def test_candidate_lists_keep_lists_and_results(inputs_root):
+    """Neighbor lists filtered from candidate lists (reused while no bead has
+    moved more than half the skin since their build) are the lists of a full
+    rebuild: states, records and lists agree bit for bit for skin 0 (off), a
+    skin small enough to rebuild the candidates nearly every time, and the
+    default -- with positions moved from the host between runs, a cell-list
+    chain beside, and runs starting between rebuilds."""
+    specs = [(90, 151, sopg.CUTOFF_NEIGHBOR), (260, 152, sopg.CUTOFF_NEIGHBOR), (700, 153, sopg.CUTOFF_NEIGHBOR),
+             (150, 154, sopg.CUTOFF_CELL)]
+    data = [chain_inputs(inputs_root, n, seed)[:2] for (n, seed, _) in specs]
+    results = []
+    for skin in (0.0, 0.2, 3.0):
+        ctx = sopg.Context()
+        for (n, seed, cut), (x, v) in zip(specs, data):
+            ctx.add_chain(sopg.chain_topology(x),
+                          sopg.chain_params(T=1.2, boxl=504.0, ncell=42.0, cutoff=cut, nnlup=20, nup=50,
+                                            noise_mode=sopg.NOISE_PHILOX, seed=10 + seed))
+        ctx.set_skin(skin)
+        for c, ((n, seed, cut), (x, v)) in enumerate(zip(specs, data)):
+            ctx.put_state(c, pos=x, unc=x, vel=v, force=np.zeros_like(x))
+            (ctx.update_cell_list if cut == sopg.CUTOFF_CELL else ctx.update_neighbor_list)(c)
+            ctx.update_pair_list(c)
+            ctx.force_eval(c)
+        print(f"[gpu] candidate lists skin={skin}", flush=True)
+        recs = ctx.run(130)
+        # move chain 1 from the host (to its native positions again) mid-run
+        st = ctx.get_state(1)
+        ctx.put_state(1, pos=data[1][0], unc=data[1][0], vel=st["vel"], force=st["force"])
+        recs += ctx.run(207)
+        recs += ctx.run(95)
+        lists = [[ctx.get_list(w, c) for w in (sopg.LIST_NL_REP, sopg.LIST_PL_REP, sopg.LIST_NL_ATT)]
+                 for c in range(len(specs))]
+        results.append(([ctx.get_state(c) for c in range(len(specs))], recs, lists))
+        ctx.close()
+    (s0, r0, l0) = results[0]
+    for (s1, r1, l1) in results[1:]:
+        for a, b in zip(s0, s1):
+            for k in ("unc", "vel", "force", "pos"):
+                assert np.array_equal(a[k], b[k]), k
+        assert r0 == r1
+        for la, lb in zip(l0, l1):
+            for (ai, aj), (bi, bj) in zip(la, lb):
+                assert np.array_equal(ai, bi) and np.array_equal(aj, bj)
