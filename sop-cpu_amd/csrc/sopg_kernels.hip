@@ -690,6 +690,12 @@ __global__ void __launch_bounds__(256) k_rebuild_grid(Dev D, const Tables* __res
 template <int LF>
 __global__ void __launch_bounds__(256) k_cand_filter(Dev D, const Tables* __restrict__ T, int cur, int par, int g0,
                                                      int g1) {
+  __shared__ double cut2[9][2];  // per type pair (it * 3 + jt): (3.2 sigma)^2, (2.5 sigma)^2
+  if (threadIdx.x < 9) {
+    cut2[threadIdx.x][0] = T->nlrc2[threadIdx.x / 3][threadIdx.x % 3];
+    cut2[threadIdx.x][1] = T->plrc2[threadIdx.x / 3][threadIdx.x % 3];
+  }
+  __syncthreads();
   const int t = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
   const int g = g0 + t / LF, l = t & (LF - 1);
   if (g >= g1) return;
@@ -700,6 +706,23 @@ __global__ void __launch_bounds__(256) k_cand_filter(Dev D, const Tables* __rest
   const double* uy = cur ? D.uy[1] : D.uy[0];
   const double* uz = cur ? D.uz[1] : D.uz[0];
   const double boxl = P.boxl, qbox = P.qbox;
+  const int off = P.off;
+  // non-natives (neighbor_list.cpp:97-103): stable compaction of the
+  // candidates.  Software pipeline as in the step kernel (candidate words two
+  // chunks ahead, partner coordinates one chunk ahead), started before the
+  // natives so that their loads overlap.
+  const double cx = ux[g], cy = uy[g], cz = uz[g];
+  const uint32_t* C = D.cand + (size_t)g * D.cap;
+  uint32_t* S = D.slot + (size_t)g * D.cap;
+  const int n = D.ccnt[g];
+  uint32_t wn = 0u, wn2 = 0u;
+  double pxn = 0.0, pyn = 0.0, pzn = 0.0;
+  if (l < n) {
+    wn = C[l];
+    const int o = off + (int)(wn & kPartnerMask);
+    pxn = ux[o]; pyn = uy[o]; pzn = uz[o];
+  }
+  if (l + LF < n) wn2 = C[l + LF];
   // natives (neighbor_list.cpp:17-67)
   const int a0 = D.att_beg[g], a1 = D.att_end[g];
   for (int q = a0 + l; q < a1; q += LF) {
@@ -712,26 +735,26 @@ __global__ void __launch_bounds__(256) k_cand_filter(Dev D, const Tables* __rest
     const bool nl = d2 < D.am_nlrc2[k], pl = nl && d2 < D.att_k[q].plrc2;
     D.att_flag[q] = (uint8_t)((nl ? 1 : 0) | (pl ? (2 << par) : 0));
   }
-  // non-natives (neighbor_list.cpp:97-103): stable compaction of the candidates
-  const double cx = ux[g], cy = uy[g], cz = uz[g];
-  const uint32_t* C = D.cand + (size_t)g * D.cap;
-  uint32_t* S = D.slot + (size_t)g * D.cap;
-  const int n = D.ccnt[g];
   const uint32_t pbit = 1u << (30 + par);
   int cnt = 0;
   for (int base = 0; base < n; base += LF) {
     const int q = base + l;
+    uint32_t w = wn;
+    const double px = pxn, py = pyn, pz = pzn;
+    if (q + LF < n) {
+      const int o = off + (int)(wn2 & kPartnerMask);
+      pxn = ux[o]; pyn = uy[o]; pzn = uz[o];
+    }
+    wn = wn2;
+    if (q + 2 * LF < n) wn2 = C[q + 2 * LF];
     bool keep = false;
-    uint32_t w = 0u;
     if (q < n) {
-      w = C[q];
-      const int o = P.off + (int)(w & kPartnerMask), side = (w >> 25) & 1;
-      const int it = (w >> 26) & 3, jt = (w >> 28) & 3;
+      const int side = (w >> 25) & 1, tpi = (int)((w >> 26) & 3) * 3 + (int)((w >> 28) & 3);
       double dx, dy, dz;
-      pair_d3(side, cx, cy, cz, ux[o], uy[o], uz[o], boxl, qbox, dx, dy, dz);
+      pair_d3(side, cx, cy, cz, px, py, pz, boxl, qbox, dx, dy, dz);
       const double d2 = dx * dx + dy * dy + dz * dz;
-      keep = d2 < T->nlrc2[it][jt];
-      w = (w & ~(3u << 30)) | ((keep && d2 < T->plrc2[it][jt]) ? pbit : 0u);
+      keep = d2 < cut2[tpi][0];
+      w = (w & ~(3u << 30)) | ((keep && d2 < cut2[tpi][1]) ? pbit : 0u);
     }
     const uint64_t m = group_mask<LF>(keep);
     if (keep) S[cnt + __popcll(m & ((1ull << l) - 1ull))] = w;
@@ -1342,6 +1365,7 @@ void launch_cellcoords(const Dev& D, int cur, int g0, int g1, hipStream_t s) {
 // chains whose grid applies (binned != 0), the full scan for the others.
 constexpr int kGridLanes = 16;                       // lanes per bead in the binned rebuild
 constexpr int kGridBeads = 256 / kGridLanes;         // beads per block
+constexpr int kFilterLanes = 16;                     // lanes per bead in the candidate filter
 constexpr int kMaxBitmapWords = 512;                 // 16 beads x 2 bitmaps x 2 KiB = 64 KiB of LDS
 void launch_rebuild(int L, const Dev& D, const Tables* T, int cur, int par, int rel, int mode, int g0, int g1,
                     int c0, int nc, int binned, int cand, hipStream_t s) {
@@ -1356,8 +1380,8 @@ void launch_rebuild(int L, const Dev& D, const Tables* T, int cur, int par, int 
     hipLaunchKernelGGL(k_rebuild_grid<kGridLanes>, dim3((unsigned)((g1 - g0 + kGridBeads - 1) / kGridBeads)),
                        dim3(256), (size_t)kGridBeads * 2 * words * sizeof(uint32_t), s, Dg, T, cur, par, mode, g0, g1);
     if (cand && mode < 0)
-      hipLaunchKernelGGL(k_cand_filter<kGridLanes>, dim3((unsigned)((g1 - g0 + kGridBeads - 1) / kGridBeads)),
-                         dim3(256), 0, s, D, T, cur, par, g0, g1);
+      hipLaunchKernelGGL(k_cand_filter<kFilterLanes>, dim3(blocks_for((long)(g1 - g0) * kFilterLanes)), dim3(256), 0, s,
+                         D, T, cur, par, g0, g1);
   }
 #define C_(X) launch_rebuild_L<X>(D, T, cur, par, rel, mode, g0, g1, s)
   SOPG_DISPATCH_L(L, C_)
