@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -627,8 +628,13 @@ int ensure_device(sopg_ctx* c) {
   const size_t init[NPOOL] = {beads, 2 * nbnd, 2 * nang, natt, 2 * natt, nrep, 2 * nrep, nch};
   for (int pi = 0; pi < NPOOL; pi++)
     if ((rc = grow_pool(c, pi, std::max<size_t>(init[pi], 1)))) return rc;
-  if ((rc = dalloc(c, &D.err, 1)) || (rc = dalloc(c, &c->d_T, 1)) || (rc = dalloc(c, &c->d_relb, 2))) return rc;
+  if ((rc = dalloc(c, &D.err, 1)) || (rc = dalloc(c, &c->d_T, 1)) || (rc = dalloc(c, &c->d_relb, 2)) ||
+      (rc = dalloc(c, &D.plan, 3)))
+    return rc;
   HIP_OK(c, hipMemsetAsync(D.err, 0, sizeof(int), c->stream));
+  const int no_plan[3] = {INT_MIN, INT_MIN, INT_MIN};
+  HIP_OK(c, hipMemcpyAsync(D.plan, no_plan, sizeof no_plan, hipMemcpyHostToDevice, c->stream));
+  HIP_OK(c, hipStreamSynchronize(c->stream));
   HIP_OK(c, hipMemsetAsync(c->d_relb, 0, 2 * sizeof(int), c->stream));
   D.relb = c->d_relb;
   HIP_OK(c, hipMemcpyAsync(c->d_T, &c->T, sizeof(Tables), hipMemcpyHostToDevice, c->stream));

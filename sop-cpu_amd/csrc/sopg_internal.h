@@ -121,6 +121,10 @@ struct Dev {
   double *cx, *cy, *cz;  // cell coordinates (cell list)
   double* red;           // per-bead reduction fields
   int* err;              // bit0: slot overflow
+  // The (run-relative) step of the latest rebuild in which some chain needed
+  // the binned kernels [0], the candidate filter [1], the full scan [2]: a
+  // kernel of a rebuild in which no chain needs it returns at once.
+  int* plan;
   ChainP* cp;
   // binned rebuild: per-chain grid (GridP), kGridCells cells per chain
   GridP* grid;

@@ -316,6 +316,7 @@ template <int L>
 __global__ void __launch_bounds__(256) k_rebuild(Dev D, const Tables* __restrict__ T, int cur, int par, int rel,
                                                  int mode, int g0, int g1) {
   rel += *D.relb;
+  if (D.plan[2] != rel) return;  // no chain takes the full scan in this rebuild
   const int l = threadIdx.x & (L - 1);
   const int stride = (int)(gridDim.x * blockDim.x / L);
   for (int g = g0 + (xcd_block() * (int)blockDim.x + (int)threadIdx.x) / L; g < g1; g += stride)
@@ -357,7 +358,7 @@ __global__ void __launch_bounds__(256) k_grid(Dev D, const Tables* __restrict__ 
     return;
   }
   if (P.n > maxn || P.rep_explicit || crit == SOPG_CUTOFF_NONE) {
-    if (threadIdx.x == 0) G.work = W_SCAN;
+    if (threadIdx.x == 0) { G.work = W_SCAN; D.plan[2] = rel; }
     return;
   }
   const bool cand = mode < 0 && crit == SOPG_CUTOFF_NEIGHBOR && T->cand_on;
@@ -386,7 +387,7 @@ __global__ void __launch_bounds__(256) k_grid(Dev D, const Tables* __restrict__ 
     const bool keep = sh[0][0] <= T->cand_disp2;  // false for NaN
     __syncthreads();
     if (keep) {
-      if (threadIdx.x == 0) G.work = W_FILTER;
+      if (threadIdx.x == 0) { G.work = W_FILTER; D.plan[1] = rel; }
       return;
     }
   }
@@ -428,6 +429,9 @@ __global__ void __launch_bounds__(256) k_grid(Dev D, const Tables* __restrict__ 
   G.inv = inv;
   G.nx = dims[0]; G.ny = dims[1]; G.nz = dims[2];
   G.work = !use ? W_SCAN : (cand ? (W_BIN | W_CAND | W_FILTER) : W_BIN);
+  if (G.work & W_BIN) D.plan[0] = rel;
+  if (G.work & W_FILTER) D.plan[1] = rel;
+  if (G.work & W_SCAN) D.plan[2] = rel;
   if (cand) G.cand_valid = use ? 1 : 0;
 }
 
@@ -438,6 +442,8 @@ __device__ __forceinline__ int grid_coord(double v, double o, double inv, int n)
 
 // Counting sort of beads into cells: count, scan, fill.
 __global__ void k_bin_count(Dev D, int cur, int rel, int mode, int g0, int g1) {
+  rel += *D.relb;
+  if (D.plan[0] != rel) return;  // no chain is binned in this rebuild
   const int g = g0 + blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= g1) return;
   const int c = D.chain[g];
@@ -455,6 +461,8 @@ __global__ void k_bin_count(Dev D, int cur, int rel, int mode, int g0, int g1) {
 }
 
 __global__ void __launch_bounds__(256) k_bin_scan(Dev D, int rel, int mode, int c0) {
+  rel += *D.relb;
+  if (D.plan[0] != rel) return;
   const int c = c0 + blockIdx.x;
   const GridP& G = D.grid[c];
   if (!(G.work & W_BIN)) return;
@@ -487,6 +495,8 @@ __global__ void __launch_bounds__(256) k_bin_scan(Dev D, int rel, int mode, int 
 }
 
 __global__ void k_bin_fill(Dev D, int rel, int mode, int g0, int g1) {
+  rel += *D.relb;
+  if (D.plan[0] != rel) return;
   const int g = g0 + blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= g1) return;
   const int c = D.chain[g];
@@ -504,8 +514,10 @@ __global__ void k_bin_fill(Dev D, int rel, int mode, int g0, int g1) {
 // neighbouring grid cells are walked as 9 x-runs: cells (x-1..x+1, y, z) are
 // consecutive in the cell-sorted bead array.
 template <int LG>
-__global__ void __launch_bounds__(256) k_rebuild_grid(Dev D, const Tables* __restrict__ T, int cur, int par,
+__global__ void __launch_bounds__(256) k_rebuild_grid(Dev D, const Tables* __restrict__ T, int cur, int par, int rel,
                                                       int mode, int g0, int g1) {
+  rel += *D.relb;
+  if (D.plan[0] != rel) return;  // block-uniform: before any barrier
   constexpr int BPB = 256 / LG;
   extern __shared__ uint32_t bitmaps[];
   // per type pair (it * 3 + jt): (3.2 sigma)^2, (2.5 sigma)^2, (3.2 sigma + skin)^2
@@ -688,8 +700,10 @@ __global__ void __launch_bounds__(256) k_rebuild_grid(Dev D, const Tables* __res
 // their (ascending, master) order, with the pair-list bit at the same d^2;
 // natives as in the full scan.  One group of LF lanes per bead.
 template <int LF>
-__global__ void __launch_bounds__(256) k_cand_filter(Dev D, const Tables* __restrict__ T, int cur, int par, int g0,
-                                                     int g1) {
+__global__ void __launch_bounds__(256) k_cand_filter(Dev D, const Tables* __restrict__ T, int cur, int par, int rel,
+                                                     int g0, int g1) {
+  rel += *D.relb;
+  if (D.plan[1] != rel) return;  // block-uniform: before the barrier
   __shared__ double cut2[9][2];  // per type pair (it * 3 + jt): (3.2 sigma)^2, (2.5 sigma)^2
   if (threadIdx.x < 9) {
     cut2[threadIdx.x][0] = T->nlrc2[threadIdx.x / 3][threadIdx.x % 3];
@@ -1378,10 +1392,11 @@ void launch_rebuild(int L, const Dev& D, const Tables* T, int cur, int par, int 
     Dev Dg = D;
     Dg.bitmap_words = words;
     hipLaunchKernelGGL(k_rebuild_grid<kGridLanes>, dim3((unsigned)((g1 - g0 + kGridBeads - 1) / kGridBeads)),
-                       dim3(256), (size_t)kGridBeads * 2 * words * sizeof(uint32_t), s, Dg, T, cur, par, mode, g0, g1);
+                       dim3(256), (size_t)kGridBeads * 2 * words * sizeof(uint32_t), s, Dg, T, cur, par, rel, mode,
+                       g0, g1);
     if (cand && mode < 0)
       hipLaunchKernelGGL(k_cand_filter<kFilterLanes>, dim3(blocks_for((long)(g1 - g0) * kFilterLanes)), dim3(256), 0, s,
-                         D, T, cur, par, g0, g1);
+                         D, T, cur, par, rel, g0, g1);
   }
 #define C_(X) launch_rebuild_L<X>(D, T, cur, par, rel, mode, g0, g1, s)
   SOPG_DISPATCH_L(L, C_)
