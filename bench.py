@@ -137,10 +137,12 @@ def timed_run(ctx, steps, warmup, profile, autotune=False):
     return t1 - t0, prof, recs
 
 
-def gpu_batched(chains, w, steps, warmup, lanes, list_lanes, topo_cache, graphs=True, resident=(0, 0)):
+def gpu_batched(chains, w, steps, warmup, lanes, list_lanes, topo_cache, graphs=True, resident=(0, 0), skin=None):
     ctx, topos = build_ctx(chains, w, lanes, list_lanes, topo_cache)
     ctx.set_graphs(graphs)
     ctx.set_resident(*resident)
+    if skin is not None:
+        ctx.set_skin(skin)
     # value: the production path (graph replay on, no per-kernel timing)
     seconds, _, recs = timed_run(ctx, steps, warmup, False, autotune=not lanes)
     step_lanes = ctx.lanes()[0]
@@ -347,6 +349,8 @@ def main():
     ap.add_argument("--resident", type=int, default=0,
                     help="chains of at most this many beads run in the persistent resident kernel (0: none)")
     ap.add_argument("--resident-lanes", type=int, default=0)
+    ap.add_argument("--skin", type=float, default=None,
+                    help="candidate-list skin of neighbor-list rebuilds in Angstrom (default: the library's; 0: off)")
     ap.add_argument("--serve-jobs", type=int, default=100)
     ap.add_argument("--serve-step-scale", type=float, default=0.1,
                     help="job step counts are log-uniform in [1e4, 1e6] times this factor")
@@ -369,7 +373,7 @@ def main():
     nb = sum(sizes)
     topo_cache = {}
     g = gpu_batched(chains, w, a.steps, a.warmup, a.lanes, a.list_lanes, topo_cache, graphs=not a.no_graphs,
-                    resident=(a.resident, a.resident_lanes))
+                    resident=(a.resident, a.resident_lanes), skin=a.skin)
     value = nb * a.steps / g["seconds"]
     print(f"[bench] batched {value:.4g} bead-steps/s", file=sys.stderr, flush=True)
     plain, plain_t = (None, None) if a.no_plain else gpu_plain(chains, w, a.plain_steps, 10, topo_cache, a.lanes,

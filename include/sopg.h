@@ -200,7 +200,7 @@ int sopg_restore(sopg_ctx* ctx, int chain, const void* buf, long size);
  * not depend on it.  No counterpart in the reference. */
 int sopg_set_resident(sopg_ctx* ctx, int max_beads, int lanes);
 /* Skin of the candidate lists of neighbor-list rebuilds during sopg_run, in
- * Angstrom (default 3; 0: off).  A candidate list holds a bead's non-native
+ * Angstrom (default 4; 0: off).  A candidate list holds a bead's non-native
  * pairs within 3.2 sigma + skin of the positions it was built at; while no
  * bead of the chain has moved more than skin/2 from them, a rebuild filters
  * the candidates instead of scanning the grid, otherwise it rebuilds them.

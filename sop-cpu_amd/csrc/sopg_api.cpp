@@ -129,7 +129,7 @@ struct sopg_ctx {
   int list_lanes = 64;  // lanes per bead, full-scan list rebuild
   int binned = 1;       // binned list rebuild where a chain's extent allows it
   int cap_user = 0;
-  double skin = 3.0;    // candidate lists of neighbor-list rebuilds (Angstrom; <= 0: off)
+  double skin = 4.0;    // candidate lists of neighbor-list rebuilds (Angstrom; <= 0: off)
   size_t reserve_beads = 0, reserve_chains = 0;
   Tables* d_T = nullptr;
   std::vector<HostChain> chains;
