@@ -256,7 +256,11 @@ int upload_at(sopg_ctx* c, T* d, size_t at, const std::vector<T>& h) {
 int grow_pool(sopg_ctx* c, int pi, size_t need) {
   Pool& P = c->pool[pi];
   if (need <= P.cap) return SOPG_OK;
-  const size_t ncap = std::max(need, P.cap * 2 + 16);
+  size_t ncap = std::max(need, P.cap * 2 + 16);
+  if (pi == P_BEAD) {
+    if (need > kMaxPoolBeads) return fail(c, SOPG_ERR_ARG, "bead pool limit (2^28 beads per context) exceeded");
+    ncap = std::min(ncap, kMaxPoolBeads);
+  }
   HIP_OK(c, hipStreamSynchronize(c->stream));
   for (Arr& a : c->arrs[pi]) {
     void* q = nullptr;
@@ -310,7 +314,7 @@ void register_arrays(sopg_ctx* c) {
   for (double** p : {&D.am_r2, &D.am_nlrc2}) add(P_ATT, p, 8, 1);
   add(P_ATT, &D.am_rf, 4, 1);
   add(P_TINC, &D.att_code, 4, 1); add(P_TINC, &D.att_partner, 4, 1); add(P_TINC, &D.att_flag, 1, 1);
-  add(P_TINC, &D.att_k, sizeof(AttK), 1);
+  add(P_TINC, &D.att_w, 4, 1); add(P_TINC, &D.att_r, 8, 1);
   add(P_REP, &D.rm_i, 4, 1); add(P_REP, &D.rm_j, 4, 1); add(P_REP, &D.rm_t, 1, 1);
   add(P_RINC, &D.rep_code, 4, 1);
   add(P_CHAIN, &D.cp, sizeof(ChainP), 1);
@@ -516,40 +520,38 @@ int materialize(sopg_ctx* c, int id) {
   // natives
   build_csr(n, h.ai, h.aj, h.at[P_ATT], h.at[P_TINC], off, beg, end, code);
   std::vector<int8_t> at(nt);
-  std::vector<double> r2(nt), r6(nt), r12(nt), nlrc2(nt), plrc2(nt);
+  std::vector<double> r2(nt), nlrc2(nt);
   std::vector<float> rf(nt);
   for (size_t k = 0; k < nt; k++) {
     at[k] = (int8_t)(h.ait[k] | (h.ajt[k] << 2));
     const double r = h.ar[k];
-    // params.cpp:203-208 and neighbor_list.cpp:44-47, sop.cpp:506-507
+    // params.cpp:203-204 and neighbor_list.cpp:44-47 (sigma^6, sigma^12 and
+    // the pair-list cutoff are computed from r on the device, nat_powers /
+    // nat_plrc2)
     r2[k] = r * r;
-    r6[k] = r2[k] * r2[k] * r2[k];
-    r12[k] = r6[k] * r6[k];
-    const double rn = 3.2 * r, rp = 2.5 * r;
+    const double rn = 3.2 * r;
     nlrc2[k] = rn * rn;
-    plrc2[k] = rp * rp;
     rf[k] = (float)r;
   }
-  // per incidence: partner, and the pair's constants in the bead's own order
+  // per incidence: partner, and the pair's word and distance in the bead's own order
   const size_t nq = code.size();
   std::vector<int> partner(nq), q0(nt), q1(nt);
-  std::vector<AttK> qk(nq);
+  std::vector<uint32_t> aw(nq);
+  std::vector<double> ar(nq);
   for (size_t q = 0; q < nq; q++) {
     const size_t k = (size_t)(code[q] >> 1) - h.at[P_ATT];
-    partner[q] = off + ((code[q] & 1) ? h.ai[k] : h.aj[k]);
+    const int local = (code[q] & 1) ? h.ai[k] : h.aj[k];
+    partner[q] = off + local;
     ((code[q] & 1) ? q1 : q0)[k] = (int)(h.at[P_TINC] + q);
-    qk[q].r6 = r6[k];
-    qk[q].r12 = r12[k];
-    qk[q].plrc2 = plrc2[k];
-    qk[q].t = at[k] | ((code[q] & 1) << 4);
-    qk[q].pad = 0;
+    aw[q] = (uint32_t)local | ((uint32_t)(at[k] | ((code[q] & 1) << 4)) << kAttTShift);
+    ar[q] = h.ar[k];
   }
   h.q0.assign(q0.begin(), q0.end());
   h.q1.assign(q1.begin(), q1.end());
   const size_t ta = h.at[P_ATT], tq = h.at[P_TINC];
   if ((rc = upload_at(c, D.att_beg, off, beg)) || (rc = upload_at(c, D.att_end, off, end)) ||
       (rc = upload_at(c, D.att_code, tq, code)) || (rc = upload_at(c, D.att_partner, tq, partner)) ||
-      (rc = upload_at(c, D.att_k, tq, qk)) ||
+      (rc = upload_at(c, D.att_w, tq, aw)) || (rc = upload_at(c, D.att_r, tq, ar)) ||
       (rc = upload_at(c, D.am_i, ta, glob(h.ai))) || (rc = upload_at(c, D.am_j, ta, glob(h.aj))) ||
       (rc = upload_at(c, D.am_q0, ta, q0)) || (rc = upload_at(c, D.am_t, ta, at)) ||
       (rc = upload_at(c, D.am_r2, ta, r2)) || (rc = upload_at(c, D.am_nlrc2, ta, nlrc2)) ||

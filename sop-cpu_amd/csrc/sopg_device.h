@@ -97,7 +97,21 @@ SOPG_HD void philox_gauss2(uint64_t seed, uint64_t ctr, uint32_t bead, uint32_t 
 //   bits 26-27 itype, bits 28-29 jtype
 //   bits 30-31 pair-list membership, one bit per parity (double-buffered)
 constexpr uint32_t kPartnerMask = 0x1FFFFFFu;
+// Native-contact constants from the pair's native distance r, computed as
+// params.cpp:204-208 and sop.cpp:506-507 do (left to right, no contraction).
+SOPG_HD void nat_powers(double r, double& r6, double& r12) {
+  const double r2 = r * r;
+  r6 = r2 * r2 * r2;
+  r12 = r6 * r6;
+}
+SOPG_HD double nat_plrc2(double r) {
+  const double rp = 2.5 * r;
+  return rp * rp;
+}
 constexpr int kMaxChainBeads = 1 << 25;
+// Beads in one context's pool: byte offsets into a bead array (8 B per
+// bead) stay below 2^32, as the kernels' buffer loads take them.
+constexpr size_t kMaxPoolBeads = (size_t)1 << 28;
 SOPG_HD uint32_t slot_make(uint32_t partner, uint32_t side, uint32_t it, uint32_t jt) {
   return partner | (side << 25) | (it << 26) | (jt << 28);
 }

@@ -75,13 +75,10 @@ struct GridP {
   int pad;
 };
 
-// Constants of one native-contact incidence that the step kernel reads, in
-// one 32-byte record (two 16-byte loads): sigma^6, sigma^12, (2.5 sigma)^2 and
-// t = itype | jtype << 2 | side << 4.
-struct alignas(32) AttK {
-  double r6, r12, plrc2;
-  int32_t t, pad;
-};
+// Word of one native-contact incidence that the step kernel reads:
+//   bits  0-24 partner bead (chain-local, 0-based)
+//   bits 25-29 t = itype | jtype << 2 | side << 4 (side 0: this bead is i)
+constexpr int kAttTShift = 25;
 
 // Per-bead reduction fields (red[g * kRed + field]).
 enum { R_EBND = 0, R_EANG, R_EATT, R_EREP, R_CONT, R_RG, R_V2, R_NLA, R_PLA, R_NLR, R_PLR, kRed = 12 };
@@ -103,10 +100,12 @@ struct Dev {
   // angles (i, k) pairs, per incidence: partner << 1 | side (0: bead is i, 1: bead is k)
   int *ang_beg, *ang_end, *ang_pq;
   // natives: code = master << 1 | side; partner = the other bead (global).
-  // Per incidence also the pair's constants the step kernel reads (AttK), so
-  // that a bead's lanes load them contiguously.
+  // Per incidence also what the step kernel reads, so that a bead's lanes load
+  // it contiguously: the packed partner / type word (kAttTShift) and the
+  // pair's native distance sigma, from which sigma^6, sigma^12 and the
+  // pair-list cutoff (2.5 sigma)^2 follow with the host's own arithmetic.
   int *att_beg, *att_end, *att_code, *att_partner; uint8_t* att_flag;  // flag: bit0 NL, bit1+p PL(parity p)
-  AttK* att_k;
+  uint32_t* att_w; double* att_r;
   // master native entries (am_t = itype | jtype << 2; am_q0 = incidence of the first bead)
   int *am_i, *am_j, *am_q0; int8_t* am_t;
   double *am_r2, *am_nlrc2; float* am_rf;

@@ -148,6 +148,29 @@ __device__ __noinline__ bool rep_tol_skip(const uint8_t* flag, const int* __rest
   return false;
 }
 
+// Raw buffer view of a device array: loads through it take a 32-bit byte
+// offset, so the three coordinate gathers of one partner share one offset
+// register instead of three 64-bit addresses.  Dword 3 is the untyped
+// 32-bit-data form of this family; bead arrays stay below 2^32 bytes (the
+// bead pool is capped at kMaxPoolBeads).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_view(const double* p) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, -1, 0x00020000);
+}
+__device__ __forceinline__ double buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, 0, 0));
+}
+
+// A lane's force term cs * d, or -0.0 (which adds nothing in the folds) when
+// the lane has none: the product's sign bit is set on such lanes, whose cs is
+// 0 and whose product is therefore +0 or -0.
+__device__ __forceinline__ void neutral3(bool c, double cs, double dx, double dy, double dz, double& fx, double& fy,
+                                         double& fz) {
+  const uint64_t nz = c ? 0ull : 0x8000000000000000ull;
+  fx = __longlong_as_double(__double_as_longlong(cs * dx) | nz);
+  fy = __longlong_as_double(__double_as_longlong(cs * dy) | nz);
+  fz = __longlong_as_double(__double_as_longlong(cs * dz) | nz);
+}
+
 // Pair displacement u[j] - u[i] seen from bead g, min-imaged, all three
 // components at once: c is g's own position, o the partner's.  The reference
 // computes o - c when g is the pair's i bead (side 0) and c - o when it is j
@@ -229,8 +252,8 @@ __device__ __forceinline__ void rebuild_scan_bead(const Dev& D, const Tables* __
     const double dz = mimg(uz[j] - uz[i], boxl, qbox);
     const double d2 = dx * dx + dy * dy + dz * dz;
     bool nl, pl;
-    if (crit == SOPG_CUTOFF_NEIGHBOR) { nl = d2 < D.am_nlrc2[k]; pl = nl && d2 < D.att_k[q].plrc2; }
-    else if (crit == SOPG_CUTOFF_CELL) { nl = cell_adjacent(D, P, i, j); pl = nl && d2 < D.att_k[q].plrc2; }
+    if (crit == SOPG_CUTOFF_NEIGHBOR) { nl = d2 < D.am_nlrc2[k]; pl = nl && d2 < nat_plrc2(D.att_r[q]); }
+    else if (crit == SOPG_CUTOFF_CELL) { nl = cell_adjacent(D, P, i, j); pl = nl && d2 < nat_plrc2(D.att_r[q]); }
     else { nl = true; pl = true; }
     D.att_flag[q] = (uint8_t)((nl ? 1 : 0) | (pl ? (2 << par) : 0));
   }
@@ -567,8 +590,8 @@ __global__ void __launch_bounds__(256) k_rebuild_grid(Dev D, const Tables* __res
       const double dz = mimg(uz[j] - uz[i], boxl, qbox);
       const double d2 = dx * dx + dy * dy + dz * dz;
       bool nl, pl;
-      if (crit == SOPG_CUTOFF_NEIGHBOR) { nl = d2 < D.am_nlrc2[k]; pl = nl && d2 < D.att_k[q].plrc2; }
-      else { nl = cell_adjacent(D, P, i, j); pl = nl && d2 < D.att_k[q].plrc2; }
+      if (crit == SOPG_CUTOFF_NEIGHBOR) { nl = d2 < D.am_nlrc2[k]; pl = nl && d2 < nat_plrc2(D.att_r[q]); }
+      else { nl = cell_adjacent(D, P, i, j); pl = nl && d2 < nat_plrc2(D.att_r[q]); }
       D.att_flag[q] = (uint8_t)((nl ? 1 : 0) | (pl ? (2 << par) : 0));
     }
     for (int w = l; w < words; w += LG) { nlb[w] = 0u; plb[w] = 0u; }
@@ -746,7 +769,7 @@ __global__ void __launch_bounds__(256) k_cand_filter(Dev D, const Tables* __rest
     const double dy = mimg(uy[j] - uy[i], boxl, qbox);
     const double dz = mimg(uz[j] - uz[i], boxl, qbox);
     const double d2 = dx * dx + dy * dy + dz * dz;
-    const bool nl = d2 < D.am_nlrc2[k], pl = nl && d2 < D.att_k[q].plrc2;
+    const bool nl = d2 < D.am_nlrc2[k], pl = nl && d2 < nat_plrc2(D.att_r[q]);
     D.att_flag[q] = (uint8_t)((nl ? 1 : 0) | (pl ? (2 << par) : 0));
   }
   const uint32_t pbit = 1u << (30 + par);
@@ -798,7 +821,7 @@ __device__ __forceinline__ void pairfilter_bead(const Dev& D, const Tables* __re
       const double dy = mimg(uy[j] - uy[i], boxl, qbox);
       const double dz = mimg(uz[j] - uz[i], boxl, qbox);
       const double d2 = dx * dx + dy * dy + dz * dz;
-      pl = (P.cutoff == SOPG_CUTOFF_NONE) ? true : d2 < D.att_k[q].plrc2;
+      pl = (P.cutoff == SOPG_CUTOFF_NONE) ? true : d2 < nat_plrc2(D.att_r[q]);
     }
     D.att_flag[q] = (uint8_t)((f & ~pbit) | (pl ? pbit : 0));
   }
@@ -897,6 +920,45 @@ __device__ __forceinline__ void load_type_tables(const Tables* __restrict__ T, d
 // The step for bead g, computed by the L lanes of its group (lane l); the
 // body of k_force (SKIP_RES: chains the resident kernel advances are left
 // alone) and of the resident kernel.
+// The chain fields the step kernel reads.  They are loaded through the
+// scalar cache when every lane of the wave is on one chain (almost every
+// wave: chains are longer than a wave's beads), so that they cost no vector
+// memory (texture data) cycles, which bound the step kernel; lanes on
+// different chains load their own.
+typedef __attribute__((address_space(4))) const ChainP ConstChainP;  // read through the scalar cache
+struct ChainV {
+  int off, att_off, natt, cutoff, noise_mode, iup0, nup, resident;
+  double boxl, qbox, thr_att, var;
+  uint64_t seed, noise_base;
+  template <class P_>
+  __device__ __forceinline__ void load(P_* __restrict__ P) {
+    off = P->off; att_off = P->att_off; natt = P->natt; cutoff = P->cutoff; noise_mode = P->noise_mode;
+    iup0 = P->iup0; nup = P->nup; resident = P->resident;
+    boxl = P->boxl; qbox = P->qbox; thr_att = P->thr_att; var = P->var;
+    seed = P->seed; noise_base = P->noise_base;
+  }
+};
+// ... and the integration's, read in the epilogue
+struct ChainI {
+  int vel_on, sim_type;
+  double a1, a2, a3, a4, a5;
+  template <class P_>
+  __device__ __forceinline__ void load(P_* __restrict__ P) {
+    vel_on = P->vel_on; sim_type = P->sim_type;
+    a1 = P->a1; a2 = P->a2; a3 = P->a3; a4 = P->a4; a5 = P->a5;
+  }
+};
+template <class V>
+__device__ __forceinline__ V wave_chain_fields(const ChainP* __restrict__ cp, int chain) {
+  V v;
+  const int c0 = __builtin_amdgcn_readfirstlane(chain);
+  if (__builtin_amdgcn_ballot_w64(chain != c0) == 0)
+    v.load((ConstChainP*)(cp + c0));
+  else
+    v.load(cp + chain);
+  return v;
+}
+
 template <int L, bool NEXT, bool SKIP_RES = false>
 __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restrict__ T, const double (*tp)[TP_N],
                                            int cur, int pr, int rel, int flags, const double* __restrict__ noise,
@@ -908,17 +970,19 @@ __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restric
   const double* __restrict__ uy = cur ? D.uy[1] : D.uy[0];
   const double* __restrict__ uz = cur ? D.uz[1] : D.uz[0];
   const double cx = ux[g], cy = uy[g], cz = uz[g];
+  const __amdgpu_buffer_rsrc_t bux = buf_view(ux), buy = buf_view(uy), buz = buf_view(uz);
   const int bond0 = D.bond_beg[g], bond1 = D.bond_end[g], ang0 = D.ang_beg[g], ang1 = D.ang_end[g];
   const int att0 = D.att_beg[g], att1 = D.att_end[g], nslot = D.cnt[g];
   if (chain < 0) return;  // a free bead
-  const ChainP* Pp = D.cp + chain;
-  // chain fields the loops use, in registers (the loops' stores could alias them)
-  const int off = Pp->off, att_off = Pp->att_off, natt = Pp->natt;
-  const int cutoff = Pp->cutoff, noise_mode = Pp->noise_mode, iup0 = Pp->iup0, nup = Pp->nup;
-  const int resident = Pp->resident;
-  const double boxl = Pp->boxl, qbox = Pp->qbox, thr_att = Pp->thr_att, var = Pp->var;
-  const uint64_t seed = Pp->seed, noise_base = Pp->noise_base;
+  // chain fields, in registers (the loops' stores could alias them)
+  const ChainV cv = wave_chain_fields<ChainV>(D.cp, chain);
+  const int off = cv.off, att_off = cv.att_off, natt = cv.natt;
+  const int cutoff = cv.cutoff, noise_mode = cv.noise_mode, iup0 = cv.iup0, nup = cv.nup;
+  const int resident = cv.resident;
+  const double boxl = cv.boxl, qbox = cv.qbox, thr_att = cv.thr_att, var = cv.var;
+  const uint64_t seed = cv.seed, noise_base = cv.noise_base;
   if (SKIP_RES && resident) return;
+  const uint32_t uoff = (uint32_t)off;  // partner indices unsigned: no sign extension in addresses
   const bool no_cut = cutoff == SOPG_CUTOFF_NONE;
   const int pw = pr ^ 1;
   const bool energy = (flags & F_ENERGY_ALL) || ((flags & F_ENERGY) && ((iup0 + rel) % nup == 0));
@@ -977,11 +1041,11 @@ __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restric
       const int q = base + l;
       const bool has = q < ntot, isb = q < nbq;
       // -0.0: what a lane without a term adds in the folds
-      double fx = -0.0, fy = -0.0, fz = -0.0, e_b = -0.0, e_a = -0.0;
+      double cs = 0.0, dx = 0.0, dy = 0.0, dz = 0.0, e_b = -0.0, e_a = -0.0;
       bool ec = false;
       if (has) {
         const int pq = isb ? D.bond_pq[b0 + q] : D.ang_pq[a0 + q - nbq], o = pq >> 1, side = pq & 1;
-        double dx, dy, dz;
+        const double sg = side ? -1.0 : 1.0;
         pair_d3(side, cx, cy, cz, ux[o], uy[o], uz[o], boxl, qbox, dx, dy, dz);
         const double d = sqrt(dx * dx + dy * dy + dz * dz);
         double co1;
@@ -994,9 +1058,11 @@ __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restric
           co1 = T->f_ang_ss_coeff / pow8_cr(d);
           if (energy && side == 0) { e_a = T->e_ang_ss_coeff / pow6_cr(d); ec = true; }
         }
-        fx = co1 * dx; fy = co1 * dy; fz = co1 * dz;
-        if (side == 0) { fx = -fx; fy = -fy; fz = -fz; }
+        // side 0 negates the force: (-co1 * 1) * d is -(co1 * d) bit for bit
+        cs = -co1 * sg;
       }
+      double fx, fy, fz;
+      neutral3(has, cs, dx, dy, dz, fx, fy, fz);
       fold3<L>(Fx, Fy, Fz, has, fx, fy, fz);
       if (energy) {
         fold1<L>(eb, ec && isb, e_b);
@@ -1009,51 +1075,59 @@ __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restric
     const int p0 = att0, p1 = att1;
     // software pipeline as in the non-native loop below
     uint8_t fn = 0, fn2 = 0;
-    int on2 = 0;
+    uint32_t an = 0u, an2 = 0u;
     double pxn = 0.0, pyn = 0.0, pzn = 0.0;
     if (p0 + l < p1) {
       fn = D.att_flag[p0 + l];
-      const int o = D.att_partner[p0 + l];
-      pxn = ux[o]; pyn = uy[o]; pzn = uz[o];
+      an = D.att_w[p0 + l];
+      const uint32_t ob = (uoff + (an & kPartnerMask)) * 8u;
+      pxn = buf_ld(bux, ob); pyn = buf_ld(buy, ob); pzn = buf_ld(buz, ob);
     }
-    if (p0 + l + L < p1) { fn2 = D.att_flag[p0 + l + L]; on2 = D.att_partner[p0 + l + L]; }
+    if (p0 + l + L < p1) { fn2 = D.att_flag[p0 + l + L]; an2 = D.att_w[p0 + l + L]; }
     for (int base = p0; base < p1; base += L) {
       const int q = base + l;
       const uint8_t f = fn;
+      const uint32_t aw = an;
       const double px = pxn, py = pyn, pz = pzn;
-      if (q + L < p1) { pxn = ux[on2]; pyn = uy[on2]; pzn = uz[on2]; }
+      if (q + L < p1) {
+        const uint32_t ob = (uoff + (an2 & kPartnerMask)) * 8u;
+        pxn = buf_ld(bux, ob); pyn = buf_ld(buy, ob); pzn = buf_ld(buz, ob);
+      }
       fn = fn2;
-      if (q + 2 * L < p1) { fn2 = D.att_flag[q + 2 * L]; on2 = D.att_partner[q + 2 * L]; }
+      an = an2;
+      if (q + 2 * L < p1) { fn2 = D.att_flag[q + 2 * L]; an2 = D.att_w[q + 2 * L]; }
       bool contrib = false, ec = false;
-      double fx = -0.0, fy = -0.0, fz = -0.0, e = -0.0;  // -0.0: adds nothing in the folds
+      double cs = 0.0, dx = 0.0, dy = 0.0, dz = 0.0, e = -0.0;
       if (q < p1) {
         if (f & 1) {
-          const AttK K = D.att_k[q];
-          const int tt = K.t, side = (tt >> 4) & 1, tpi = (tt & 3) * 3 + ((tt >> 2) & 3);
-          double dx, dy, dz;
+          const double r = D.att_r[q];
+          const int tt = (int)(aw >> kAttTShift), side = (tt >> 4) & 1, tpi = (tt & 3) * 3 + ((tt >> 2) & 3);
+          const double sg = side ? -1.0 : 1.0;
           pair_d3(side, cx, cy, cz, px, py, pz, boxl, qbox, dx, dy, dz);
           const double d2 = dx * dx + dy * dy + dz * dz;
           if ((f >> (1 + pr)) & 1) {
-            const double r6 = K.r6, r12 = K.r12;
+            double r6, r12;
+            nat_powers(r, r6, r12);
             const double d6 = d2 * d2 * d2;
             const double d12 = d6 * d6;
             const bool skip = (d2 < thr_att) &&
                               att_tol_skip(D.att_flag, D.am_q0, D.am_r2, att_off, D.att_code[q] >> 1, pr, d2);
             if (!skip) {
-              const double co1 = tp[tpi][TP_FATT] / d2 * ((r12 / d12) - (r6 / d6));
-              fx = co1 * dx; fy = co1 * dy; fz = co1 * dz;
-              if (side == 1) { fx = -fx; fy = -fy; fz = -fz; }
+              // side 1 negates the force: (co1 * -1) * d is -(co1 * d) bit for bit
+              cs = (tp[tpi][TP_FATT] / d2 * ((r12 / d12) - (r6 / d6))) * sg;
               contrib = true;
             }
             if (energy && side == 0) { e = tp[tpi][TP_CATT] * (r12 / d12) - 2.0 * (r6 / d6); ec = true; }
           }
           if constexpr (NEXT) {
-            const bool npl = no_cut || d2 < K.plrc2;
+            const bool npl = no_cut || d2 < nat_plrc2(r);
             const uint8_t nf = (uint8_t)((f & ~(2 << pw)) | (npl ? (2 << pw) : 0));
             if (nf != f) D.att_flag[q] = nf;
           }
         }
       }
+      double fx, fy, fz;
+      neutral3(contrib, cs, dx, dy, dz, fx, fy, fz);
       fold3<L>(Fx, Fy, Fz, contrib, fx, fy, fz);
       if (energy) fold1<L>(eatt, ec, e);
     }
@@ -1071,8 +1145,8 @@ __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restric
     double pxn = 0.0, pyn = 0.0, pzn = 0.0;
     if (l < cnt) {
       wn = S[l];
-      const int o = off + (int)(wn & kPartnerMask);
-      pxn = ux[o]; pyn = uy[o]; pzn = uz[o];
+      const uint32_t ob = (uoff + (wn & kPartnerMask)) * 8u;
+      pxn = buf_ld(bux, ob); pyn = buf_ld(buy, ob); pzn = buf_ld(buz, ob);
     }
     if (l + L < cnt) wn2 = S[l + L];
     for (int base = 0; base < cnt; base += L) {
@@ -1080,41 +1154,43 @@ __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restric
       const uint32_t w = wn;
       const double px = pxn, py = pyn, pz = pzn;
       if (q + L < cnt) {
-        const int o = off + (int)(wn2 & kPartnerMask);
-        pxn = ux[o]; pyn = uy[o]; pzn = uz[o];
+        const uint32_t ob = (uoff + (wn2 & kPartnerMask)) * 8u;
+        pxn = buf_ld(bux, ob); pyn = buf_ld(buy, ob); pzn = buf_ld(buz, ob);
       }
       wn = wn2;
       if (q + 2 * L < cnt) wn2 = S[q + 2 * L];
+      // every lane evaluates the displacement (a lane past the list end holds
+      // finite stale coordinates); the branches cover only what needs the entry
+      const bool in = q < cnt;
+      const int side = (w >> 25) & 1;
+      const double sg = side ? -1.0 : 1.0;
+      const int tpi = (int)((w >> 26) & 3) * 3 + (int)((w >> 28) & 3);
+      double dx, dy, dz;
+      pair_d3(side, cx, cy, cz, px, py, pz, boxl, qbox, dx, dy, dz);
+      const double d2 = dx * dx + dy * dy + dz * dz;
+      // the type pair's constants, read together (one LDS round trip)
+      const double frep = tp[tpi][TP_FREP], s6 = tp[tpi][TP_S6], s12 = tp[tpi][TP_S12];
+      const double plrc2 = tp[tpi][TP_PLRC2];
       bool contrib = false, ec = false;
-      double fx = -0.0, fy = -0.0, fz = -0.0, e = -0.0;  // -0.0: adds nothing in the folds
-      if (q < cnt) {
-        const int side = (w >> 25) & 1;
-        const int tpi = (int)((w >> 26) & 3) * 3 + (int)((w >> 28) & 3);
-        double dx, dy, dz;
-        pair_d3(side, cx, cy, cz, px, py, pz, boxl, qbox, dx, dy, dz);
-        const double d2 = dx * dx + dy * dy + dz * dz;
-        // the type pair's constants, read together (one LDS round trip)
-        const double frep = tp[tpi][TP_FREP], s6 = tp[tpi][TP_S6], s12 = tp[tpi][TP_S12];
-        const double plrc2 = tp[tpi][TP_PLRC2];
-        if (w & rbit) {
-          const double d6 = d2 * d2 * d2;
-          const double d12 = d6 * d6;
-          const bool skip =
-              (d2 < thr_rep) && rep_tol_skip(D.att_flag, D.am_q0, D.am_t, T, att_off, natt, pr, d2);
-          if (!skip) {
-            const double co1 = frep / d2 * (2.0 * s12 / d12 + s6 / d6);
-            fx = co1 * dx; fy = co1 * dy; fz = co1 * dz;
-            if (side == 1) { fx = -fx; fy = -fy; fz = -fz; }
-            contrib = true;
-          }
-          if (energy && side == 0) { e = tp[tpi][TP_CREP] * (s12 / d12 + s6 / d6); ec = true; }
+      double cs = 0.0, e = -0.0;
+      if (in && (w & rbit)) {
+        const double d6 = d2 * d2 * d2;
+        const double d12 = d6 * d6;
+        const bool skip = (d2 < thr_rep) && rep_tol_skip(D.att_flag, D.am_q0, D.am_t, T, att_off, natt, pr, d2);
+        if (!skip) {
+          // side 1 negates the force: (co1 * -1) * d is -(co1 * d) bit for bit
+          cs = (frep / d2 * (2.0 * s12 / d12 + s6 / d6)) * sg;
+          contrib = true;
         }
-        if constexpr (NEXT) {
-          const bool npl = no_cut || d2 < plrc2;
-          const uint32_t nw = (w & ~wbit) | (npl ? wbit : 0u);
-          if (nw != w) S[q] = nw;
-        }
+        if (energy && side == 0) { e = tp[tpi][TP_CREP] * (s12 / d12 + s6 / d6); ec = true; }
       }
+      if constexpr (NEXT) {
+        const bool npl = no_cut || d2 < plrc2;
+        const uint32_t nw = (w & ~wbit) | (npl ? wbit : 0u);
+        if (in && nw != w) S[q] = nw;
+      }
+      double fx, fy, fz;
+      neutral3(contrib, cs, dx, dy, dz, fx, fy, fz);
       fold3<L>(Fx, Fy, Fz, contrib, fx, fy, fz);
       if (energy) fold1<L>(erep, ec, e);
     }
@@ -1128,8 +1204,9 @@ __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restric
   if (!(flags & F_STORE)) return;
   // every load of the integration first (one memory round trip), then the
   // arithmetic and the stores
-  const int vel_on = Pp->vel_on, sim = Pp->sim_type;
-  const double a1 = Pp->a1, a2 = Pp->a2, a3 = Pp->a3, a4 = Pp->a4, a5 = Pp->a5;
+  const ChainI ci = wave_chain_fields<ChainI>(D.cp, chain);
+  const int vel_on = ci.vel_on, sim = ci.sim_type;
+  const double a1 = ci.a1, a2 = ci.a2, a3 = ci.a3, a4 = ci.a4, a5 = ci.a5;
   const double vx0 = D.vx[g], vy0 = D.vy[g], vz0 = D.vz[g];
   const double ix0 = D.ix[g], iy0 = D.iy[g], iz0 = D.iz[g];
   double psx = 0.0, psy = 0.0, psz = 0.0;
