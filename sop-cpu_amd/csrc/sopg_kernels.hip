@@ -46,6 +46,13 @@ __device__ __forceinline__ double group_bcast(double v) {
   return __builtin_bit_cast(double, (uint64_t)lo | ((uint64_t)hi << 32));
 }
 
+template <int K, int L>
+__device__ __forceinline__ int group_bcast_i(int v) {
+  constexpr int s01 = K, s23 = (L == 4) ? K : 2 + K;
+  constexpr int ctrl = s01 | s01 << 2 | s23 << 4 | s23 << 6;
+  return __builtin_amdgcn_mov_dpp(v, ctrl, 0xF, 0xF, false);
+}
+
 // F += contributions of the group's lanes, in lane order (= list order).
 // For L = 2 and 4 every lane is added in turn: callers leave -0.0 in the
 // values of a lane that does not contribute (c false), and x + (-0.0) is x
@@ -965,14 +972,32 @@ __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restric
                                            int g, int l) {
   // loads in two batches: bead-indexed ones (any bead below the high-water
   // mark is valid memory), then the chain's fields
-  const int chain = D.chain[g];
   const double* __restrict__ ux = cur ? D.ux[1] : D.ux[0];
   const double* __restrict__ uy = cur ? D.uy[1] : D.uy[0];
   const double* __restrict__ uz = cur ? D.uz[1] : D.uz[0];
-  const double cx = ux[g], cy = uy[g], cz = uz[g];
   const __amdgpu_buffer_rsrc_t bux = buf_view(ux), buy = buf_view(uy), buz = buf_view(uz);
-  const int bond0 = D.bond_beg[g], bond1 = D.bond_end[g], ang0 = D.ang_beg[g], ang1 = D.ang_end[g];
-  const int att0 = D.att_beg[g], att1 = D.att_end[g], nslot = D.cnt[g];
+  int chain, bond0, bond1, ang0, ang1, att0, att1, nslot;
+  double cx, cy, cz;
+  if constexpr (L == 4) {
+    // the group's four lanes share the bead's fields: each lane loads two of
+    // the eight index fields and one coordinate, and the group exchanges them
+    // (a quarter of the vector-memory data of every lane loading every field)
+    const int* __restrict__ pa = l == 0 ? D.chain : l == 1 ? D.bond_end : l == 2 ? D.ang_end : D.att_end;
+    const int* __restrict__ pb = l == 0 ? D.bond_beg : l == 1 ? D.ang_beg : l == 2 ? D.att_beg : D.cnt;
+    const double* __restrict__ pc = l == 0 ? ux : l == 1 ? uy : uz;
+    const int va = pa[g], vb = pb[g];
+    const double vc = pc[g];
+    chain = group_bcast_i<0, 4>(va); bond0 = group_bcast_i<0, 4>(vb);
+    bond1 = group_bcast_i<1, 4>(va); ang0 = group_bcast_i<1, 4>(vb);
+    ang1 = group_bcast_i<2, 4>(va); att0 = group_bcast_i<2, 4>(vb);
+    att1 = group_bcast_i<3, 4>(va); nslot = group_bcast_i<3, 4>(vb);
+    cx = group_bcast<0, 4>(vc); cy = group_bcast<1, 4>(vc); cz = group_bcast<2, 4>(vc);
+  } else {
+    chain = D.chain[g];
+    cx = ux[g]; cy = uy[g]; cz = uz[g];
+    bond0 = D.bond_beg[g]; bond1 = D.bond_end[g]; ang0 = D.ang_beg[g]; ang1 = D.ang_end[g];
+    att0 = D.att_beg[g]; att1 = D.att_end[g]; nslot = D.cnt[g];
+  }
   if (chain < 0) return;  // a free bead
   // chain fields, in registers (the loops' stores could alias them)
   const ChainV cv = wave_chain_fields<ChainV>(D.cp, chain);
