@@ -1674,7 +1674,9 @@ int sopg_run(sopg_ctx* c, long nsteps, const double* noise, sopg_record* rec, lo
     const bool last = (r == nsteps - 1);
     const double* nz = noise ? c->d_noise + (size_t)r * 3 * nb : nullptr;
     int pe = prof_begin(c);
-    launch_force(L, Dx, c->d_T, c->cur, c->par, k, F_STORE | F_VEL | F_ENERGY | F_ALLTERMS, !last, nz, 0, nb, s);
+    // energy terms only in a step at which some chain records
+    const int fe = records(r) ? F_ENERGY : 0;
+    launch_force(L, Dx, c->d_T, c->cur, c->par, k, F_STORE | F_VEL | fe | F_ALLTERMS, !last, nz, 0, nb, s);
     prof_end(c, SOPG_KERNEL_STEP, pe);
     if (records(r)) {
       pe = prof_begin(c);

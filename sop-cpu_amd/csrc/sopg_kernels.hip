@@ -966,7 +966,9 @@ __device__ __forceinline__ V wave_chain_fields(const ChainP* __restrict__ cp, in
   return v;
 }
 
-template <int L, bool NEXT, bool SKIP_RES = false>
+// EN false: the launch records no energy (no chain's record step), and the
+// energy terms are compiled out.
+template <int L, bool NEXT, bool SKIP_RES = false, bool EN = true>
 __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restrict__ T, const double (*tp)[TP_N],
                                            int cur, int pr, int rel, int flags, const double* __restrict__ noise,
                                            int g, int l) {
@@ -1010,7 +1012,7 @@ __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restric
   const uint32_t uoff = (uint32_t)off;  // partner indices unsigned: no sign extension in addresses
   const bool no_cut = cutoff == SOPG_CUTOFF_NONE;
   const int pw = pr ^ 1;
-  const bool energy = (flags & F_ENERGY_ALL) || ((flags & F_ENERGY) && ((iup0 + rel) % nup == 0));
+  const bool energy = EN && ((flags & F_ENERGY_ALL) || ((flags & F_ENERGY) && ((iup0 + rel) % nup == 0)));
   double eb = 0.0, ea = 0.0, eatt = 0.0, erep = 0.0;
 
   // clear_forces, then random_force (energy.cpp:35-45, 449-465):
@@ -1265,7 +1267,7 @@ __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restric
   }
 }
 
-template <int L, bool NEXT>
+template <int L, bool NEXT, bool EN>
 __global__ void __launch_bounds__(256, 5) k_force(Dev D, const Tables* __restrict__ T, int cur, int pr, int rel, int flags,
                                                const double* __restrict__ noise, int g0, int g1) {
   rel += *D.relb;
@@ -1275,7 +1277,7 @@ __global__ void __launch_bounds__(256, 5) k_force(Dev D, const Tables* __restric
   const int t = xcd_block() * blockDim.x + threadIdx.x;
   const int g = g0 + t / L;
   if (g >= g1) return;
-  force_bead<L, NEXT, true>(D, T, tp, cur, pr, rel, flags, noise, g, t & (L - 1));
+  force_bead<L, NEXT, true, EN>(D, T, tp, cur, pr, rel, flags, noise, g, t & (L - 1));
 }
 
 // Observables (sop.cpp:200-280) and list counts, per bead.
@@ -1457,10 +1459,15 @@ template <int L>
 static void launch_force_L(const Dev& D, const Tables* T, int cur, int pr, int rel, int flags, bool next,
                            const double* noise, int g0, int g1, hipStream_t s) {
   const dim3 grid(blocks_for((long)(g1 - g0) * L)), block(256);
-  if (next)
-    hipLaunchKernelGGL((k_force<L, true>), grid, block, 0, s, D, T, cur, pr, rel, flags, noise, g0, g1);
+  const bool en = (flags & (F_ENERGY | F_ENERGY_ALL)) != 0;
+  if (next && en)
+    hipLaunchKernelGGL((k_force<L, true, true>), grid, block, 0, s, D, T, cur, pr, rel, flags, noise, g0, g1);
+  else if (next)
+    hipLaunchKernelGGL((k_force<L, true, false>), grid, block, 0, s, D, T, cur, pr, rel, flags, noise, g0, g1);
+  else if (en)
+    hipLaunchKernelGGL((k_force<L, false, true>), grid, block, 0, s, D, T, cur, pr, rel, flags, noise, g0, g1);
   else
-    hipLaunchKernelGGL((k_force<L, false>), grid, block, 0, s, D, T, cur, pr, rel, flags, noise, g0, g1);
+    hipLaunchKernelGGL((k_force<L, false, false>), grid, block, 0, s, D, T, cur, pr, rel, flags, noise, g0, g1);
 }
 
 #define SOPG_DISPATCH_L(L, CALL)          \
