@@ -152,6 +152,10 @@ SOPG_HD double nat_plrc2(double r) {
   const double rp = 2.5 * r;
   return rp * rp;
 }
+SOPG_HD double nat_nlrc2(double r) {  // neighbor_list.cpp:44-47
+  const double rn = 3.2 * r;
+  return rn * rn;
+}
 constexpr int kMaxChainBeads = 1 << 25;
 // Beads in one context's pool: byte offsets into a bead array (8 B per
 // bead) stay below 2^32, as the kernels' buffer loads take them.

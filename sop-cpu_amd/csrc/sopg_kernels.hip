@@ -725,6 +725,54 @@ __global__ void __launch_bounds__(256) k_rebuild_grid(Dev D, const Tables* __res
   }
 }
 
+// The chain fields the step kernel and the candidate filter read.  They are
+// loaded through the scalar cache when every lane of the wave is on one chain
+// (almost every wave: chains are longer than a wave's beads), so that they
+// cost no vector memory (texture data) cycles, which bound both kernels;
+// lanes on different chains load their own.
+typedef __attribute__((address_space(4))) const ChainP ConstChainP;  // read through the scalar cache
+struct ChainV {
+  int off, att_off, natt, cutoff, noise_mode, iup0, nup, resident;
+  double boxl, qbox, thr_att, var;
+  uint64_t seed, noise_base;
+  template <class P_>
+  __device__ __forceinline__ void load(P_* __restrict__ P) {
+    off = P->off; att_off = P->att_off; natt = P->natt; cutoff = P->cutoff; noise_mode = P->noise_mode;
+    iup0 = P->iup0; nup = P->nup; resident = P->resident;
+    boxl = P->boxl; qbox = P->qbox; thr_att = P->thr_att; var = P->var;
+    seed = P->seed; noise_base = P->noise_base;
+  }
+};
+// ... the candidate filter's
+struct ChainF {
+  int off;
+  double boxl, qbox;
+  template <class P_>
+  __device__ __forceinline__ void load(P_* __restrict__ P) {
+    off = P->off; boxl = P->boxl; qbox = P->qbox;
+  }
+};
+// ... and the integration's, read in the epilogue
+struct ChainI {
+  int vel_on, sim_type;
+  double a1, a2, a3, a4, a5;
+  template <class P_>
+  __device__ __forceinline__ void load(P_* __restrict__ P) {
+    vel_on = P->vel_on; sim_type = P->sim_type;
+    a1 = P->a1; a2 = P->a2; a3 = P->a3; a4 = P->a4; a5 = P->a5;
+  }
+};
+template <class V>
+__device__ __forceinline__ V wave_chain_fields(const ChainP* __restrict__ cp, int chain) {
+  V v;
+  const int c0 = __builtin_amdgcn_readfirstlane(chain);
+  if (__builtin_amdgcn_ballot_w64(chain != c0) == 0)
+    v.load((ConstChainP*)(cp + c0));
+  else
+    v.load(cp + chain);
+  return v;
+}
+
 // Neighbor list from the candidate list (chains planned W_FILTER): the
 // candidates within the neighbor-list cutoff at the current positions, in
 // their (ascending, master) order, with the pair-list bit at the same d^2;
@@ -745,12 +793,12 @@ __global__ void __launch_bounds__(256) k_cand_filter(Dev D, const Tables* __rest
   if (g >= g1) return;
   const int c = D.chain[g];
   if (c < 0 || !(D.grid[c].work & W_FILTER)) return;
-  const ChainP& P = D.cp[c];
+  const ChainF cf = wave_chain_fields<ChainF>(D.cp, c);
   const double* ux = cur ? D.ux[1] : D.ux[0];
   const double* uy = cur ? D.uy[1] : D.uy[0];
   const double* uz = cur ? D.uz[1] : D.uz[0];
-  const double boxl = P.boxl, qbox = P.qbox;
-  const int off = P.off;
+  const double boxl = cf.boxl, qbox = cf.qbox;
+  const int off = cf.off;
   // non-natives (neighbor_list.cpp:97-103): stable compaction of the
   // candidates.  Software pipeline as in the step kernel (candidate words two
   // chunks ahead, partner coordinates one chunk ahead), started before the
@@ -767,16 +815,17 @@ __global__ void __launch_bounds__(256) k_cand_filter(Dev D, const Tables* __rest
     pxn = ux[o]; pyn = uy[o]; pzn = uz[o];
   }
   if (l + LF < n) wn2 = C[l + LF];
-  // natives (neighbor_list.cpp:17-67)
+  // natives (neighbor_list.cpp:17-67): the displacement from the bead's own
+  // position as the step kernel forms it, the cutoffs from sigma
   const int a0 = D.att_beg[g], a1 = D.att_end[g];
   for (int q = a0 + l; q < a1; q += LF) {
-    const int code = D.att_code[q], k = code >> 1, side = code & 1, o = D.att_partner[q];
-    const int i = side ? o : g, j = side ? g : o;
-    const double dx = mimg(ux[j] - ux[i], boxl, qbox);
-    const double dy = mimg(uy[j] - uy[i], boxl, qbox);
-    const double dz = mimg(uz[j] - uz[i], boxl, qbox);
+    const uint32_t aw = D.att_w[q];
+    const double r = D.att_r[q];
+    const int o = off + (int)(aw & kPartnerMask), side = (int)(aw >> (kAttTShift + 4)) & 1;
+    double dx, dy, dz;
+    pair_d3(side, cx, cy, cz, ux[o], uy[o], uz[o], boxl, qbox, dx, dy, dz);
     const double d2 = dx * dx + dy * dy + dz * dz;
-    const bool nl = d2 < D.am_nlrc2[k], pl = nl && d2 < nat_plrc2(D.att_r[q]);
+    const bool nl = d2 < nat_nlrc2(r), pl = nl && d2 < nat_plrc2(r);
     D.att_flag[q] = (uint8_t)((nl ? 1 : 0) | (pl ? (2 << par) : 0));
   }
   const uint32_t pbit = 1u << (30 + par);
@@ -927,44 +976,6 @@ __device__ __forceinline__ void load_type_tables(const Tables* __restrict__ T, d
 // The step for bead g, computed by the L lanes of its group (lane l); the
 // body of k_force (SKIP_RES: chains the resident kernel advances are left
 // alone) and of the resident kernel.
-// The chain fields the step kernel reads.  They are loaded through the
-// scalar cache when every lane of the wave is on one chain (almost every
-// wave: chains are longer than a wave's beads), so that they cost no vector
-// memory (texture data) cycles, which bound the step kernel; lanes on
-// different chains load their own.
-typedef __attribute__((address_space(4))) const ChainP ConstChainP;  // read through the scalar cache
-struct ChainV {
-  int off, att_off, natt, cutoff, noise_mode, iup0, nup, resident;
-  double boxl, qbox, thr_att, var;
-  uint64_t seed, noise_base;
-  template <class P_>
-  __device__ __forceinline__ void load(P_* __restrict__ P) {
-    off = P->off; att_off = P->att_off; natt = P->natt; cutoff = P->cutoff; noise_mode = P->noise_mode;
-    iup0 = P->iup0; nup = P->nup; resident = P->resident;
-    boxl = P->boxl; qbox = P->qbox; thr_att = P->thr_att; var = P->var;
-    seed = P->seed; noise_base = P->noise_base;
-  }
-};
-// ... and the integration's, read in the epilogue
-struct ChainI {
-  int vel_on, sim_type;
-  double a1, a2, a3, a4, a5;
-  template <class P_>
-  __device__ __forceinline__ void load(P_* __restrict__ P) {
-    vel_on = P->vel_on; sim_type = P->sim_type;
-    a1 = P->a1; a2 = P->a2; a3 = P->a3; a4 = P->a4; a5 = P->a5;
-  }
-};
-template <class V>
-__device__ __forceinline__ V wave_chain_fields(const ChainP* __restrict__ cp, int chain) {
-  V v;
-  const int c0 = __builtin_amdgcn_readfirstlane(chain);
-  if (__builtin_amdgcn_ballot_w64(chain != c0) == 0)
-    v.load((ConstChainP*)(cp + c0));
-  else
-    v.load(cp + chain);
-  return v;
-}
 
 // EN false: the launch records no energy (no chain's record step), and the
 // energy terms are compiled out.
