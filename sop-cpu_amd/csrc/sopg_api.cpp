@@ -306,6 +306,7 @@ void register_arrays(sopg_ctx* c) {
   add(P_BEAD, &D.type, 1, 1);
   add(P_BEAD, &D.slot, 4, cap);
   add(P_BEAD, &D.cand, 4, cap);
+  add(P_BEAD, &D.fpos, sizeof(float4), 1);
   add(P_BEAD, &D.slot_k, 4, cap);
   add(P_BEAD, &D.red, 8, kRed);
   add(P_BINC, &D.bond_pq, 4, 1); add(P_BINC, &D.bond_r0, 8, 1);
@@ -329,7 +330,7 @@ void register_arrays(sopg_ctx* c) {
 
 // skin <= 0: no candidate lists
 void derive_tables(const sopg_tables& t, double skin, Tables& T) {
-  double thr = 0.0;
+  double thr = 0.0, reach = 0.0;
   for (int i = 0; i < 3; i++)
     for (int j = 0; j < 3; j++) {
       T.fatt[i][j] = t.force_coeff_att[i][j];
@@ -347,6 +348,7 @@ void derive_tables(const sopg_tables& t, double skin, Tables& T) {
       T.plrc2[i][j] = rp * rp;
       const double rc = rn + skin;
       T.candc2[i][j] = rc * rc;
+      reach = std::max(reach, rn + 2.0 * skin + 1.0);
       thr = std::max(thr, T.s2[i][j] * kTol);
     }
   // A pair of the neighbor list at positions P (d < 3.2 sigma) was within
@@ -355,6 +357,7 @@ void derive_tables(const sopg_tables& t, double skin, Tables& T) {
   T.cand_on = skin > 0.0 ? 1 : 0;
   const double half = 0.5 * skin * (1.0 - 1e-6);
   T.cand_disp2 = skin > 0.0 ? half * half : -1.0;
+  T.cand_reach = reach;
   T.k_bnd = t.k_bnd;
   T.R0sq = t.R0sq;
   T.e_bnd_coeff = t.e_bnd_coeff;

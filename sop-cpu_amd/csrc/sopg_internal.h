@@ -32,6 +32,7 @@ struct Tables {
   int cand_on;                      // skin > 0
   double candc2[3][3];              // (3.2 sigma_rep + skin)^2
   double cand_disp2;                // (skin/2)^2, less a margin far above rounding
+  double cand_reach;                // bound on a valid candidate's distance: max 3.2 sigma + 2 skin + 1
 };
 
 struct ChainP {
@@ -117,6 +118,7 @@ struct Dev {
   // and the positions they were built at
   uint32_t* cand; int* ccnt;
   double *u0x, *u0y, *u0z;
+  float4* fpos;  // single-precision copy of the positions at a candidate-list rebuild (w unused)
   double *cx, *cy, *cz;  // cell coordinates (cell list)
   double* red;           // per-bead reduction fields
   int* err;              // bit0: slot overflow

@@ -401,9 +401,11 @@ __global__ void __launch_bounds__(256) k_grid(Dev D, const Tables* __restrict__ 
     double dmax = 0.0;
     for (int b = threadIdx.x; b < P.n; b += blockDim.x) {
       const int g = P.off + b;
-      const double ex = px[g] - D.u0x[g], ey = py[g] - D.u0y[g], ez = pz[g] - D.u0z[g];
+      const double x = px[g], y = py[g], z = pz[g];
+      const double ex = x - D.u0x[g], ey = y - D.u0y[g], ez = z - D.u0z[g];
       const double dd = ex * ex + ey * ey + ez * ez;
       dmax = (dd > dmax || dd != dd) ? dd : dmax;  // NaN propagates (a rebuild)
+      D.fpos[g] = make_float4((float)x, (float)y, (float)z, 0.0f);  // for the filter's gathers
     }
     sh[0][threadIdx.x] = dmax;
     __syncthreads();
@@ -430,7 +432,10 @@ __global__ void __launch_bounds__(256) k_grid(Dev D, const Tables* __restrict__ 
     const int g = P.off + b;
     const double v[3] = {ax[g], ay[g], az[g]};
     for (int a = 0; a < 3; a++) { lo[a] = fmin(lo[a], v[a]); hi[a] = fmax(hi[a], v[a]); }
-    if (cand) { D.u0x[g] = v[0]; D.u0y[g] = v[1]; D.u0z[g] = v[2]; }
+    if (cand) {
+      D.u0x[g] = v[0]; D.u0y[g] = v[1]; D.u0z[g] = v[2];
+      D.fpos[g] = make_float4((float)v[0], (float)v[1], (float)v[2], 0.0f);
+    }
   }
   for (int a = 0; a < 3; a++) { sh[a][threadIdx.x] = lo[a]; sh[3 + a][threadIdx.x] = hi[a]; }
   __syncthreads();
@@ -807,12 +812,24 @@ __global__ void __launch_bounds__(256) k_cand_filter(Dev D, const Tables* __rest
   const uint32_t* C = D.cand + (size_t)g * D.cap;
   uint32_t* S = D.slot + (size_t)g * D.cap;
   const int n = D.ccnt[g];
+  // The candidates' positions are gathered in single precision (half the
+  // data of the double positions).  A single-precision coordinate is within
+  // 2^-24 |x| of the double one, so for a partner within R = cand_reach (every
+  // candidate of a valid list is) each component of the displacement is within
+  // ec of the exact one and the squared distance within eps2 (a bound on
+  // 2 R sqrt(3) ec + 3 ec^2, with room for the double roundings).  A candidate
+  // whose distance lies within eps2 of a cutoff, or near the min-image limit,
+  // is decided in double precision from the double positions.
+  const double R = T->cand_reach;
+  const double cxs = (double)(float)cx, cys = (double)(float)cy, czs = (double)(float)cz;
+  const double ec = 0x1p-24 * (2.0 * fmax(fabs(cx), fmax(fabs(cy), fabs(cz))) + R);
+  const double eps2 = 2.0 * R * 1.7320508075688773 * ec + 3.0 * ec * ec + 1e-9;
+  const double qlim = qbox - 1.0;
   uint32_t wn = 0u, wn2 = 0u;
-  double pxn = 0.0, pyn = 0.0, pzn = 0.0;
+  float4 pn = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   if (l < n) {
     wn = C[l];
-    const int o = off + (int)(wn & kPartnerMask);
-    pxn = ux[o]; pyn = uy[o]; pzn = uz[o];
+    pn = D.fpos[off + (int)(wn & kPartnerMask)];
   }
   if (l + LF < n) wn2 = C[l + LF];
   // natives (neighbor_list.cpp:17-67): the displacement from the bead's own
@@ -833,21 +850,31 @@ __global__ void __launch_bounds__(256) k_cand_filter(Dev D, const Tables* __rest
   for (int base = 0; base < n; base += LF) {
     const int q = base + l;
     uint32_t w = wn;
-    const double px = pxn, py = pyn, pz = pzn;
-    if (q + LF < n) {
-      const int o = off + (int)(wn2 & kPartnerMask);
-      pxn = ux[o]; pyn = uy[o]; pzn = uz[o];
-    }
+    const float4 pf = pn;
+    if (q + LF < n) pn = D.fpos[off + (int)(wn2 & kPartnerMask)];
     wn = wn2;
     if (q + 2 * LF < n) wn2 = C[q + 2 * LF];
     bool keep = false;
     if (q < n) {
       const int side = (w >> 25) & 1, tpi = (int)((w >> 26) & 3) * 3 + (int)((w >> 28) & 3);
-      double dx, dy, dz;
-      pair_d3(side, cx, cy, cz, px, py, pz, boxl, qbox, dx, dy, dz);
-      const double d2 = dx * dx + dy * dy + dz * dz;
-      keep = d2 < cut2[tpi][0];
-      w = (w & ~(3u << 30)) | ((keep && d2 < cut2[tpi][1]) ? pbit : 0u);
+      const double sx = (double)pf.x - cxs, sy = (double)pf.y - cys, sz = (double)pf.z - czs;
+      const double s2 = sx * sx + sy * sy + sz * sz;
+      const double cn = cut2[tpi][0], cp = cut2[tpi][1];
+      bool pl;
+      if (fabs(sx) <= qlim && fabs(sy) <= qlim && fabs(sz) <= qlim && fabs(s2 - cn) > eps2 &&
+          fabs(s2 - cp) > eps2) {
+        keep = s2 < cn;
+        pl = s2 < cp;
+      } else {
+        // the exact decision, in double precision from the double positions
+        const int o = off + (int)(w & kPartnerMask);
+        double dx, dy, dz;
+        pair_d3(side, cx, cy, cz, ux[o], uy[o], uz[o], boxl, qbox, dx, dy, dz);
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        keep = d2 < cn;
+        pl = d2 < cp;
+      }
+      w = (w & ~(3u << 30)) | ((keep && pl) ? pbit : 0u);
     }
     const uint64_t m = group_mask<LF>(keep);
     if (keep) S[cnt + __popcll(m & ((1ull << l) - 1ull))] = w;

@@ -69,3 +69,31 @@ def test_ordered_sum_with_neutral_terms_unchanged():
         for v, k in zip(f, keep):
             b = b + (v if k else -0.0)
         assert _bits(a) == _bits(b)
+
+
+def test_single_precision_candidate_distance_decides_as_double():
+    """The candidate filter decides from single-precision positions when the
+    squared distance lies farther than eps2 from every cutoff, and in double
+    precision otherwise: every fast decision equals the double one."""
+    rng = np.random.default_rng(14)
+    n = 1 << 20
+    R = 22.4 + 2 * 4.0 + 1.0                       # largest 3.2 sigma + 2 skin + 1
+    c = rng.uniform(-3000.0, 3000.0, (n, 3))
+    c[: n // 4] = rng.uniform(-60.0, 60.0, (n // 4, 3))
+    cuts = np.array([3.2 * 3.8, 2.5 * 3.8, 3.2 * 7.0, 2.5 * 5.4])
+    r = rng.choice(cuts, n) + rng.normal(0.0, 1e-4, n)   # distances at the cutoffs
+    r[: n // 8] = rng.uniform(0.5, R - 1.0, n // 8)
+    u = rng.normal(size=(n, 3))
+    p = c + u / np.linalg.norm(u, axis=1)[:, None] * r[:, None]
+    d = p - c
+    d2 = (d * d).sum(1)                              # the double computation
+    m = np.abs(c).max(1)
+    ec = 2.0 ** -24 * (2.0 * m + R)
+    eps2 = 2.0 * R * np.sqrt(3.0) * ec + 3.0 * ec * ec + 1e-9
+    s = p.astype(np.float32).astype(np.float64) - c.astype(np.float32).astype(np.float64)
+    s2 = (s * s).sum(1)
+    for cut in cuts:
+        cn = cut * cut
+        fast = np.abs(s2 - cn) > eps2
+        assert fast.mean() > 0.5
+        assert np.array_equal((s2 < cn)[fast], (d2 < cn)[fast])
