@@ -405,7 +405,7 @@ __global__ void __launch_bounds__(256) k_grid(Dev D, const Tables* __restrict__ 
       const double ex = x - D.u0x[g], ey = y - D.u0y[g], ez = z - D.u0z[g];
       const double dd = ex * ex + ey * ey + ez * ez;
       dmax = (dd > dmax || dd != dd) ? dd : dmax;  // NaN propagates (a rebuild)
-      D.fpos[g] = make_float4((float)x, (float)y, (float)z, 0.0f);  // for the filter's gathers
+      D.fpos[g] = make_float4((float)x, (float)y, (float)z, (float)D.type[g]);  // for the filter's gathers
     }
     sh[0][threadIdx.x] = dmax;
     __syncthreads();
@@ -434,7 +434,7 @@ __global__ void __launch_bounds__(256) k_grid(Dev D, const Tables* __restrict__ 
     for (int a = 0; a < 3; a++) { lo[a] = fmin(lo[a], v[a]); hi[a] = fmax(hi[a], v[a]); }
     if (cand) {
       D.u0x[g] = v[0]; D.u0y[g] = v[1]; D.u0z[g] = v[2];
-      D.fpos[g] = make_float4((float)v[0], (float)v[1], (float)v[2], 0.0f);
+      D.fpos[g] = make_float4((float)v[0], (float)v[1], (float)v[2], (float)D.type[g]);
     }
   }
   for (int a = 0; a < 3; a++) { sh[a][threadIdx.x] = lo[a]; sh[3 + a][threadIdx.x] = hi[a]; }
@@ -648,35 +648,52 @@ __global__ void __launch_bounds__(256) k_rebuild_grid(Dev D, const Tables* __res
       for (int k = 1; k < 9; k++) at = (pre[k] <= t) ? rs[k] + (t - pre[k]) : at;
       return at;
     };
+    // A candidate-list build gathers the single-precision positions (and
+    // types) of the planning step and keeps a superset of the pairs within
+    // 3.2 sigma + skin: the cutoff is widened by the bound eps2 on the error of
+    // the squared distance (as in the candidate filter), and the filter decides
+    // every kept pair exactly.  A neighbor-list build uses the double positions.
+    const double R = T->cand_reach;
+    const double oxs = (double)(float)ox, oys = (double)(float)oy, ozs = (double)(float)oz;
+    const double ec = 0x1p-24 * (2.0 * fmax(fabs(ox), fmax(fabs(oy), fabs(oz))) + R);
+    const double eps2 = 2.0 * R * 1.7320508075688773 * ec + 3.0 * ec * ec + 1e-9;
+    const double qlim = qbox - 1.0;
     int ccn = -1;
     double pxn = 0.0, pyn = 0.0, pzn = 0.0;
-    if (l < total) {
-      ccn = cb[locate(l)];
+    float4 pfn = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    auto fetch = [&](int t) {
+      ccn = cb[locate(t)];
       const int gc = off + ccn;
-      pxn = ux[gc]; pyn = uy[gc]; pzn = uz[gc];
-    }
+      if (cmode) pfn = D.fpos[gc];
+      else { pxn = ux[gc]; pyn = uy[gc]; pzn = uz[gc]; }
+    };
+    if (l < total) fetch(l);
     for (int base = 0; base < total; base += LG) {
       const int cc = ccn;
       const double px = pxn, py = pyn, pz = pzn;
-      if (base + LG + l < total) {
-        ccn = cb[locate(base + LG + l)];
-        const int gc = off + ccn;
-        pxn = ux[gc]; pyn = uy[gc]; pzn = uz[gc];
-      }
+      const float4 pf = pfn;
+      if (base + LG + l < total) fetch(base + LG + l);
       if (base + l >= total || !(cc - b > 2 || b - cc > 2)) continue;
       const int gc = off + cc;
       const int side = gc < g ? 1 : 0;  // 1: the partner is the pair's first bead
-      const int ct = D.type[gc];
-      const int tpi = side ? ct * 3 + gt : gt * 3 + ct;
-      double ddx, ddy, ddz;
-      pair_d3(side, ox, oy, oz, px, py, pz, boxl, qbox, ddx, ddy, ddz);
-      const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-      const bool keep = nlcrit ? d2 < cut2[tpi][cmode ? 2 : 0] : cell_adjacent(D, P, side ? gc : g, side ? g : gc);
-      if (keep) {
-        const uint32_t bit = 1u << (cc & 31);
-        atomicOr(&nlb[cc >> 5], bit);
-        if (!cmode && d2 < cut2[tpi][1]) atomicOr(&plb[cc >> 5], bit);
+      bool keep;
+      if (cmode) {
+        const int ct = (int)pf.w;
+        const int tpi = side ? ct * 3 + gt : gt * 3 + ct;
+        const double sx = (double)pf.x - oxs, sy = (double)pf.y - oys, sz = (double)pf.z - ozs;
+        // beyond the min-image fast path: kept (the filter decides exactly)
+        keep = !(fabs(sx) <= qlim && fabs(sy) <= qlim && fabs(sz) <= qlim) ||
+               sx * sx + sy * sy + sz * sz < cut2[tpi][2] + eps2;
+      } else {
+        const int ct = D.type[gc];
+        const int tpi = side ? ct * 3 + gt : gt * 3 + ct;
+        double ddx, ddy, ddz;
+        pair_d3(side, ox, oy, oz, px, py, pz, boxl, qbox, ddx, ddy, ddz);
+        const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+        keep = nlcrit ? d2 < cut2[tpi][0] : cell_adjacent(D, P, side ? gc : g, side ? g : gc);
+        if (keep && d2 < cut2[tpi][1]) atomicOr(&plb[cc >> 5], 1u << (cc & 31));
       }
+      if (keep) atomicOr(&nlb[cc >> 5], 1u << (cc & 31));
     }
   }
   __syncthreads();
