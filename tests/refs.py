@@ -193,7 +193,7 @@ def setup_system(sysobj, d, cmds, files=("bonds", "angles", "vdw", "init")):
         sysobj.load(kind, os.path.join(d, kind + ".dat"))
 
 
-# BASELINE dynamics settings, in the order the scripts must use (set_params is
+# Standard dynamics settings of the workloads, in the order the scripts must use (set_params is
 # order-sensitive: `set zeta` resets h, then `set t_step` re-derives a1..a5).
 def std_cmds(temp=0.6, cutoff="neighborlist", nnlup=50, nup=1000, extra=()):
     c = [("dynamics", "underdamped"), ("zeta", "50"), ("t_step", "0.05"), ("temp", str(temp)),
