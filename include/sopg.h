@@ -119,7 +119,8 @@ int sopg_set_binned(sopg_ctx* ctx, int enable);
  * the largest no-cutoff chain).  Fixed at the first device operation. */
 int sopg_set_capacity(sopg_ctx* ctx, int cap);
 /* Pre-size the device pools for `beads` beads in `chains` chains (optional:
- * pools grow on demand, but growing copies every array once). */
+ * pools grow on demand, but growing copies every array once).  A context
+ * holds at most 2^28 beads (SOPG_ERR_ARG beyond) and a chain 2^25. */
 int sopg_reserve(sopg_ctx* ctx, long beads, int chains);
 
 /* ---------------------------------------------------------------- chains */
