@@ -684,3 +684,20 @@ def test_candidate_lists_keep_lists_and_results(inputs_root):
         for la, lb in zip(l0, l1):
             for (ai, aj), (bi, bj) in zip(la, lb):
                 assert np.array_equal(ai, bi) and np.array_equal(aj, bj)
+
+
+def test_bead_pool_limit_refused_and_context_usable():
+    """Reserving beyond the bead pool's 2^28-bead limit fails with a clear error
+    before anything is allocated; the context keeps working."""
+    x = sopg.gen_native(60, 61)
+    v = sopg.gen_maxwell(60, 0.6, 61)
+    ctx = sopg.Context()
+    c = ctx.add_chain(sopg.chain_topology(x), sopg.chain_params(cutoff=sopg.CUTOFF_NEIGHBOR,
+                                                                 noise_mode=sopg.NOISE_PHILOX, seed=61))
+    _setup_chain(ctx, c, x, v)
+    with pytest.raises(sopg.SopgError, match="bead pool limit"):
+        ctx.reserve((1 << 28) + 1, 4)
+    ctx.run(5)
+    st = ctx.get_state(c)
+    assert np.all(np.isfinite(st["unc"]))
+    ctx.close()
