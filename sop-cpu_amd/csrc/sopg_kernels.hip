@@ -376,7 +376,10 @@ __device__ __forceinline__ bool rebuild_due(const ChainP& P, int rel, int mode, 
 // largest displacement from u0 stays within half the skin, the neighbor list
 // is filtered from the candidate list; otherwise the candidate list is
 // rebuilt (binned, cells >= 3.2 sigma + skin) at the current positions.
-__global__ void __launch_bounds__(256) k_grid(Dev D, const Tables* __restrict__ T, int cur, int rel, int mode,
+// One block per chain; a chain's beads (up to thousands) are strided over the
+// block's threads, so a wide block keeps the per-chain loop short.
+constexpr int kGridThreads = 1024;
+__global__ void __launch_bounds__(kGridThreads) k_grid(Dev D, const Tables* __restrict__ T, int cur, int rel, int mode,
                                               int c0, int maxn) {
   rel += *D.relb;
   const int c = c0 + blockIdx.x;
@@ -395,7 +398,7 @@ __global__ void __launch_bounds__(256) k_grid(Dev D, const Tables* __restrict__ 
   const double* px = cur ? D.ux[1] : D.ux[0];
   const double* py = cur ? D.uy[1] : D.uy[0];
   const double* pz = cur ? D.uz[1] : D.uz[0];
-  __shared__ double sh[6][256];
+  __shared__ double sh[6][kGridThreads];
   if (cand && G.cand_valid) {
     // largest squared displacement from the candidate lists' positions
     double dmax = 0.0;
@@ -1548,7 +1551,7 @@ constexpr int kMaxBitmapWords = 512;                 // 16 beads x 2 bitmaps x 2
 void launch_rebuild(int L, const Dev& D, const Tables* T, int cur, int par, int rel, int mode, int g0, int g1,
                     int c0, int nc, int binned, int cand, hipStream_t s) {
   const int words = D.bitmap_words < kMaxBitmapWords ? D.bitmap_words : kMaxBitmapWords;
-  hipLaunchKernelGGL(k_grid, dim3(nc), dim3(256), 0, s, D, T, cur, rel, mode, c0, binned ? 32 * words : 0);
+  hipLaunchKernelGGL(k_grid, dim3(nc), dim3(kGridThreads), 0, s, D, T, cur, rel, mode, c0, binned ? 32 * words : 0);
   if (binned) {
     hipLaunchKernelGGL(k_bin_count, dim3(blocks_for(g1 - g0)), dim3(256), 0, s, D, cur, rel, mode, g0, g1);
     hipLaunchKernelGGL(k_bin_scan, dim3(nc), dim3(256), 0, s, D, rel, mode, c0);
