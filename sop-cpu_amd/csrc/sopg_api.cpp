@@ -320,7 +320,6 @@ void register_arrays(sopg_ctx* c) {
   add(P_RINC, &D.rep_code, 4, 1);
   add(P_CHAIN, &D.cp, sizeof(ChainP), 1);
   add(P_CHAIN, &D.grid, sizeof(GridP), 1);
-  add(P_CHAIN, &D.cell_cnt, 4, kGridCells);
   add(P_CHAIN, &D.cell_start, 4, kGridCells);
   add(P_CHAIN, &D.cell_fill, 4, kGridCells);
   add(P_CHAIN, &c->d_out, 8, kRed);
@@ -583,7 +582,6 @@ int materialize(sopg_ctx* c, int id) {
   HIP_OK(c, hipMemsetAsync(D.cnt + off, 0, sizeof(int) * n, c->stream));
   HIP_OK(c, hipMemsetAsync(D.red + (size_t)off * kRed, 0, sizeof(double) * n * kRed, c->stream));
   HIP_OK(c, hipMemsetAsync(D.grid + id, 0, sizeof(GridP), c->stream));
-  HIP_OK(c, hipMemsetAsync(D.cell_cnt + (size_t)id * kGridCells, 0, sizeof(int) * kGridCells, c->stream));
   h.on_device = true;
   D.nb = (int)c->pool[P_BEAD].hw;
   h.pl_fresh = false;  // the new chain has no pair list yet

@@ -129,7 +129,7 @@ struct Dev {
   ChainP* cp;
   // binned rebuild: per-chain grid (GridP), kGridCells cells per chain
   GridP* grid;
-  int *cell_cnt, *cell_start, *cell_fill;  // per cell
+  int *cell_start, *cell_fill;             // per cell: first position and bead count
   int *bead_cell, *cell_beads;             // per bead: its cell, and beads sorted by cell
   int bitmap_words;                        // ceil(max chain size / 32)
 };

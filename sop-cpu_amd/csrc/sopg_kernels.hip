@@ -376,6 +376,11 @@ __device__ __forceinline__ bool rebuild_due(const ChainP& P, int rel, int mode, 
 // largest displacement from u0 stays within half the skin, the neighbor list
 // is filtered from the candidate list; otherwise the candidate list is
 // rebuilt (binned, cells >= 3.2 sigma + skin) at the current positions.
+__device__ __forceinline__ int grid_coord(double v, double o, double inv, int n) {
+  int k = (int)floor((v - o) * inv);
+  return k < 0 ? 0 : (k >= n ? n - 1 : k);
+}
+
 // One block per chain; a chain's beads (up to thousands) are strided over the
 // block's threads, so a wide block keeps the per-chain loop short.
 constexpr int kGridThreads = 1024;
@@ -398,7 +403,15 @@ __global__ void __launch_bounds__(kGridThreads) k_grid(Dev D, const Tables* __re
   const double* px = cur ? D.ux[1] : D.ux[0];
   const double* py = cur ? D.uy[1] : D.uy[0];
   const double* pz = cur ? D.uz[1] : D.uz[0];
-  __shared__ double sh[6][kGridThreads];
+  // the reductions' scratch, then (binning) the per-cell counts and positions
+  __shared__ union {
+    double sh[6][kGridThreads];
+    int cells[kGridCells];
+  } u;
+  auto& sh = u.sh;
+  __shared__ int part[kGridThreads];
+  __shared__ double gf[4];
+  __shared__ int gi[4];
   if (cand && G.cand_valid) {
     // largest squared displacement from the candidate lists' positions
     double dmax = 0.0;
@@ -450,102 +463,82 @@ __global__ void __launch_bounds__(kGridThreads) k_grid(Dev D, const Tables* __re
       }
     __syncthreads();
   }
-  if (threadIdx.x != 0) return;
-  // every pair difference stays below half the period: the min image is the
-  // plain difference, so all kept pairs are in adjacent cells of a local grid
-  const double limit = 0.49 * (cell ? P.ncell : P.boxl);
-  const double inv = cell ? 1.0 : 1.0 / (cand ? P.gcand : P.gcell);
-  int dims[3];
-  bool use = true;
-  for (int a = 0; a < 3; a++) {
-    const double ext = sh[3 + a][0] - sh[a][0];
-    use = use && ext <= limit && isfinite(ext);
-    dims[a] = use ? (int)floor(ext * inv) + 1 : 0;
-    use = use && dims[a] <= kGridDim;
+  if (threadIdx.x == 0) {
+    // every pair difference stays below half the period: the min image is the
+    // plain difference, so all kept pairs are in adjacent cells of a local grid
+    const double limit = 0.49 * (cell ? P.ncell : P.boxl);
+    const double inv = cell ? 1.0 : 1.0 / (cand ? P.gcand : P.gcell);
+    int dims[3];
+    bool use = true;
+    for (int a = 0; a < 3; a++) {
+      const double ext = sh[3 + a][0] - sh[a][0];
+      use = use && ext <= limit && isfinite(ext);
+      dims[a] = use ? (int)floor(ext * inv) + 1 : 0;
+      use = use && dims[a] <= kGridDim;
+    }
+    G.ox = sh[0][0]; G.oy = sh[1][0]; G.oz = sh[2][0];
+    G.inv = inv;
+    G.nx = dims[0]; G.ny = dims[1]; G.nz = dims[2];
+    G.work = !use ? W_SCAN : (cand ? (W_BIN | W_CAND | W_FILTER) : W_BIN);
+    if (G.work & W_BIN) D.plan[0] = rel;
+    if (G.work & W_FILTER) D.plan[1] = rel;
+    if (G.work & W_SCAN) D.plan[2] = rel;
+    if (cand) G.cand_valid = use ? 1 : 0;
+    gf[0] = G.ox; gf[1] = G.oy; gf[2] = G.oz; gf[3] = inv;
+    gi[0] = dims[0]; gi[1] = dims[1]; gi[2] = dims[2]; gi[3] = G.work;
   }
-  G.ox = sh[0][0]; G.oy = sh[1][0]; G.oz = sh[2][0];
-  G.inv = inv;
-  G.nx = dims[0]; G.ny = dims[1]; G.nz = dims[2];
-  G.work = !use ? W_SCAN : (cand ? (W_BIN | W_CAND | W_FILTER) : W_BIN);
-  if (G.work & W_BIN) D.plan[0] = rel;
-  if (G.work & W_FILTER) D.plan[1] = rel;
-  if (G.work & W_SCAN) D.plan[2] = rel;
-  if (cand) G.cand_valid = use ? 1 : 0;
-}
-
-__device__ __forceinline__ int grid_coord(double v, double o, double inv, int n) {
-  int k = (int)floor((v - o) * inv);
-  return k < 0 ? 0 : (k >= n ? n - 1 : k);
-}
-
-// Counting sort of beads into cells: count, scan, fill.
-__global__ void k_bin_count(Dev D, int cur, int rel, int mode, int g0, int g1) {
-  rel += *D.relb;
-  if (D.plan[0] != rel) return;  // no chain is binned in this rebuild
-  const int g = g0 + blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= g1) return;
-  const int c = D.chain[g];
-  if (c < 0) return;
-  const GridP& G = D.grid[c];
-  if (!(G.work & W_BIN)) return;
-  const bool cell = (mode >= 0 ? mode : D.cp[c].cutoff) == SOPG_CUTOFF_CELL;
-  const double x = cell ? D.cx[g] : (cur ? D.ux[1] : D.ux[0])[g];
-  const double y = cell ? D.cy[g] : (cur ? D.uy[1] : D.uy[0])[g];
-  const double z = cell ? D.cz[g] : (cur ? D.uz[1] : D.uz[0])[g];
-  const int id = (grid_coord(z, G.oz, G.inv, G.nz) * G.ny + grid_coord(y, G.oy, G.inv, G.ny)) * G.nx +
-                 grid_coord(x, G.ox, G.inv, G.nx);
-  D.bead_cell[g] = id;
-  atomicAdd(&D.cell_cnt[(size_t)c * kGridCells + id], 1);
-}
-
-__global__ void __launch_bounds__(256) k_bin_scan(Dev D, int rel, int mode, int c0) {
-  rel += *D.relb;
-  if (D.plan[0] != rel) return;
-  const int c = c0 + blockIdx.x;
-  const GridP& G = D.grid[c];
-  if (!(G.work & W_BIN)) return;
-  const int ncell = G.nx * G.ny * G.nz;
-  int* cnt = D.cell_cnt + (size_t)c * kGridCells;
+  __syncthreads();
+  if (!(gi[3] & W_BIN)) return;  // block-uniform
+  // Counting sort of the chain's beads into the grid's cells (in LDS): the
+  // cell of each bead, the cells' starts (exclusive scan of the counts), and
+  // the beads of each cell, contiguous in cell order (any order inside a cell:
+  // the rebuild's bitmaps make the lists independent of it).
+  const int nx = gi[0], ny = gi[1], nz = gi[2], ncell = nx * ny * nz;
+  const double gox = gf[0], goy = gf[1], goz = gf[2], ginv = gf[3];
+  int* cells = u.cells;
+  const int t = (int)threadIdx.x, nt = (int)blockDim.x;
+  for (int k = t; k < ncell; k += nt) cells[k] = 0;
+  __syncthreads();
+  for (int b = t; b < P.n; b += nt) {
+    const int g = P.off + b;
+    const int id = (grid_coord(az[g], goz, ginv, nz) * ny + grid_coord(ay[g], goy, ginv, ny)) * nx +
+                   grid_coord(ax[g], gox, ginv, nx);
+    D.bead_cell[g] = id;
+    atomicAdd(&cells[id], 1);
+  }
+  __syncthreads();
+  // each thread scans a contiguous run of cells; the runs' sums are scanned
+  // across the block
+  const int per = (ncell + nt - 1) / nt, k0 = min(ncell, t * per), k1 = min(ncell, k0 + per);
+  int sum = 0;
+  for (int k = k0; k < k1; k++) sum += cells[k];
+  part[t] = sum;
+  __syncthreads();
+  for (int o = 1; o < nt; o <<= 1) {
+    const int v = t >= o ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = part[t] - sum;
   int* st = D.cell_start + (size_t)c * kGridCells;
   int* fl = D.cell_fill + (size_t)c * kGridCells;
-  // block-wide exclusive scan in chunks of 256 cells
-  __shared__ int sh[256];
-  int carry = 0;
-  for (int base = 0; base < ncell; base += 256) {
-    const int k = base + threadIdx.x;
-    const int v = k < ncell ? cnt[k] : 0;
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int o = 1; o < 256; o <<= 1) {
-      const int add = (int)threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
-      __syncthreads();
-      sh[threadIdx.x] += add;
-      __syncthreads();
-    }
-    if (k < ncell) {
-      st[k] = carry + sh[threadIdx.x] - v;
-      fl[k] = 0;
-      cnt[k] = 0;  // ready for the next rebuild
-    }
-    carry += sh[255];
-    __syncthreads();
+  for (int k = k0; k < k1; k++) {
+    const int nk = cells[k];
+    st[k] = run;
+    fl[k] = nk;
+    cells[k] = run;  // the cell's next free position
+    run += nk;
+  }
+  __syncthreads();
+  for (int b = t; b < P.n; b += nt) {
+    const int g = P.off + b;
+    D.cell_beads[P.off + atomicAdd(&cells[D.bead_cell[g]], 1)] = b;
   }
 }
 
-__global__ void k_bin_fill(Dev D, int rel, int mode, int g0, int g1) {
-  rel += *D.relb;
-  if (D.plan[0] != rel) return;
-  const int g = g0 + blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= g1) return;
-  const int c = D.chain[g];
-  if (c < 0) return;
-  if (!(D.grid[c].work & W_BIN)) return;
-  const ChainP& P = D.cp[c];
-  const size_t cb = (size_t)c * kGridCells + D.bead_cell[g];
-  const int at = D.cell_start[cb] + atomicAdd(&D.cell_fill[cb], 1);
-  D.cell_beads[P.off + at] = g - P.off;
-}
 
+// Counting sort of beads into cells: count, scan, fill.
 // One group of LG lanes per bead, 256 / LG beads per block.  Each bead keeps
 // two LDS bitmaps with one bit per bead of its chain: partners that pass the
 // list criterion, and those of them inside the pair-list cutoff.  The 27
@@ -1553,9 +1546,6 @@ void launch_rebuild(int L, const Dev& D, const Tables* T, int cur, int par, int 
   const int words = D.bitmap_words < kMaxBitmapWords ? D.bitmap_words : kMaxBitmapWords;
   hipLaunchKernelGGL(k_grid, dim3(nc), dim3(kGridThreads), 0, s, D, T, cur, rel, mode, c0, binned ? 32 * words : 0);
   if (binned) {
-    hipLaunchKernelGGL(k_bin_count, dim3(blocks_for(g1 - g0)), dim3(256), 0, s, D, cur, rel, mode, g0, g1);
-    hipLaunchKernelGGL(k_bin_scan, dim3(nc), dim3(256), 0, s, D, rel, mode, c0);
-    hipLaunchKernelGGL(k_bin_fill, dim3(blocks_for(g1 - g0)), dim3(256), 0, s, D, rel, mode, g0, g1);
     Dev Dg = D;
     Dg.bitmap_words = words;
     hipLaunchKernelGGL(k_rebuild_grid<kGridLanes>, dim3((unsigned)((g1 - g0 + kGridBeads - 1) / kGridBeads)),
