@@ -641,16 +641,20 @@ def test_resident_kernel_overdamped(inputs_root):
     assert ra == rb and len(ra) == 2 * (200 // 40)
 
 
-def test_candidate_lists_keep_lists_and_results(inputs_root):
+@pytest.mark.parametrize("offset", [0.0, 2.0e4])
+def test_candidate_lists_keep_lists_and_results(inputs_root, offset):
     """Neighbor lists filtered from candidate lists (reused while no bead has
     moved more than half the skin since their build) are the lists of a full
     rebuild: states, records and lists agree bit for bit for skin 0 (off), a
     skin small enough to rebuild the candidates nearly every time, and the
     default -- with positions moved from the host between runs, a cell-list
-    chain beside, and runs starting between rebuilds."""
+    chain beside, and runs starting between rebuilds.  The chains far from the
+    origin (offset) widen the single-precision error band of the candidate
+    build and filter, so that more pairs are decided in double precision."""
     specs = [(90, 151, sopg.CUTOFF_NEIGHBOR), (260, 152, sopg.CUTOFF_NEIGHBOR), (700, 153, sopg.CUTOFF_NEIGHBOR),
              (150, 154, sopg.CUTOFF_CELL)]
     data = [chain_inputs(inputs_root, n, seed)[:2] for (n, seed, _) in specs]
+    data = [(x + offset, v) for (x, v) in data]
     results = []
     for skin in (0.0, 0.2, 3.0):
         ctx = sopg.Context()
@@ -664,7 +668,7 @@ def test_candidate_lists_keep_lists_and_results(inputs_root):
             (ctx.update_cell_list if cut == sopg.CUTOFF_CELL else ctx.update_neighbor_list)(c)
             ctx.update_pair_list(c)
             ctx.force_eval(c)
-        print(f"[gpu] candidate lists skin={skin}", flush=True)
+        print(f"[gpu] candidate lists skin={skin} offset={offset}", flush=True)
         recs = ctx.run(130)
         # move chain 1 from the host (to its native positions again) mid-run
         st = ctx.get_state(1)
