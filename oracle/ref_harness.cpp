@@ -88,6 +88,7 @@ void ref_vdw_forces() { vdw_forces(); }
 void ref_energy_eval() { energy_eval(); }
 
 void ref_underdamped_iteration() { underdamped_iteration(incr_array()); }
+void ref_overdamped_iteration() { overdamped_iteration(incr_array()); }
 void ref_calculate_observables() { calculate_observables(incr_array()); }
 
 // Reference Ran_Gen (random_generator.cpp:88-146) on the global generator.
@@ -153,13 +154,15 @@ double ref_get_double(const char* name) {
   return -999999.0;
 }
 
-// which: 0 pos, 1 unc_pos, 2 vel, 3 force.  Layout out[3*(i-1)+c], beads 1..nbead.
+// which: 0 pos, 1 unc_pos, 2 vel, 3 force, 4 the ctrl loop's incr (sop.cpp:104).
+// Layout out[3*(i-1)+c], beads 1..nbead.
 static coord* coord_array(int which) {
   switch (which) {
     case 0: return pos;
     case 1: return unc_pos;
     case 2: return vel;
     case 3: return force;
+    case 4: return incr_array();
     default: return 0;
   }
 }

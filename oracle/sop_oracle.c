@@ -361,30 +361,6 @@ int orc_load(orc_sys* s, const char* kind, const char* path) {
   return 0;
 }
 
-static int read_idx_xyz(const char* path, orc_vec* a, int n) {
-  char line[1024];
-  char* t;
-  int i;
-  FILE* f = fopen(path, "r");
-  if (!f) return -1;
-  for (i = 1; i <= n; i++) {
-    if (!fgets(line, sizeof line, f)) line[0] = 0;
-    line[strcspn(line, "\n")] = 0;
-    t = strtok(line, " ");
-    t = strtok(NULL, " "); a[i].x = atof(t);
-    t = strtok(NULL, " "); a[i].y = atof(t);
-    t = strtok(NULL, " "); a[i].z = atof(t);
-  }
-  fclose(f);
-  return 0;
-}
-
-int orc_load_coords(orc_sys* s, const char* cf, const char* ucf) {
-  if (read_idx_xyz(cf, s->pos, s->nbead)) return -1;
-  return read_idx_xyz(ucf, s->unc_pos, s->nbead);
-}
-int orc_load_vels(orc_sys* s, const char* vf) { return read_idx_xyz(vf, s->vel, s->nbead); }
-
 /* ---------------------------------------------------------------- lists */
 
 /* neighbor_list.cpp:6-117 */
@@ -751,53 +727,6 @@ void orc_calculate_observables(orc_sys* s) {  /* sop.cpp:200-280 */
     sumvsq *= s->zeta / (2.0 * s->h);
     s->kinT = sumvsq / (3.0 * (double)(s->nbead));
   }
-}
-
-/* underdamped_ctrl (sop.cpp:93-198) for a fresh run, without file output. */
-long orc_underdamped_ctrl(orc_sys* s, const orc_vec* rpos, const orc_vec* runc,
-                          const orc_vec* rvel, orc_record* rec, long cap) {
-  double istep = 1.0;
-  int iup = 1, inlup = 1, i;
-  long nrec = 0;
-  if (!s->restart) {                           /* sop.cpp:106-115 */
-    for (i = 1; i <= s->nbead; i++) {
-      s->vel[i].x = s->vel[i].y = s->vel[i].z = 0.0;
-      s->force[i].x = s->force[i].y = s->force[i].z = 0.0;
-    }
-  }
-  if (s->neighborlist == 1) { orc_update_neighbor_list(s); orc_update_pair_list(s); }
-  else if (s->celllist == 1) { orc_update_cell_list(s); orc_update_pair_list(s); }
-  if (s->restart) {                            /* sop.cpp:132-136: after the list build */
-    for (i = 1; i <= s->nbead; i++) { s->pos[i] = rpos[i - 1]; s->unc_pos[i] = runc[i - 1]; s->vel[i] = rvel[i - 1]; }
-    istep = s->istep_restart + 1.0;
-  }
-  orc_energy_eval(s);
-  orc_force_eval(s);
-  while (istep <= s->nstep) {
-    if ((inlup % s->nnlup) == 0) {
-      if (s->neighborlist == 1) orc_update_neighbor_list(s);
-      else if (s->celllist == 1) orc_update_cell_list(s);
-      inlup = 0;
-    }
-    inlup++;
-    if (s->neighborlist == 1 || s->celllist == 1) orc_update_pair_list(s);
-    orc_underdamped_iteration(s);
-    if (!(iup % s->nup)) {
-      orc_energy_eval(s);
-      orc_calculate_observables(s);
-      if (rec && nrec < cap) {
-        orc_record* r = &rec[nrec];
-        r->istep = istep; r->T = s->T; r->kinT = s->kinT; r->e_bnd = s->e_bnd;
-        r->e_ang_ss = s->e_ang_ss; r->e_vdw_rr = s->e_vdw_rr; r->rna_etot = s->rna_etot;
-        r->Q = s->Q; r->contct_nat = s->contct_nat; r->rgsq = s->rgsq;
-      }
-      nrec++;
-      iup = 0;
-    }
-    istep += 1.0;
-    iup++;
-  }
-  return nrec;
 }
 
 /* ---------------------------------------------------------------- accessors */

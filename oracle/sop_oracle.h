@@ -68,18 +68,11 @@ typedef struct orc_sys {
   const double* noise; long noise_pos;
 } orc_sys;
 
-/* One `update.out` record (sop.cpp:176-179). */
-typedef struct {
-  double istep, T, kinT, e_bnd, e_ang_ss, e_vdw_rr, rna_etot, Q; int contct_nat; double rgsq;
-} orc_record;
-
 /* life cycle / parameters */
 orc_sys* orc_create(void);                 /* alloc_arrays defaults (params.cpp:262-400) */
 void orc_destroy(orc_sys* s);
 int orc_set(orc_sys* s, const char* key, const char* val);   /* set_params (params.cpp:12-101) */
 int orc_load(orc_sys* s, const char* kind, const char* path); /* load (params.cpp:110-260) */
-int orc_load_coords(orc_sys* s, const char* cfname, const char* uncfname); /* io.cpp:180-216 */
-int orc_load_vels(orc_sys* s, const char* vfname);                          /* io.cpp:218-242 */
 
 /* helpers */
 extern int orc_pow_mode;                 /* 0: glibc pow (reference), 1: correctly rounded */
@@ -106,10 +99,6 @@ void orc_energy_eval(orc_sys* s);          /* energy.cpp:6-19 */
 void orc_underdamped_iteration(orc_sys* s);/* sop.cpp:282-329 */
 void orc_overdamped_iteration(orc_sys* s); /* sop.cpp:331-363 */
 void orc_calculate_observables(orc_sys* s);/* sop.cpp:200-280 */
-/* underdamped_ctrl hot loop (sop.cpp:93-198) without file output: lists,
- * initial energy+force, then steps; one record per nup steps into rec (cap). */
-long orc_underdamped_ctrl(orc_sys* s, const orc_vec* restart_pos, const orc_vec* restart_unc,
-                          const orc_vec* restart_vel, orc_record* rec, long cap);
 
 /* flat accessors for ctypes */
 int orc_get_int(orc_sys* s, const char* name);

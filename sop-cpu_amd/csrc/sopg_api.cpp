@@ -32,7 +32,6 @@ void launch_pairfilter(int L, const Dev& D, const Tables* T, int cur, int par, i
 void launch_force(int L, const Dev& D, const Tables* T, int cur, int pr, int rel, int flags, bool next,
                   const double* noise, int g0, int g1, hipStream_t s);
 void launch_posupdate(const Dev& D, int src, int dst, int g0, int g1, hipStream_t s);
-void launch_velupdate(const Dev& D, int g0, int g1, hipStream_t s);
 void launch_observe(const Dev& D, int cur, int par, int rel, int all, int what, int g0, int g1, hipStream_t s);
 void launch_chain_reduce(const Dev& D, int c0, int nc, double* out, hipStream_t s);
 void launch_set_base(int* base, int value, hipStream_t s);
@@ -104,6 +103,9 @@ struct HostChain {
   bool rep_explicit = false;
   std::vector<int> ri, rj, rit, rjt;
   std::vector<int8_t> type;
+  // derived once from the topology (fill_chainp runs at every sopg_run)
+  double max_r2 = 0.0;  // largest lj_nat_pdb_dist^2
+  int type_mask = 0;    // bit t: some bead has type t
   std::vector<int> q0, q1;  // device incidence index of each native entry's first / second bead
   double istep = 1.0;
   int inlup = 1, iup = 1;
@@ -296,8 +298,9 @@ void register_arrays(sopg_ctx* c) {
   for (int s = 0; s < 2; s++) {
     add(P_BEAD, &D.ux[s], 8, 1); add(P_BEAD, &D.uy[s], 8, 1); add(P_BEAD, &D.uz[s], 8, 1);
     add(P_BEAD, &D.px[s], 8, 1); add(P_BEAD, &D.py[s], 8, 1); add(P_BEAD, &D.pz[s], 8, 1);
+    add(P_BEAD, &D.ix[s], 8, 1); add(P_BEAD, &D.iy[s], 8, 1); add(P_BEAD, &D.iz[s], 8, 1);
   }
-  for (double** p : {&D.vx, &D.vy, &D.vz, &D.fx, &D.fy, &D.fz, &D.ix, &D.iy, &D.iz, &D.cx, &D.cy, &D.cz, &D.u0x,
+  for (double** p : {&D.vx, &D.vy, &D.vz, &D.fx, &D.fy, &D.fz, &D.cx, &D.cy, &D.cz, &D.u0x,
                      &D.u0y, &D.u0z})
     add(P_BEAD, p, 8, 1);
   for (int** p : {&D.chain, &D.bond_beg, &D.bond_end, &D.ang_beg, &D.ang_end, &D.att_beg, &D.att_end, &D.rep_beg,
@@ -392,9 +395,7 @@ void fill_chainp(const HostChain& h, const Tables& T, ChainP& P) {
   P.att_off = (int)h.at[P_ATT];
   P.natt = (int)h.ai.size();
   P.rep_explicit = h.rep_explicit ? 1 : 0;
-  double mx = 0.0;
-  for (double r : h.ar) mx = std::max(mx, r * r);
-  P.thr_att = kTol * mx;
+  P.thr_att = kTol * h.max_r2;
   P.cutoff = p.cutoff;
   P.nnlup = p.nnlup;
   P.nup = p.nup;
@@ -416,8 +417,8 @@ void fill_chainp(const HostChain& h, const Tables& T, ChainP& P) {
   // grid edge of the binned rebuild: the largest neighbor-list cutoff among
   // the chain's bead types, with margin for the rounding of the cell index
   double rc2 = 0.0;
-  bool present[3] = {false, false, false};
-  for (int8_t t : h.type) present[t] = true;
+  bool present[3];
+  for (int a = 0; a < 3; a++) present[a] = (h.type_mask >> a) & 1;
   for (int a = 0; a < 3; a++)
     for (int b = 0; b < 3; b++)
       if (present[a] && present[b]) rc2 = std::max(rc2, T.nlrc2[a][b]);
@@ -575,9 +576,9 @@ int materialize(sopg_ctx* c, int id) {
   }
   // zero state
   for (int s = 0; s < 2; s++)
-    for (double* p : {D.ux[s], D.uy[s], D.uz[s], D.px[s], D.py[s], D.pz[s]})
+    for (double* p : {D.ux[s], D.uy[s], D.uz[s], D.px[s], D.py[s], D.pz[s], D.ix[s], D.iy[s], D.iz[s]})
       HIP_OK(c, hipMemsetAsync(p + off, 0, sizeof(double) * n, c->stream));
-  for (double* p : {D.vx, D.vy, D.vz, D.fx, D.fy, D.fz, D.ix, D.iy, D.iz})
+  for (double* p : {D.vx, D.vy, D.vz, D.fx, D.fy, D.fz})
     HIP_OK(c, hipMemsetAsync(p + off, 0, sizeof(double) * n, c->stream));
   HIP_OK(c, hipMemsetAsync(D.cnt + off, 0, sizeof(int) * n, c->stream));
   HIP_OK(c, hipMemsetAsync(D.red + (size_t)off * kRed, 0, sizeof(double) * n * kRed, c->stream));
@@ -712,8 +713,10 @@ int rebuild_one(sopg_ctx* c, int chain, int mode, int* nnl_att, int* nnl_rep) {
   if (mode == SOPG_CUTOFF_CELL && (!(h->p.ncell > 0.0) || !(h->p.lcell > 0.0)))
     return fail(c, SOPG_ERR_ARG, "cell list needs ncell > 0 and lcell > 0");
   if (mode == SOPG_CUTOFF_CELL) launch_cellcoords(c->D, c->cur, P->off, P->off + P->n, c->stream);
-  launch_rebuild(c->list_lanes, c->D, c->d_T, c->cur, c->par, 0, mode, P->off, P->off + P->n, chain, 1, c->binned, 0,
-                 c->stream);
+  // a neighbor-list rebuild also (re)builds the chain's candidate list, which
+  // the rebuilds of the following runs filter
+  launch_rebuild(c->list_lanes, c->D, c->d_T, c->cur, c->par, 0, mode, P->off, P->off + P->n, chain, 1, c->binned,
+                 mode == SOPG_CUTOFF_NEIGHBOR && c->skin > 0.0, c->stream);
   HIP_OK(c, hipGetLastError());
   if ((rc = check_err_flag(c))) return rc;
   h->pl_fresh = true;
@@ -826,13 +829,23 @@ int parse_topology(sopg_ctx* c, const sopg_topology* t, HostChain& h) {
 
 namespace {
 
+// Captured step segments bake in which rebuild kernels run (the candidate
+// filter only with a skin): a change of skin or tables drops them.
+int drop_graphs(sopg_ctx* c) {
+  if (c->gcache.empty()) return SOPG_OK;
+  HIP_OK(c, hipStreamSynchronize(c->stream));
+  for (auto& ge : c->gcache) (void)hipGraphExecDestroy(ge.exec);
+  c->gcache.clear();
+  return SOPG_OK;
+}
+
 // Candidate lists built with other cutoffs or another skin are rebuilt at
 // each chain's next neighbor-list rebuild.
-int invalidate_candidates(sopg_ctx* c) {
+int invalidate_candidates(sopg_ctx* c, int only = -1) {
   if (!c->device_ready) return SOPG_OK;
   const int zero = 0;
   for (int id = 0; id < (int)c->chains.size(); id++)
-    if (c->chains[id].active && c->chains[id].on_device)
+    if (c->chains[id].active && c->chains[id].on_device && (only < 0 || id == only))
       HIP_OK(c, hipMemcpyAsync(&c->D.grid[id].cand_valid, &zero, sizeof zero, hipMemcpyHostToDevice, c->stream));
   HIP_OK(c, hipStreamSynchronize(c->stream));
   return SOPG_OK;
@@ -914,7 +927,7 @@ int sopg_set_tables(sopg_ctx* c, const sopg_tables* t) {
     HIP_OK(c, hipMemcpyAsync(c->d_T, &c->T, sizeof(Tables), hipMemcpyHostToDevice, c->stream));
     HIP_OK(c, hipStreamSynchronize(c->stream));
     int rc;
-    if ((rc = upload_chainp(c))) return rc;
+    if ((rc = upload_chainp(c)) || (rc = drop_graphs(c))) return rc;
     return invalidate_candidates(c);
   }
   return SOPG_OK;
@@ -970,6 +983,8 @@ int sopg_add_chain(sopg_ctx* c, const sopg_topology* t, const sopg_chain_params*
   HostChain h;
   h.p = *p;
   if ((rc = parse_topology(c, t, h))) return rc;
+  for (double r : h.ar) h.max_r2 = std::max(h.max_r2, r * r);
+  for (int8_t ty : h.type) h.type_mask |= 1 << ty;
   h.active = true;
   // reuse the lowest free chain slot
   size_t slot = c->chains.size();
@@ -1010,7 +1025,9 @@ int sopg_set_chain_params(sopg_ctx* c, int chain, const sopg_chain_params* p) {
   int rc = check_params(c, p);
   if (rc) return rc;
   c->chains[chain].p = *p;
-  return upload_chainp(c, chain);
+  // a new box or cutoff changes what the chain's candidate list must hold
+  if ((rc = upload_chainp(c, chain))) return rc;
+  return invalidate_candidates(c, chain);
 }
 
 int sopg_nchains(const sopg_ctx* c) {
@@ -1468,7 +1485,7 @@ int sopg_set_skin(sopg_ctx* c, double skin) {
     HIP_OK(c, hipMemcpyAsync(c->d_T, &c->T, sizeof(Tables), hipMemcpyHostToDevice, c->stream));
     HIP_OK(c, hipStreamSynchronize(c->stream));
     int rc;
-    if ((rc = upload_chainp(c)) || (rc = invalidate_candidates(c))) return rc;
+    if ((rc = upload_chainp(c)) || (rc = drop_graphs(c)) || (rc = invalidate_candidates(c))) return rc;
   }
   return SOPG_OK;
 }
@@ -1664,7 +1681,7 @@ int sopg_run(sopg_ctx* c, long nsteps, const double* noise, sopg_record* rec, lo
     if (h.active && !h.resident && !h.pl_fresh) fresh = false;
   if (!fresh) {
     pe = prof_begin(c);
-    launch_pairfilter(c->list_lanes, D, c->d_T, c->cur, c->par, 0, nb, s);
+    launch_pairfilter(L, D, c->d_T, c->cur, c->par, 0, nb, s);  // a few dozen entries per bead: step lanes
     prof_end(c, SOPG_KERNEL_PAIRLIST, pe);
   }
   if (stepped) launch_posupdate(D, c->cur, c->cur, 0, nb, s);
@@ -1728,7 +1745,7 @@ int sopg_run(sopg_ctx* c, long nsteps, const double* noise, sopg_record* rec, lo
     // key: launch shape, parities, the segment's rebuild pattern, device layout
     std::vector<char> key;
     auto add = [&](const void* p, size_t n) { key.insert(key.end(), (const char*)p, (const char*)p + n); };
-    const int hdr[6] = {(int)S, c->cur, c->par, L, c->list_lanes, c->binned};
+    const int hdr[7] = {(int)S, c->cur, c->par, L, c->list_lanes, c->binned, c->skin > 0.0 ? 1 : 0};
     add(hdr, sizeof hdr);
     for (long k = 0; k < S; k++) {
       const auto d = due(r + k + 1);

@@ -5,7 +5,9 @@
 // chain offset + local bead index):
 //   unc_pos x/y/z  two buffers (the step kernel reads one, writes the next)
 //   pos x/y/z      two buffers (wrapped copy, output only)
-//   vel, force, incr x/y/z
+//   vel, force x/y/z
+//   incr x/y/z     two buffers, paired with the position buffers: incr[b] is the
+//                  increment that moved the beads into positions b
 //   per-bead incidence lists ([beg, end) ranges, in the reference's list order):
 //     bonds, angles, native (attractive) contacts, and -- for explicit
 //     non-native master lists -- non-native contacts
@@ -93,7 +95,8 @@ struct Dev {
   int cap;   // non-native slots per bead
   double *ux[2], *uy[2], *uz[2];
   double *px[2], *py[2], *pz[2];
-  double *vx, *vy, *vz, *fx, *fy, *fz, *ix, *iy, *iz;
+  double *vx, *vy, *vz, *fx, *fy, *fz;
+  double *ix[2], *iy[2], *iz[2];  // incr[b]: the increment that produced positions b
   int* chain;   // chain id of each bead, -1 for a free bead
   int8_t* type;
   // bonds, per incidence: partner << 1 | side (0: bead is i, 1: bead is j), r0

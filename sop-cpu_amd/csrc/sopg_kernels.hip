@@ -399,7 +399,8 @@ __global__ void __launch_bounds__(kGridThreads) k_grid(Dev D, const Tables* __re
     if (threadIdx.x == 0) { G.work = W_SCAN; D.plan[2] = rel; }
     return;
   }
-  const bool cand = mode < 0 && crit == SOPG_CUTOFF_NEIGHBOR && T->cand_on;
+  // neighbor-list rebuilds, in a run or standalone, go through candidate lists
+  const bool cand = crit == SOPG_CUTOFF_NEIGHBOR && T->cand_on;
   const double* px = cur ? D.ux[1] : D.ux[0];
   const double* py = cur ? D.uy[1] : D.uy[0];
   const double* pz = cur ? D.uz[1] : D.uz[0];
@@ -739,7 +740,11 @@ __global__ void __launch_bounds__(256) k_rebuild_grid(Dev D, const Tables* __res
   }
   if (l == 0) {
     (cmode ? D.ccnt : D.cnt)[g] = cnt < D.cap ? cnt : D.cap;
-    if (cnt > D.cap) atomicOr(D.err, 1);
+    if (cnt > D.cap) {
+      atomicOr(D.err, 1);
+      // a truncated candidate list must not serve later rebuilds
+      if (cmode) D.grid[D.chain[g]].cand_valid = 0;
+    }
   }
 }
 
@@ -954,7 +959,7 @@ __device__ __forceinline__ void posupdate_bead(const Dev& D, int src, int dst, i
   const double ix = od ? P.a5 * D.fx[g] : P.a1 * D.vx[g] + P.a2 * D.fx[g];
   const double iy = od ? P.a5 * D.fy[g] : P.a1 * D.vy[g] + P.a2 * D.fy[g];
   const double iz = od ? P.a5 * D.fz[g] : P.a1 * D.vz[g] + P.a2 * D.fz[g];
-  D.ix[g] = ix; D.iy[g] = iy; D.iz[g] = iz;
+  (dst ? D.ix[1] : D.ix[0])[g] = ix; (dst ? D.iy[1] : D.iy[0])[g] = iy; (dst ? D.iz[1] : D.iz[0])[g] = iz;
   (dst ? D.px[1] : D.px[0])[g] = mimg((src ? D.px[1] : D.px[0])[g] + ix, P.boxl, P.qbox);
   (dst ? D.py[1] : D.py[0])[g] = mimg((src ? D.py[1] : D.py[0])[g] + iy, P.boxl, P.qbox);
   (dst ? D.pz[1] : D.pz[0])[g] = mimg((src ? D.pz[1] : D.pz[0])[g] + iz, P.boxl, P.qbox);
@@ -968,17 +973,6 @@ __global__ void k_posupdate(Dev D, int src, int dst, int g0, int g1) {
   if (g >= g1 || D.chain[g] < 0) return;  // past the end, or a free bead
   if (D.cp[D.chain[g]].resident) return;  // the resident kernel moves its own
   posupdate_bead(D, src, dst, g);
-}
-
-// Velocity half of underdamped_iteration (sop.cpp:316-328).
-__global__ void k_velupdate(Dev D, int g0, int g1) {
-  const int g = g0 + blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= g1 || D.chain[g] < 0) return;  // past the end, or a free bead
-  const ChainP& P = D.cp[D.chain[g]];
-  if (!P.vel_on || P.sim_type == 2) return;
-  D.vx[g] = P.a3 * D.ix[g] + P.a4 * D.fx[g];
-  D.vy[g] = P.a3 * D.iy[g] + P.a4 * D.fy[g];
-  D.vz[g] = P.a3 * D.iz[g] + P.a4 * D.fz[g];
 }
 
 // One force evaluation (force_eval, energy.cpp:21-33) at unc_pos[cur] with the
@@ -1286,7 +1280,8 @@ __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restric
   const int vel_on = ci.vel_on, sim = ci.sim_type;
   const double a1 = ci.a1, a2 = ci.a2, a3 = ci.a3, a4 = ci.a4, a5 = ci.a5;
   const double vx0 = D.vx[g], vy0 = D.vy[g], vz0 = D.vz[g];
-  const double ix0 = D.ix[g], iy0 = D.iy[g], iz0 = D.iz[g];
+  const double ix0 = (cur ? D.ix[1] : D.ix[0])[g], iy0 = (cur ? D.iy[1] : D.iy[0])[g];
+  const double iz0 = (cur ? D.iz[1] : D.iz[0])[g];
   double psx = 0.0, psy = 0.0, psz = 0.0;
   if constexpr (NEXT) {
     psx = (cur ? D.px[1] : D.px[0])[g];
@@ -1308,7 +1303,7 @@ __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restric
     const double ix = od ? a5 * Fx : a1 * vx + a2 * Fx;
     const double iy = od ? a5 * Fy : a1 * vy + a2 * Fy;
     const double iz = od ? a5 * Fz : a1 * vz + a2 * Fz;
-    D.ix[g] = ix; D.iy[g] = iy; D.iz[g] = iz;
+    (cur ? D.ix[0] : D.ix[1])[g] = ix; (cur ? D.iy[0] : D.iy[1])[g] = iy; (cur ? D.iz[0] : D.iz[1])[g] = iz;
     (cur ? D.px[0] : D.px[1])[g] = mimg(psx + ix, boxl, qbox);
     (cur ? D.py[0] : D.py[1])[g] = mimg(psy + iy, boxl, qbox);
     (cur ? D.pz[0] : D.pz[1])[g] = mimg(psz + iz, boxl, qbox);
@@ -1382,7 +1377,12 @@ __device__ __forceinline__ void observe_bead(const Dev& D, int cur, int par, int
   // kinT source: velocities (underdamped, sop.cpp:263-269) or position
   // increments (overdamped, sop.cpp:271-278; scaled by zeta/(2h) on the host)
   const bool od = P.sim_type == 2;
-  const double qx = od ? D.ix[g] : D.vx[g], qy = od ? D.iy[g] : D.vy[g], qz = od ? D.iz[g] : D.vz[g];
+  // the increment of the step that produced positions cur (the reference's
+  // calculate_observables(incr) right after the iteration)
+  const double* ix = cur ? D.ix[1] : D.ix[0];
+  const double* iy = cur ? D.iy[1] : D.iy[0];
+  const double* iz = cur ? D.iz[1] : D.iz[0];
+  const double qx = od ? ix[g] : D.vx[g], qy = od ? iy[g] : D.vy[g], qz = od ? iz[g] : D.vz[g];
   const double v2 = qx * qx + qy * qy + qz * qz;
   r[R_CONT] = cont; r[R_RG] = rg; r[R_V2] = v2;
 }
@@ -1551,7 +1551,7 @@ void launch_rebuild(int L, const Dev& D, const Tables* T, int cur, int par, int 
     hipLaunchKernelGGL(k_rebuild_grid<kGridLanes>, dim3((unsigned)((g1 - g0 + kGridBeads - 1) / kGridBeads)),
                        dim3(256), (size_t)kGridBeads * 2 * words * sizeof(uint32_t), s, Dg, T, cur, par, rel, mode,
                        g0, g1);
-    if (cand && mode < 0)
+    if (cand)
       hipLaunchKernelGGL(k_cand_filter<kFilterLanes>, dim3(blocks_for((long)(g1 - g0) * kFilterLanes)), dim3(256), 0, s,
                          D, T, cur, par, rel, g0, g1);
   }
@@ -1572,9 +1572,6 @@ void launch_force(int L, const Dev& D, const Tables* T, int cur, int pr, int rel
 }
 void launch_posupdate(const Dev& D, int src, int dst, int g0, int g1, hipStream_t s) {
   hipLaunchKernelGGL(k_posupdate, dim3(blocks_for(g1 - g0)), dim3(256), 0, s, D, src, dst, g0, g1);
-}
-void launch_velupdate(const Dev& D, int g0, int g1, hipStream_t s) {
-  hipLaunchKernelGGL(k_velupdate, dim3(blocks_for(g1 - g0)), dim3(256), 0, s, D, g0, g1);
 }
 void launch_observe(const Dev& D, int cur, int par, int rel, int all, int what, int g0, int g1, hipStream_t s) {
   hipLaunchKernelGGL(k_observe, dim3(blocks_for(g1 - g0)), dim3(256), 0, s, D, cur, par, rel, all, what, g0, g1);

@@ -431,6 +431,10 @@ class Context:
         nz = None if noise is None else np.ascontiguousarray(noise, dtype=np.float64)
         self._chk(self.L.sopg_underdamped_iteration(self.h, chain, _ptr(nz)))
 
+    def overdamped_iteration(self, chain=0, noise=None):
+        nz = None if noise is None else np.ascontiguousarray(noise, dtype=np.float64)
+        self._chk(self.L.sopg_overdamped_iteration(self.h, chain, _ptr(nz)))
+
     def energy_eval(self, chain=0):
         e = Energies()
         self._chk(self.L.sopg_energy_eval(self.h, chain, C.byref(e)))

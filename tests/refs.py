@@ -29,8 +29,6 @@ class Oracle:
             L.orc_destroy.argtypes = [_vp]
             for f in ("orc_set", "orc_load"):
                 getattr(L, f).argtypes = [_vp, C.c_char_p, C.c_char_p]
-            L.orc_load_coords.argtypes = [_vp, C.c_char_p, C.c_char_p]
-            L.orc_load_vels.argtypes = [_vp, C.c_char_p]
             L.orc_get_int.argtypes = [_vp, C.c_char_p]
             L.orc_get_double.argtypes = [_vp, C.c_char_p]
             L.orc_get_double.restype = C.c_double

@@ -705,3 +705,168 @@ def test_bead_pool_limit_refused_and_context_usable():
     st = ctx.get_state(c)
     assert np.all(np.isfinite(st["unc"]))
     ctx.close()
+
+
+def run_oracle_od_ctrl(orc, nsteps, nnlup, nup, inlup=0, iup=1, istep0=1):
+    """Hot loop of overdamped_ctrl (sop.cpp:428-466) on the oracle: the global inlup starts at 0,
+    so the list is rebuilt on the first step; records use the step's increments (kinT)."""
+    recs = []
+    for istep in range(istep0, istep0 + nsteps):
+        if inlup % nnlup == 0:
+            orc.call("update_neighbor_list")
+            inlup = 0
+        inlup += 1
+        orc.call("update_pair_list")
+        orc.call("overdamped_iteration")
+        if iup % nup == 0:
+            orc.call("energy_eval")
+            orc.call("calculate_observables")
+            recs.append(dict(istep=istep, e_bnd=orc.getd("e_bnd"), e_ang_ss=orc.getd("e_ang_ss"),
+                             e_vdw_rr=orc.getd("e_vdw_rr"), rna_etot=orc.getd("rna_etot"),
+                             Q=orc.getd("Q"), contct_nat=orc.geti("contct_nat"),
+                             rgsq=orc.getd("rgsq"), kinT=orc.getd("kinT")))
+            iup = 0
+        iup += 1
+    return recs
+
+
+def od_cmds(nnlup, nup, temp=0.6):
+    return [("dynamics", "overdamped"), ("zeta", "50"), ("temp", str(temp)), ("cutofftype", "neighborlist"),
+            ("nnlup", str(nnlup)), ("nup", str(nup))]
+
+
+@pytest.mark.parametrize("lanes", [1, 4])
+def test_overdamped_trajectory_vs_oracle(inputs_root, pow_cr, lanes):
+    """Overdamped BD (sop.cpp:331-471) with the reference's noise replayed: 1000 steps in runs
+    of 157 and 843 steps, so that records (every 40 steps, kinT from the step's increments,
+    sop.cpp:271-278) fall in the middle of a run, where the next step's increment is already
+    computed.  Forces bit-identical, positions within 1e-9 A, records as the oracle's."""
+    n, nnlup, nup = 120, 20, 40
+    params = sopg.chain_params(h=0.02, coeffs=(0.0, 0.0, 0.0, 0.0, 0.02 / 50.0), cutoff=sopg.CUTOFF_NEIGHBOR,
+                               nnlup=nnlup, nup=nup, noise_mode=sopg.NOISE_REPLAY, sim_type=2)
+    x, v, d, orc, ctx = make_pair(inputs_root, n, 19, od_cmds(nnlup, nup), params, coords_vel=False, lanes=lanes)
+    assert orc.getd("a5") == params.a5 and orc.getd("h") == params.h
+    print(f"[gpu] overdamped N={n} lanes={lanes}", flush=True)
+    noise = Oracle().gasdev(3 * n * 1002)
+    # overdamped_ctrl preamble (sop.cpp:396-424): lists, energy, first force
+    orc.call("update_neighbor_list"); orc.call("update_pair_list")
+    ctx.update_neighbor_list(); ctx.update_pair_list()
+    orc.set_noise(noise)
+    orc.call("energy_eval"); orc.call("force_eval")
+    ctx.force_eval(0, noise[:3 * n])
+    assert np.array_equal(ctx.get_state(0)["force"], orc.coords(3))
+    ctx.set_step(0, 1.0, 0, 1)  # the global inlup = 0: rebuild on the first step
+    orecs = run_oracle_od_ctrl(orc, 1000, nnlup, nup)
+    q = 3 * n
+    drecs = ctx.run(157, noise[q:q + 157 * 3 * n])
+    q += 157 * 3 * n
+    drecs += ctx.run(843, noise[q:q + 843 * 3 * n])
+    s = ctx.get_state(0)
+    for w, k in ((1, "unc"), (0, "pos")):
+        assert np.abs(s[k] - orc.coords(w)).max() <= TRAJ_ABS, k
+    assert np.array_equal(s["unc"], orc.coords(1)), "expected bit-identical trajectory"
+    assert np.array_equal(s["force"], orc.coords(3))
+    compare_records(drecs, orecs)
+    assert len(drecs) == 25 and all(r["kinT"] > 0.0 for r in drecs)
+    # the standalone entry point (sop.cpp:331-363) continues the same trajectory
+    q = 3 * n * 1001
+    orc.call("update_pair_list")
+    orc.call("overdamped_iteration")
+    ctx.update_pair_list()
+    ctx.overdamped_iteration(0, noise[q:q + 3 * n])
+    s = ctx.get_state(0)
+    assert np.array_equal(s["unc"], orc.coords(1)) and np.array_equal(s["force"], orc.coords(3))
+    orc.call("calculate_observables")
+    ob = ctx.observables()
+    assert rel(ob["kinT"], orc.getd("kinT")) <= 1e-12
+
+
+@pytest.mark.parametrize("lanes", [1, 4, 16])
+def test_vdw_quirks_vs_oracle(tmp_path, lanes):
+    """The reference's two skip tests as coded (energy.cpp:292-293, 331) on a constructed case in
+    which both decide: a native pair-list entry skipped through another master entry's
+    lj_nat_pdb_dist2, and a non-native pair skipped through the last native pair-list entry's
+    rep_tol.  At T = 0 the skipped beads stay at rest, so the quirks fire at every step of a
+    50-step run too: forces, energies and positions equal the oracle's bit for bit."""
+    from conftest import quirk_system
+    d = str(tmp_path / "q")
+    x, topo = quirk_system(d)
+    nnlup = 10
+    orc = Oracle()
+    setup_system(orc, d, std_cmds(temp=0.0, nnlup=nnlup, nup=10))
+    params = sopg.chain_params(T=0.0, cutoff=sopg.CUTOFF_NEIGHBOR, nnlup=nnlup, nup=10,
+                               noise_mode=sopg.NOISE_REPLAY)
+    ctx = sopg.Context(lanes=lanes, list_lanes=lanes)
+    ctx.add_chain(topo, params, implicit=False)
+    z = np.zeros_like(x)
+    ctx.put_state(0, pos=x, unc=x, vel=z, force=z)
+    orc.set_coords(2, z)
+    print(f"[gpu] vdw quirks lanes={lanes}", flush=True)
+    assert ctx.update_neighbor_list() == (2, 2)  # (1,5) and (2,6) are too far for the list
+    ctx.update_pair_list()
+    orc.call("update_neighbor_list"); orc.call("update_pair_list")
+    for w in range(4):
+        check_lists(ctx, orc, w, w)
+    noise = Oracle().gasdev(3 * 10 * 52)
+    orc.set_noise(noise)
+    orc.call("energy_eval"); orc.call("force_eval")
+    ctx.force_eval(0, noise[:30])
+    f = ctx.get_state(0)["force"]
+    assert np.array_equal(f, orc.coords(3))
+    F = f.reshape(-1, 3)
+    assert all(np.all(F[b - 1] == 0.0) for b in (1, 3, 7, 9))  # skipped: no force at all
+    e = ctx.energy_eval()
+    for k in ("e_vdw_rr", "rna_etot"):
+        assert rel(e[k], orc.getd(k)) <= ENERGY_REL, k
+    orecs = run_oracle_ctrl(orc, 50, nnlup, 10)
+    drecs = ctx.run(50, noise[30:30 + 50 * 30])
+    s = ctx.get_state(0)
+    assert np.array_equal(s["unc"], orc.coords(1)) and np.array_equal(s["force"], orc.coords(3))
+    assert np.array_equal(s["unc"].reshape(-1, 3)[[0, 2, 6, 8]], x.reshape(-1, 3)[[0, 2, 6, 8]])
+    compare_records(drecs, orecs)
+
+
+def test_skin_change_between_runs(inputs_root):
+    """Runs with captured graphs at skin 0, then at skin 3 on the same context (the second needs
+    the candidate filter the first's graphs never launched), equal a context that ran at the
+    default skin throughout, bit for bit."""
+    chains = []
+    for n, seed in ((80, 161), (220, 162), (400, 163)):
+        x, v, _ = chain_inputs(inputs_root, n, seed)
+        chains.append((x, v, 30 + seed))
+    print("[gpu] skin change between runs", flush=True)
+    a = _philox_ctx(chains)
+    a.set_skin(0.0)
+    ra = a.run(140)
+    a.set_skin(3.0)
+    ra += a.run(200)
+    b = _philox_ctx(chains)
+    rb = b.run(140) + b.run(200)
+    assert a.graph_stats()[1] > 0
+    for c in range(len(chains)):
+        sa, sb = a.get_state(c), b.get_state(c)
+        for k in ("unc", "vel", "force", "pos"):
+            assert np.array_equal(sa[k], sb[k]), (c, k)
+        for w in (sopg.LIST_NL_REP, sopg.LIST_PL_REP):
+            (ai, aj), (bi, bj) = a.get_list(w, c), b.get_list(w, c)
+            assert np.array_equal(ai, bi) and np.array_equal(aj, bj)
+    assert ra == rb
+
+
+def test_candidate_overflow_is_an_error_every_time(inputs_root):
+    """A candidate list longer than the capacity (its radius is wider than the neighbor list's)
+    is reported by the run that builds it and by every later run: a truncated candidate list
+    never serves a rebuild."""
+    x, v, _ = chain_inputs(inputs_root, 300, 111)
+    ctx = sopg.Context(capacity=96)
+    ctx.set_skin(0.0)
+    ctx.add_chain(sopg.chain_topology(x), sopg.chain_params(cutoff=sopg.CUTOFF_NEIGHBOR, nnlup=20, nup=100,
+                                                             noise_mode=sopg.NOISE_PHILOX, seed=5))
+    _setup_chain(ctx, 0, x, v)
+    print("[gpu] candidate overflow", flush=True)
+    ctx.run(40)  # neighbor lists alone fit
+    ctx.set_skin(4.0)
+    for _ in range(3):
+        with pytest.raises(sopg.SopgError, match="capacity"):
+            ctx.run(40)
+    ctx.close()

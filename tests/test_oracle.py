@@ -80,6 +80,142 @@ def test_oracle_equals_reference_trajectory(inputs_root, cutoff):
         assert same_lists(o, r, w), w
 
 
+def od_cmds(temp=0.6, cutoff="neighborlist", nnlup=20, nup=1000, extra=()):
+    """Overdamped dynamics settings (params.cpp:30-33: h = 0.02, a5 = h / zeta)."""
+    c = [("dynamics", "overdamped"), ("zeta", "50"), ("temp", str(temp)), ("cutofftype", cutoff),
+         ("nnlup", str(nnlup)), ("nup", str(nup))]
+    return c + list(extra)
+
+
+@needs_ref
+@pytest.mark.parametrize("cutoff", ["neighborlist", "celllist"])
+def test_oracle_equals_reference_overdamped(inputs_root, cutoff):
+    """Overdamped BD (sop.cpp:331-363, 365-471): oracle vs compiled reference bit-identical for
+    400 steps -- increments, state, lists, energies and observables (kinT from the increments,
+    sop.cpp:271-278).  The schedule is overdamped_ctrl's: its first rebuild falls on step 1
+    (the global inlup starts at 0, sop.cpp:428)."""
+    n = 120
+    extra = [("boxl", "504"), ("ncell", "42")] if cutoff == "celllist" else []
+    x, v, d, o, r = both(inputs_root, n, 31, od_cmds(cutoff=cutoff, extra=extra))
+    assert o.geti("sim_type") == 2 and r.geti("sim_type") == 2
+    assert o.getd("a5") == r.getd("a5") == 0.02 / 50.0
+    upd = "update_neighbor_list" if cutoff == "neighborlist" else "update_cell_list"
+    noise = Oracle().gasdev(3 * n * 401)
+    o.set_noise(noise)
+    r.reset_rng(-101, inputs_root)
+    for f in (upd, "update_pair_list"):
+        o.call(f)
+        r.call(f)
+    r.call("set_potential_forces")
+    for f in ("energy_eval", "force_eval"):
+        o.call(f)
+        r.call(f)
+    assert np.array_equal(o.coords(3), r.coords(3))
+    inlup = 0
+    for step in range(1, 401):
+        if inlup % 20 == 0:
+            o.call(upd)
+            r.call(upd)
+            inlup = 0
+        inlup += 1
+        o.call("update_pair_list")
+        r.call("update_pair_list")
+        o.call("overdamped_iteration")
+        r.call("overdamped_iteration")
+        if step % 50 == 0 or step in (1, 7, 333):
+            assert np.array_equal(o.coords(4), r.coords(4)), step  # incr
+            for f in ("energy_eval", "calculate_observables"):
+                o.call(f)
+                r.call(f)
+            for k in ("e_bnd", "e_ang_ss", "e_vdw_rr", "rna_etot", "Q", "rgsq", "kinT"):
+                assert o.getd(k) == r.getd(k), (step, k)
+            assert o.getd("kinT") > 0.0
+    for w in range(5):
+        assert np.array_equal(o.coords(w), r.coords(w)), w
+    for w in range(4):
+        assert same_lists(o, r, w), w
+    # the dynamics moved the chain: the lists are not the native ones any more
+    assert not np.array_equal(o.coords(1), x)
+
+
+def test_overdamped_free_particle_closed_form(tmp_path):
+    """F = const: one overdamped step moves a bead by a5 F (sop.cpp:339-351); kinT is
+    zeta / (2 h) * |incr|^2 / 3 (sop.cpp:271-278)."""
+    with open(tmp_path / "bonds.dat", "w") as fh:
+        fh.write("bonds 0\n")
+    with open(tmp_path / "angles.dat", "w") as fh:
+        fh.write("angles 0\n")
+    with open(tmp_path / "vdw.dat", "w") as fh:
+        fh.write("natt 0 nrep 0\n")
+    with open(tmp_path / "init.dat", "w") as fh:
+        fh.write("beads 1\n1 1.0 2.0 3.0\n")
+    o = Oracle()
+    setup_system(o, str(tmp_path), [("dynamics", "overdamped"), ("zeta", "50"), ("temp", "0.5")])
+    g = np.array([0.7, -1.1, 0.3])
+    o.set_noise(np.concatenate([g, g]))
+    F0 = np.array([2.0, -4.0, 6.0])
+    o.set_coords(3, F0)
+    a5 = 0.02 / 50.0
+    o.call("overdamped_iteration")
+    assert np.array_equal(o.coords(4), a5 * F0)
+    assert np.array_equal(o.coords(1), np.array([1.0, 2.0, 3.0]) + a5 * F0)
+    var = math.sqrt(2.0 * 0.5 * 50.0 / 0.02)
+    assert np.allclose(o.coords(3), var * g, rtol=1e-15)  # the new force: the noise term alone
+    o.call("calculate_observables")
+    want = (a5 * F0 * (a5 * F0)).sum() * 50.0 / (2.0 * 0.02) / 3.0
+    assert o.getd("kinT") == pytest.approx(want, rel=1e-14)
+
+
+def _quirk_forces(sysobj):
+    for f in ("update_neighbor_list", "update_pair_list", "clear_forces", "vdw_forces"):
+        sysobj.call(f)
+    return sysobj.coords(3).reshape(-1, 3)
+
+
+def test_vdw_quirks_closed_form(tmp_path):
+    """Both skip tests of vdw_forces as coded (energy.cpp:292-293, 331) on the constructed case:
+    the native pair at pair-list position 1 is skipped through lj_nat_pdb_dist2[1] of ANOTHER
+    master entry, and the non-native pair through the rep_tol of the last native pair-list
+    entry's types; only (4,8) and (2,10) act, with their analytic pair forces."""
+    from conftest import quirk_system
+    x, topo = quirk_system(str(tmp_path / "q"))
+    o = Oracle()
+    setup_system(o, str(tmp_path / "q"), std_cmds())
+    F = _quirk_forces(o)
+    ai, aj = o.lst(2)
+    assert list(zip(ai, aj)) == [(3, 7), (4, 8)]   # pair-list order: (3,7) sits at position 1
+    ri, rj = o.lst(3)
+    assert list(zip(ri, rj)) == [(1, 9), (2, 10)]
+    for b in (1, 3, 7, 9):                           # the skipped pairs' beads feel nothing
+        assert np.all(F[b - 1] == 0.0), b
+    # native (4,8), types 2 2: force_coeff_att = -12 * 0.7, r = 5, d = 5.5 along +x
+    d = 5.5
+    co = -12.0 * 0.7 / d ** 2 * ((5.0 / d) ** 12 - (5.0 / d) ** 6)
+    assert F[3, 0] == pytest.approx(co * d, rel=1e-13) and F[7, 0] == pytest.approx(-co * d, rel=1e-13)
+    # non-native (2,10), types 1 1: sigma 3.8, d = 5 along +x
+    d, s = 5.0, 3.8
+    co = -6.0 / d ** 2 * (2 * (s / d) ** 12 + (s / d) ** 6)
+    assert F[1, 0] == pytest.approx(co * d, rel=1e-13) and F[9, 0] == pytest.approx(-co * d, rel=1e-13)
+
+
+@needs_ref
+def test_vdw_quirks_equal_reference(tmp_path):
+    """The constructed quirk case: oracle forces and energies bit-identical to the reference's."""
+    from conftest import quirk_system
+    quirk_system(str(tmp_path / "q"))
+    o, r = Oracle(), Ref()
+    setup_system(o, str(tmp_path / "q"), std_cmds())
+    setup_system(r, str(tmp_path / "q"), std_cmds())
+    r.call("set_potential_forces")
+    assert np.array_equal(_quirk_forces(o), _quirk_forces(r))
+    for w in range(4):
+        assert same_lists(o, r, w), w
+    o.call("energy_eval")
+    r.call("energy_eval")
+    for k in ("e_bnd", "e_ang_ss", "e_vdw_rr", "rna_etot"):
+        assert o.getd(k) == r.getd(k), k
+
+
 @needs_ref
 def test_ran2_gasdev_match_reference(tmp_path):
     o = Oracle()
