@@ -181,6 +181,18 @@ int sopg_get_list(sopg_ctx* ctx, int chain, int which, int* ibead, int* jbead, l
  * chains in ascending id order, 3N each, bead-major.  Records go to rec (up to
  * rec_cap), in step order and ascending chain id; *nrec = count produced. */
 int sopg_run(sopg_ctx* ctx, long nsteps, const double* noise, sopg_record* rec, long rec_cap, long* nrec);
+/* sopg_run without any device allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * steps, states and records as sopg_run, with these limits: every chain uses
+ * the device generator (no replayed noise), the context has already made its
+ * first device operation (it allocates there), kernel timing is off, and the
+ * records of one call fit a fixed 4 MiB per-GPU scratch array: (record steps
+ * in the call) x (chain slots) x 12 doubles <= 2^19, else SOPG_ERR_ARG.  It
+ * launches every kernel directly (no graph capture, no resident kernel).
+ * Every context on one GPU shares that GPU's scratch array: calls that
+ * overlap on one GPU (two host threads) race on it. */
+int sopg_run_noalloc(sopg_ctx* ctx, long nsteps, sopg_record* rec, long rec_cap, long* nrec);
 /* Set the step counters of a chain: istep of the next step, and the
  * reference's inlup / iup counters as they stand before that step. */
 int sopg_set_step(sopg_ctx* ctx, int chain, double istep, int inlup, int iup);

@@ -35,6 +35,7 @@ void launch_posupdate(const Dev& D, int src, int dst, int g0, int g1, hipStream_
 void launch_observe(const Dev& D, int cur, int par, int rel, int all, int what, int g0, int g1, hipStream_t s);
 void launch_chain_reduce(const Dev& D, int c0, int nc, double* out, hipStream_t s);
 void launch_set_base(int* base, int value, hipStream_t s);
+int scratch_address(int device, double** out);
 void launch_resident(int L, const Dev& D, const Tables* T, const int* chains, int nchains, int threads, int cur,
                      int par, int nsteps, const int* rec_slot, double* rec, int nc, hipStream_t s);
 enum { F_STORE = 1, F_VEL = 2, F_ENERGY = 4, F_ENERGY_ALL = 8, F_NOISE = 16, F_BOND = 32, F_ANGLE = 64,
@@ -146,6 +147,7 @@ struct sopg_ctx {
   double* d_noise = nullptr;    // replay noise
   size_t noise_cap = 0;         // doubles
   double* d_rec = nullptr;      // per-record-step reductions
+  double* d_scratch = nullptr;  // this GPU's file-scope record staging (sopg_run_noalloc)
   size_t rec_cap_d = 0;
   // graph replay of sopg_run's step segments
   int* d_relb = nullptr;        // [0] = 0 (direct launches), [1] = step base of the replayed segment
@@ -870,6 +872,9 @@ int sopg_create(int device, sopg_ctx** out) {
   }
   sopg_default_tables(&c->tab);
   derive_tables(c->tab, c->skin, c->T);
+  // device address of this GPU's fixed record staging (the device is current);
+  // on failure it stays null and sopg_run_noalloc reports it
+  if (scratch_address(device, &c->d_scratch) != 0) c->d_scratch = nullptr;
   *out = c;
   return SOPG_OK;
 }
@@ -1553,20 +1558,37 @@ int sopg_autotune(sopg_ctx* c, int* step_lanes) {
   return SOPG_OK;
 }
 
-int sopg_run(sopg_ctx* c, long nsteps, const double* noise, sopg_record* rec, long rec_cap, long* nrec) {
+// The body of sopg_run and sopg_run_noalloc.  fixed: stage the records in
+// this GPU's file-scope scratch array and allocate no device memory -- no
+// replayed noise (it is staged in context memory), no graph capture (an
+// instantiated graph holds device memory), no resident kernel (its chain and
+// record-slot lists are context memory), no kernel timing.
+static int run_impl(sopg_ctx* c, long nsteps, const double* noise, sopg_record* rec, long rec_cap, long* nrec,
+                    bool fixed) {
   if (!c) return SOPG_ERR_ARG;
   if (nrec) *nrec = 0;
   if (nsteps < 0) return fail(c, SOPG_ERR_ARG, "nsteps < 0");
-  int rc = ensure_device(c);
-  if (rc) return rc;
+  int rc;
+  if (fixed) {
+    if (!c->device_ready)
+      return fail(c, SOPG_ERR_ARG, "sopg_run_noalloc: the context allocates its device memory at its first device "
+                                   "operation; add the chains and set their state first");
+    if (!c->d_scratch) return fail(c, SOPG_ERR_HIP, "sopg_run_noalloc: no device address for the scratch array");
+    if (c->prof) return fail(c, SOPG_ERR_ARG, "sopg_run_noalloc: kernel timing is on (sopg_profile)");
+  } else if ((rc = ensure_device(c))) {
+    return rc;
+  }
   if (nsteps == 0 || sopg_nchains(c) == 0) return SOPG_OK;
   const bool replay = any_replay(c);
+  if (fixed && replay)
+    return fail(c, SOPG_ERR_ARG, "sopg_run_noalloc: replayed-noise chains need sopg_run (the noise is staged in "
+                                 "context memory)");
   if (replay && !noise) return fail(c, SOPG_ERR_ARG, "replayed-noise chains need the run's noise");
   if (!replay && noise) return fail(c, SOPG_ERR_ARG, "all chains use the device generator: noise must be NULL");
   // small chains run in the resident kernel (not with replayed noise or kernel timing)
   std::vector<int> rlist;
   int rmaxn = 0;
-  if (c->resident_max > 0 && !noise && !c->prof)
+  if (c->resident_max > 0 && !noise && !c->prof && !fixed)
     for (int id = 0; id < (int)c->chains.size(); id++) {
       HostChain& h = c->chains[id];
       if (h.active && h.n <= c->resident_max) {
@@ -1620,7 +1642,16 @@ int sopg_run(sopg_ctx* c, long nsteps, const double* noise, sopg_record* rec, lo
         break;
       }
   const long nrs = (long)rec_steps.size();
-  if ((size_t)nrs * nc * kRed > c->rec_cap_d) {
+  if (fixed && (size_t)nrs * nc * kRed > kScratchDoubles) {
+    char m[256];
+    std::snprintf(m, sizeof m,
+                  "sopg_run_noalloc: %ld record steps x %d chain slots x %d fields exceed the %zu-double scratch "
+                  "array; run fewer steps per call",
+                  nrs, nc, (int)kRed, kScratchDoubles);
+    return fail(c, SOPG_ERR_ARG, m);
+  }
+  double* const rec_buf = fixed ? c->d_scratch : nullptr;
+  if (!fixed && (size_t)nrs * nc * kRed > c->rec_cap_d) {
     HIP_OK(c, hipStreamSynchronize(s));
     dfree(c, c->d_rec);
     c->d_rec = nullptr;
@@ -1699,7 +1730,7 @@ int sopg_run(sopg_ctx* c, long nsteps, const double* noise, sopg_record* rec, lo
     if (records(r)) {
       pe = prof_begin(c);
       launch_observe(Dx, c->cur, c->par, k, 0, OBS_PHYS, 0, nb, s);
-      launch_chain_reduce(Dx, 0, nc, c->d_rec + (size_t)row_of[r] * nc * kRed, s);
+      launch_chain_reduce(Dx, 0, nc, (fixed ? rec_buf : c->d_rec) + (size_t)row_of[r] * nc * kRed, s);
       prof_end(c, SOPG_KERNEL_OBSERVE, pe);
     }
     if (!last) {
@@ -1728,7 +1759,7 @@ int sopg_run(sopg_ctx* c, long nsteps, const double* noise, sopg_record* rec, lo
       if (S > 64) break;
     }
   if (S > 64 || S < 8) S = S > 64 ? 32 : S * ((8 + S - 1) / S);
-  const bool use_graphs = c->graphs && !noise && !c->prof;
+  const bool use_graphs = c->graphs && !noise && !c->prof && !fixed;
   Dev Dg;
   std::memcpy(&Dg, &D, sizeof Dg);  // bytewise: the struct is part of the graph key
   Dg.relb = c->d_relb + 1;
@@ -1791,7 +1822,8 @@ int sopg_run(sopg_ctx* c, long nsteps, const double* noise, sopg_record* rec, lo
   }
   std::vector<double> hr(rec_steps.size() * nc * kRed);
   if (!hr.empty())
-    HIP_OK(c, hipMemcpyAsync(hr.data(), c->d_rec, sizeof(double) * hr.size(), hipMemcpyDeviceToHost, s));
+    HIP_OK(c, hipMemcpyAsync(hr.data(), fixed ? rec_buf : c->d_rec, sizeof(double) * hr.size(),
+                             hipMemcpyDeviceToHost, s));
   HIP_OK(c, hipStreamSynchronize(s));
   prof_collect(c);
   if ((rc = check_err_flag(c))) return rc;
@@ -1828,6 +1860,14 @@ int sopg_run(sopg_ctx* c, long nsteps, const double* noise, sopg_record* rec, lo
     h.noise_base += (uint64_t)nsteps;
   }
   return upload_chainp(c);
+}
+
+int sopg_run(sopg_ctx* c, long nsteps, const double* noise, sopg_record* rec, long rec_cap, long* nrec) {
+  return run_impl(c, nsteps, noise, rec, rec_cap, nrec, false);
+}
+
+int sopg_run_noalloc(sopg_ctx* c, long nsteps, sopg_record* rec, long rec_cap, long* nrec) {
+  return run_impl(c, nsteps, nullptr, rec, rec_cap, nrec, true);
 }
 
 }  // extern "C"

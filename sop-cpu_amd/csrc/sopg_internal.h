@@ -86,6 +86,10 @@ constexpr int kAttTShift = 25;
 // Per-bead reduction fields (red[g * kRed + field]).
 enum { R_EBND = 0, R_EANG, R_EATT, R_EREP, R_CONT, R_RG, R_V2, R_NLA, R_PLA, R_NLR, R_PLR, kRed = 12 };
 
+// Records of sopg_run_noalloc are staged in a file-scope device array of
+// this many doubles (4 MiB; one copy per GPU, held until the process ends).
+constexpr size_t kScratchDoubles = (size_t)1 << 19;
+
 struct Dev {
   // Added to every kernel's relative step argument: 0 for direct launches;
   // for a replayed graph segment the run-relative step at which it starts

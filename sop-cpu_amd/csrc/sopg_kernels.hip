@@ -14,6 +14,7 @@
 // and applies +f / -f); both evaluations use the pair's own (i, j) orientation
 // and therefore produce the same bits.
 #include <hip/hip_runtime.h>
+#include <mutex>
 #include "sopg_device.h"
 #include "sopg_internal.h"
 #include "../../include/sopg.h"
@@ -1485,6 +1486,36 @@ __global__ void __launch_bounds__(512) k_resident(Dev D, const Tables* __restric
       cur ^= 1;
     }
   }
+}
+
+// ------------------------------------------------------------ fixed scratch
+//
+// Record staging of the allocation-free run (sopg_run_noalloc): a file-scope
+// device array instead of context memory.  In host code `scratch` names a
+// host-side stand-in, not a device address, so it is never passed to a kernel
+// or a copy: scratch_address() resolves the device address of this GPU's copy
+// once per device (with that device current) and every use goes through it.
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+static double* g_scratch_ptr[kMaxDevices];
+static bool g_scratch_done[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+// The calling thread's current device must be `device`.
+int scratch_address(int device, double** out) {
+  *out = nullptr;
+  if (device < 0 || device >= kMaxDevices) return -1;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_done[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) return -1;
+    g_scratch_ptr[device] = (double*)p;
+    g_scratch_done[device] = true;
+  }
+  *out = g_scratch_ptr[device];
+  return 0;
 }
 
 // Step base of a replayed graph segment (see Dev::relb).

@@ -94,7 +94,7 @@ EXPORTS = [
     "sopg_server_create", "sopg_server_destroy", "sopg_server_last_error", "sopg_server_now",
     "sopg_server_footprint", "sopg_server_submit", "sopg_server_poll", "sopg_server_job",
     "sopg_server_job_records", "sopg_server_job_state", "sopg_server_release", "sopg_server_stats",
-    "sopg_server_evict",
+    "sopg_server_evict", "sopg_run_noalloc",
 ]
 
 _lib = None
@@ -144,6 +144,7 @@ def lib():
         L.sopg_calculate_observables.argtypes = [vp, C.c_int, C.POINTER(Observables)]
         L.sopg_get_list.argtypes = [vp, C.c_int, C.c_int, _IP, _IP, C.c_long, C.POINTER(C.c_long)]
         L.sopg_run.argtypes = [vp, C.c_long, vp, C.POINTER(Record), C.c_long, C.POINTER(C.c_long)]
+        L.sopg_run_noalloc.argtypes = [vp, C.c_long, C.POINTER(Record), C.c_long, C.POINTER(C.c_long)]
         L.sopg_set_step.argtypes = [vp, C.c_int, C.c_double, C.c_int, C.c_int]
         L.sopg_sync.argtypes = [vp]
         L.sopg_list_counts.argtypes = [vp, C.c_int, _IP, _IP, _IP, _IP]
@@ -463,6 +464,13 @@ class Context:
             r = recs[k]
             out.append({f: getattr(r, f) for f, _ in Record._fields_})
         return out
+
+    def run_noalloc(self, nsteps, rec_cap=100000):
+        """sopg_run_noalloc: the same steps with no device allocation (device noise only)."""
+        recs = (Record * max(rec_cap, 1))()
+        nr = C.c_long()
+        self._chk(self.L.sopg_run_noalloc(self.h, nsteps, recs, rec_cap, C.byref(nr)))
+        return [{f: getattr(recs[k], f) for f, _ in Record._fields_} for k in range(min(nr.value, rec_cap))]
 
     def set_step(self, chain, istep, inlup, iup):
         self._chk(self.L.sopg_set_step(self.h, chain, istep, inlup, iup))
