@@ -875,6 +875,7 @@ int sopg_create(int device, sopg_ctx** out) {
   // device address of this GPU's fixed record staging (the device is current);
   // on failure it stays null and sopg_run_noalloc reports it
   if (scratch_address(device, &c->d_scratch) != 0) c->d_scratch = nullptr;
+
   *out = c;
   return SOPG_OK;
 }

@@ -10,8 +10,10 @@
 // smallest remaining step count among resident jobs, capped by the config.
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <deque>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -70,7 +72,9 @@ struct sopg_server {
   int resident = 0;
   double bead_steps = 0, busy = 0;
   long rounds = 0;
-  double tuned_beads = 0;  // resident beads when the lanes were last autotuned
+  // step-kernel lanes per bead by resident-bead bucket (floor(log2(beads))):
+  // each bucket is autotuned once, the first time the batch falls into it
+  std::map<int, int> lanes_by_bucket;
   std::vector<sopg_record> recbuf;
   double now() const {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -92,7 +96,7 @@ int ctx_fail(sopg_server* s, int rc, const char* what) {
 // native and explicit non-native entry (both incidences included), plus the
 // chain slot and its binning grid.
 double footprint(int cap, const sopg_topology* t) {
-  const double per_bead = 12 * 8 + 12 * 8 + 12 * 4 + 1 + 8.0 * cap + 12 * 8;
+  const double per_bead = 12 * 8 + 15 * 8 + 12 * 4 + 1 + 8.0 * cap + 12 * 8;
   const double per_chain = 512 + 3.0 * 24 * 24 * 24 * 4 + 12 * 8;
   const double nrep = t->ncon_rep > 0 ? t->ncon_rep : 0;
   const double per_native = 4 + 4 + 4 + 1 + 8 + 8 + 4 + 2 * (4 + 4 + 1 + 1 + 8 + 8 + 8);
@@ -186,7 +190,7 @@ int sopg_server_create(int device, const sopg_server_config* cfg, sopg_server** 
     s->cap = s->cfg.capacity;
   }
   // size the bead pool for the budget once, so admissions do not regrow it
-  const double per_bead = 12 * 8 + 12 * 8 + 12 * 4 + 1 + 8.0 * s->cap + 12 * 8;
+  const double per_bead = 12 * 8 + 15 * 8 + 12 * 4 + 1 + 8.0 * s->cap + 12 * 8;
   const long beads = (long)std::min(s->cfg.hbm_budget / per_bead, 4.0e7);
   sopg_reserve(s->ctx, beads, std::min(s->cfg.max_chains, 65536));
   s->t0 = std::chrono::steady_clock::now();
@@ -276,11 +280,18 @@ int sopg_server_poll(sopg_server* s, int* active, int* queued) {
         chunk = std::min(chunk, j.nsteps - j.done);
         beads += (double)j.nbead;
       }
-    // the fastest launch shape depends on the batch: retune when the resident
-    // bead count has moved by more than a quarter since the last tuning
-    if (s->cfg.lanes <= 0 && (s->tuned_beads == 0 || beads > 1.25 * s->tuned_beads || beads < 0.8 * s->tuned_beads)) {
-      if ((rc = sopg_autotune(s->ctx, nullptr))) return ctx_fail(s, rc, "sopg_autotune");
-      s->tuned_beads = beads;
+    // the fastest launch shape depends on the batch size: a bucket of resident
+    // bead counts (a factor of two wide) is autotuned once and then reused
+    if (s->cfg.lanes <= 0) {
+      const int bucket = beads >= 1.0 ? (int)std::floor(std::log2(beads)) : 0;
+      auto it = s->lanes_by_bucket.find(bucket);
+      if (it == s->lanes_by_bucket.end()) {
+        int lanes = 0;
+        if ((rc = sopg_autotune(s->ctx, &lanes))) return ctx_fail(s, rc, "sopg_autotune");
+        s->lanes_by_bucket[bucket] = lanes;
+      } else if ((rc = sopg_set_lanes(s->ctx, it->second, 0))) {
+        return ctx_fail(s, rc, "sopg_set_lanes");
+      }
     }
     long nrec_max = 0;
     for (long id : s->by_chain)

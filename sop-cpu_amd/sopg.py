@@ -454,8 +454,16 @@ class Context:
         self._chk(self.L.sopg_get_list(self.h, chain, which, _ip(i), _ip(j), n.value, C.byref(n)))
         return i[:n.value], j[:n.value]
 
+    def _records(self, rec_cap):
+        # one record buffer per context, reused across runs (allocating and
+        # zeroing megabytes per call would land inside every timed run)
+        rec_cap = max(rec_cap, 1)
+        if getattr(self, "_recbuf", None) is None or len(self._recbuf) < rec_cap:
+            self._recbuf = (Record * rec_cap)()
+        return self._recbuf
+
     def run(self, nsteps, noise=None, rec_cap=100000):
-        recs = (Record * max(rec_cap, 1))()
+        recs = self._records(rec_cap)
         nr = C.c_long()
         nz = None if noise is None else np.ascontiguousarray(noise, dtype=np.float64)
         self._chk(self.L.sopg_run(self.h, nsteps, _ptr(nz), recs, rec_cap, C.byref(nr)))
@@ -467,7 +475,7 @@ class Context:
 
     def run_noalloc(self, nsteps, rec_cap=100000):
         """sopg_run_noalloc: the same steps with no device allocation (device noise only)."""
-        recs = (Record * max(rec_cap, 1))()
+        recs = self._records(rec_cap)
         nr = C.c_long()
         self._chk(self.L.sopg_run_noalloc(self.h, nsteps, recs, rec_cap, C.byref(nr)))
         return [{f: getattr(recs[k], f) for f, _ in Record._fields_} for k in range(min(nr.value, rec_cap))]
