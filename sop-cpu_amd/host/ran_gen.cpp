@@ -3,6 +3,8 @@
 // the reference's state-file layout (random_generator.cpp:24-70).
 #include "ran_gen.h"
 
+extern int sopg_restart_lossless;  // sop_shim.cpp
+
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -76,17 +78,15 @@ double Ran_Gen::gasdev() {
   return v2 * fac;
 }
 
-void Ran_Gen::gasdev_n(double* out, long n) {
-  for (long i = 0; i < n; i++) out[i] = gasdev();
-}
-
 // State file: mseed, iset, gset, idum2, iy, iv[0..31], one value per line.
-// gset is written with %.17g (the reference writes %f, which loses the cached
-// deviate's low digits; reading accepts either).
+// gset is written with %f as the reference writes it (which loses the cached
+// deviate's low digits), or with %.17g under `set restart_format lossless;`;
+// reading accepts either.
 void Ran_Gen::save_state() {
   FILE* f = std::fopen(fname, "w");
   if (!f) return;
-  std::fprintf(f, "%d\n%d\n%.17g\n%d\n%d\n", mseed, iset, gset, idum2, iy);
+  std::fprintf(f, sopg_restart_lossless ? "%d\n%d\n%.17g\n%d\n%d\n" : "%d\n%d\n%f\n%d\n%d\n", mseed, iset, gset,
+               idum2, iy);
   for (int i = 0; i < kTab; i++) std::fprintf(f, "%d\n", iv[i]);
   std::fclose(f);
 }

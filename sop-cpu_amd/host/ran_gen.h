@@ -18,8 +18,6 @@ class Ran_Gen {
   void set_seed(int);
   double gasdev();
   double ran2();
-  // Fill out[0..n) with consecutive gasdev() deviates.
-  void gasdev_n(double* out, long n);
 
  private:
   char fname[2048];

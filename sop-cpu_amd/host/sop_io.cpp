@@ -86,10 +86,13 @@ void print_sim_params() {
   std::cout << std::endl;
 }
 
+// "idx x y z" at %f (io.cpp:156-178), or at %.17g under `set restart_format lossless;`
+// so that a restart reads back the exact state.
 static void write_idx_xyz(const char* fname, const coord* a) {
   FILE* f = std::fopen(fname, "w");
   if (!f) return;
-  for (int i = 1; i <= nbead; i++) std::fprintf(f, "%d %f %f %f\n", i, a[i].x, a[i].y, a[i].z);
+  const char* fmt = sopg_restart_lossless ? "%d %.17g %.17g %.17g\n" : "%d %f %f %f\n";
+  for (int i = 1; i <= nbead; i++) std::fprintf(f, fmt, i, a[i].x, a[i].y, a[i].z);
   std::fclose(f);
 }
 

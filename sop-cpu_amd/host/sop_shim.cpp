@@ -13,6 +13,7 @@
 // The random force consumes `generator` exactly as random_force does (3N
 // gasdev() per force evaluation, bead-major x,y,z), so trajectories reproduce
 // the reference's; `set noise device;` switches to device Philox noise.
+#include <cstddef>
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -20,6 +21,10 @@
 #include "../../include/sopg.h"
 #include "sop_shim.h"
 #include "sop_state.h"
+
+int sopg_noise_device = 0;
+int sopg_lanes = 0;
+int sopg_restart_lossless = 0;
 
 namespace {
 
@@ -140,7 +145,7 @@ void sopg_shim_get_state() {
 const double* sopg_shim_noise(long nforce_evals) {
   if (sopg_noise_device) return nullptr;
   g_noise.resize((size_t)nforce_evals * 3 * nbead);
-  generator.gasdev_n(g_noise.data(), (long)g_noise.size());
+  for (double& g : g_noise) g = generator.gasdev();  // random_force's order (energy.cpp:456-462)
   return g_noise.data();
 }
 

@@ -77,8 +77,6 @@ double rna_etot, system_etot;
 double Q, rgsq, kinT;
 int contct_nat;
 
-int sopg_noise_device = 0;
-int sopg_lanes = 0;
 
 double rnd(double x) { return (x > 0) ? std::floor(x + 0.5) : std::ceil(x - 0.5); }
 
@@ -264,6 +262,8 @@ void set_params(int icmd) {
     sopg_noise_device = !std::strcmp(val, "device") ? 1 : 0;
   } else if (!std::strcmp(key, "lanes")) {  // driver extension
     sopg_lanes = std::atoi(val);
+  } else if (!std::strcmp(key, "restart_format")) {  // driver extension
+    sopg_restart_lossless = !std::strcmp(val, "lossless") ? 1 : 0;
   }
 }
 

@@ -11,6 +11,19 @@
 
 #include <cstddef>
 
+#ifdef SOPG_REFERENCE_GLOBALS
+// Built into SOP-CPU's own program (the shim and the step loops linked with
+// the reference's objects): the reference's headers declare the state and the
+// entry points, and its global.cpp / params.cpp / io.cpp define them.
+#include "global.h"
+#include "energy.h"
+#include "neighbor_list.h"
+#include "cell_list.h"
+#include "sop.h"
+#include "params.h"
+#include "io.h"
+#else
+
 #include "ran_gen.h"
 
 class coord {
@@ -92,12 +105,6 @@ extern force_term_Ptr force_term[];
 extern double Q, rgsq, kinT;
 extern int contct_nat;
 
-// driver extensions (ignored by the reference, which skips unknown `set` keys):
-//   set noise device;   Philox noise on the GPU instead of replaying `generator`
-//   set lanes L;        lanes per bead for the step kernels
-extern int sopg_noise_device;
-extern int sopg_lanes;
-
 double rnd(double);
 
 // energy.h
@@ -143,5 +150,18 @@ void save_unccoords(char*);
 void save_vels(char*);
 void load_vels(char*);
 void print_sim_params();
+
+#endif  // SOPG_REFERENCE_GLOBALS
+
+// Driver extensions (defined in sop_shim.cpp; set only by this driver's
+// set_params -- the reference skips unknown `set` keys, so in its program
+// they keep these defaults):
+//   set noise device;             Philox noise on the GPU instead of replaying `generator`
+//   set lanes L;                  lanes per bead for the step kernels
+//   set restart_format lossless;  coordinate, velocity and generator-state files at %.17g
+//                                 (the reference's %f loses digits; reading takes either)
+extern int sopg_noise_device;
+extern int sopg_lanes;
+extern int sopg_restart_lossless;
 
 #endif

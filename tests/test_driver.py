@@ -129,7 +129,110 @@ def test_driver_run_matches_reference_outputs(tmp_path, name):
     for f in ("traj.bin", "traj_uncorrected.bin", "coord.out", "unccoord.out", "veloc.out"):
         assert open(os.path.join(dr, f)).read() == open(os.path.join(dd, f)).read(), f
     _compare_text_numbers(os.path.join(dr, "update.out"), os.path.join(dd, "update.out"), 1e-10)
-    a = np.loadtxt(os.path.join(dr, "ran_gen.dat"))
-    b = np.loadtxt(os.path.join(dd, "ran_gen.dat"))
-    assert np.array_equal(np.delete(a, 2), np.delete(b, 2))
-    assert abs(a[2] - b[2]) <= 5e-7  # the reference writes the cached deviate with %f
+    assert open(os.path.join(dr, "ran_gen.dat")).read() == open(os.path.join(dd, "ran_gen.dat")).read()
+
+
+RESTART = ("set restart on;\nset rgen_restart on;\nset istep_restart 200;\n"
+           + BASE.format(nstep=400).replace("run;", "").rstrip("\n") + "\nrun;\n")
+OUTPUTS = ("traj.bin", "traj_uncorrected.bin", "coord.out", "unccoord.out", "veloc.out", "ran_gen.dat")
+
+
+def _run_driver(d, exe=DRIVER):
+    return subprocess.run([exe, "input.txt"], cwd=d, capture_output=True, text=True, check=True, timeout=300)
+
+
+def _same_outputs(dr, dd):
+    for f in OUTPUTS:
+        assert open(os.path.join(dr, f)).read() == open(os.path.join(dd, f)).read(), f
+    _compare_text_numbers(os.path.join(dr, "update.out"), os.path.join(dd, "update.out"), 1e-10)
+
+
+@needs_ref
+@pytest.mark.gpu
+def test_driver_file_restart_matches_reference(tmp_path):
+    """A run to step 200, then a restart from its files (`set restart on; set rgen_restart on;
+    set istep_restart 200`: coordinates, velocities and the generator state read back,
+    sop.cpp:126-140, io.cpp:180-242, random_generator.cpp:51-70) to step 400: every output
+    file equals the reference program's, including the appended update.out and trajectories."""
+    dr = _case_dir(tmp_path / "ref", "standard")
+    dd = _case_dir(tmp_path / "gpu", "standard")
+    for d in (dr, dd):
+        with open(os.path.join(d, "input.txt"), "w") as f:
+            f.write(BASE.format(nstep=200))
+    _run_reference(dr)
+    print("[gpu] driver run to step 200", flush=True)
+    _run_driver(dd)
+    _same_outputs(dr, dd)
+    for d in (dr, dd):
+        with open(os.path.join(d, "input.txt"), "w") as f:
+            f.write(RESTART)
+    _run_reference(dr)
+    print("[gpu] driver restart to step 400", flush=True)
+    _run_driver(dd)
+    _same_outputs(dr, dd)
+    assert len(_read_lines(os.path.join(dd, "update.out"))) == 4
+
+
+def _reference_final_state(d):
+    """Run the reference program on d/input.txt and return its final pos, unc_pos, vel at full
+    precision (read from its globals, not from its %f files)."""
+    code = ("import sys, numpy as np; sys.path.insert(0, %r); from refs import Ref; r = Ref();"
+            "r.L.ref_read_input(b'input.txt'); r.L.ref_ex_cmds();"
+            "np.save('state.npy', np.stack([r.coords(0), r.coords(1), r.coords(2)]))" % os.path.join(ROOT, "tests"))
+    subprocess.run([sys.executable, "-c", code], cwd=d, capture_output=True, text=True, check=True)
+    return np.load(os.path.join(d, "state.npy"))
+
+
+def _idx_xyz(path):
+    return np.array([[float(t) for t in l.split()[1:4]] for l in open(path) if l.strip()]).ravel()
+
+
+@needs_ref
+@pytest.mark.gpu
+def test_driver_lossless_restart_files(tmp_path):
+    """`set restart_format lossless;` (a key the reference ignores) writes coordinates,
+    velocities and the generator state at %.17g: the files hold the reference's final state
+    bit for bit, and a restart from them -- by the reference and by the driver -- gives the
+    same files again."""
+    lossless = "set restart_format lossless;\n"
+    dr = _case_dir(tmp_path / "ref", "standard")
+    dd = _case_dir(tmp_path / "gpu", "standard")
+    for d in (dr, dd):
+        with open(os.path.join(d, "input.txt"), "w") as f:
+            f.write(lossless + BASE.format(nstep=200))
+    ref_state = _reference_final_state(dr)
+    print("[gpu] driver run to step 200, lossless files", flush=True)
+    _run_driver(dd)
+    for k, f in enumerate(("coord.out", "unccoord.out", "veloc.out")):
+        assert np.array_equal(_idx_xyz(os.path.join(dd, f)), ref_state[k]), f
+    a = open(os.path.join(dr, "ran_gen.dat")).read().split()
+    b = open(os.path.join(dd, "ran_gen.dat")).read().split()
+    assert a[:2] + a[3:] == b[:2] + b[3:] and abs(float(a[2]) - float(b[2])) <= 5e-7
+    # both programs restart from the driver's lossless files
+    for f in OUTPUTS + ("update.out",):
+        shutil.copy(os.path.join(dd, f), os.path.join(dr, f))
+    for d in (dr, dd):
+        with open(os.path.join(d, "input.txt"), "w") as f:
+            f.write(RESTART)  # %f output again, as the reference writes
+    _run_reference(dr)
+    print("[gpu] driver restart from lossless files", flush=True)
+    _run_driver(dd)
+    _same_outputs(dr, dd)
+
+
+ROUTE2 = os.path.join(ROOT, "sop-cpu_amd", "sop_ref_gpu.x")
+
+
+@needs_ref
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(ROUTE2), reason="sop_ref_gpu.x not built (needs the reference's sources)")
+@pytest.mark.parametrize("name", ["standard", "overdamped"])
+def test_reference_program_with_gpu_shim_matches_reference(tmp_path, name):
+    """SOP-CPU's own program with the shim linked in place of its hot path (sop_ref_gpu.x,
+    built by `make -C sop-cpu_amd route2`) writes the reference program's files."""
+    dr = _case_dir(tmp_path / "ref", name)
+    dd = _case_dir(tmp_path / "gpu", name)
+    _run_reference(dr)
+    print(f"[gpu] reference program with the GPU shim: {name}", flush=True)
+    _run_driver(dd, ROUTE2)
+    _same_outputs(dr, dd)
