@@ -58,10 +58,10 @@ struct ChainP {
 };
 
 // Per-chain grid of the binned rebuild, recomputed at every rebuild.  A chain
-// uses it only when its extent keeps every pair in the exact min-image fast
-// path (|d| <= boxl/4, or a cell-index difference <= ncell/4 for the cell
-// list): then every pair the criterion keeps lies in adjacent grid cells.
-constexpr int kGridDim = 24;                       // cells per axis, at most
+// uses it only when its extent on every axis is at most the period less the
+// grid edge (ncell - 2 cells for the cell list): then every pair the criterion
+// keeps lies in adjacent grid cells.
+constexpr int kGridDim = 32;                       // cells per axis, at most
 constexpr int kGridCells = kGridDim * kGridDim * kGridDim;
 // What a list rebuild does for the chain (GridP::work, set by k_grid for every
 // chain of the launch, 0 for a chain not due).

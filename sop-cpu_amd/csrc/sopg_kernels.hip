@@ -249,10 +249,14 @@ __device__ __forceinline__ void rebuild_scan_bead(const Dev& D, const Tables* __
   const double* uy = cur ? D.uy[1] : D.uy[0];
   const double* uz = cur ? D.uz[1] : D.uz[0];
   const double boxl = P.boxl, qbox = P.qbox;
+  // candidate-list build (a neighbor-list chain too wide for the grid): the
+  // pairs within 3.2 sigma + skin, exactly, no pair-list bits; the natives are
+  // left to the candidate filter, which runs next
+  const bool cmode = !RES && (D.grid[c].work & W_CAND);
 
   // native contacts (neighbor_list.cpp:17-67 / cell_list.cpp:16-57)
   const int a0 = D.att_beg[g], a1 = D.att_end[g];
-  for (int q = a0 + l; q < a1; q += L) {
+  for (int q = a0 + l; q < a1 && !cmode; q += L) {
     const int code = D.att_code[q], k = code >> 1, side = code & 1, o = D.att_partner[q];
     const int i = side ? o : g, j = side ? g : o;
     const double dx = mimg(ux[j] - ux[i], boxl, qbox);
@@ -268,7 +272,7 @@ __device__ __forceinline__ void rebuild_scan_bead(const Dev& D, const Tables* __
 
   // non-native contacts: stable (ballot + prefix) compaction into the slots
   const int b = g - P.off;
-  uint32_t* S = D.slot + (size_t)g * D.cap;
+  uint32_t* S = (cmode ? D.cand : D.slot) + (size_t)g * D.cap;
   int* SK = D.slot_k ? D.slot_k + (size_t)g * D.cap : nullptr;
   int cnt = 0;
   if (!P.rep_explicit) {
@@ -291,7 +295,8 @@ __device__ __forceinline__ void rebuild_scan_bead(const Dev& D, const Tables* __
           const double dz = mimg(uz[j] - uz[i], boxl, qbox);
           const double d2 = dx * dx + dy * dy + dz * dz;
           bool pl;
-          if (crit == SOPG_CUTOFF_NEIGHBOR) { keep = d2 < T->nlrc2[it][jt]; pl = keep && d2 < T->plrc2[it][jt]; }
+          if (cmode) { keep = d2 < T->candc2[it][jt]; pl = false; }
+          else if (crit == SOPG_CUTOFF_NEIGHBOR) { keep = d2 < T->nlrc2[it][jt]; pl = keep && d2 < T->plrc2[it][jt]; }
           else if (crit == SOPG_CUTOFF_CELL) { keep = cell_adjacent(D, P, i, j); pl = keep && d2 < T->plrc2[it][jt]; }
           else { keep = true; pl = true; }
           w = slot_make((uint32_t)c, (uint32_t)side, (uint32_t)it, (uint32_t)jt) | (pl ? (1u << (30 + par)) : 0u);
@@ -336,8 +341,11 @@ __device__ __forceinline__ void rebuild_scan_bead(const Dev& D, const Tables* __
     }
   }
   if (l == 0) {
-    D.cnt[g] = cnt < D.cap ? cnt : D.cap;
-    if (cnt > D.cap) atomicOr(D.err, 1);
+    (cmode ? D.ccnt : D.cnt)[g] = cnt < D.cap ? cnt : D.cap;
+    if (cnt > D.cap) {
+      atomicOr(D.err, 1);
+      if (cmode) D.grid[c].cand_valid = 0;  // a truncated candidate list must not serve later rebuilds
+    }
   }
 }
 
@@ -466,10 +474,16 @@ __global__ void __launch_bounds__(kGridThreads) k_grid(Dev D, const Tables* __re
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    // every pair difference stays below half the period: the min image is the
-    // plain difference, so all kept pairs are in adjacent cells of a local grid
-    const double limit = 0.49 * (cell ? P.ncell : P.boxl);
-    const double inv = cell ? 1.0 : 1.0 / (cand ? P.gcand : P.gcell);
+    // A chain whose extent on every axis is at most the period less the grid
+    // edge has no pair that the criterion keeps across the boundary: a pair
+    // whose plain difference exceeds half the period has the min image
+    // period - |difference| >= the grid edge > the cutoff (the cell list: >= 2
+    // cells).  So every kept pair lies in adjacent cells of a local grid over
+    // the unwrapped coordinates; the distances themselves keep the reference's
+    // min image.
+    const double gedge = cell ? 1.0 : (cand ? P.gcand : P.gcell);
+    const double limit = cell ? P.ncell - 2.0 : P.boxl - gedge;
+    const double inv = 1.0 / gedge;
     int dims[3];
     bool use = true;
     for (int a = 0; a < 3; a++) {
@@ -481,11 +495,12 @@ __global__ void __launch_bounds__(kGridThreads) k_grid(Dev D, const Tables* __re
     G.ox = sh[0][0]; G.oy = sh[1][0]; G.oz = sh[2][0];
     G.inv = inv;
     G.nx = dims[0]; G.ny = dims[1]; G.nz = dims[2];
-    G.work = !use ? W_SCAN : (cand ? (W_BIN | W_CAND | W_FILTER) : W_BIN);
+    // a chain too wide for the grid builds its candidate list by the full scan
+    G.work = (use ? W_BIN : W_SCAN) | (cand ? (W_CAND | W_FILTER) : 0);
     if (G.work & W_BIN) D.plan[0] = rel;
     if (G.work & W_FILTER) D.plan[1] = rel;
     if (G.work & W_SCAN) D.plan[2] = rel;
-    if (cand) G.cand_valid = use ? 1 : 0;
+    if (cand) G.cand_valid = 1;
     gf[0] = G.ox; gf[1] = G.oy; gf[2] = G.oz; gf[3] = inv;
     gi[0] = dims[0]; gi[1] = dims[1]; gi[2] = dims[2]; gi[3] = G.work;
   }
@@ -1582,13 +1597,15 @@ void launch_rebuild(int L, const Dev& D, const Tables* T, int cur, int par, int 
     hipLaunchKernelGGL(k_rebuild_grid<kGridLanes>, dim3((unsigned)((g1 - g0 + kGridBeads - 1) / kGridBeads)),
                        dim3(256), (size_t)kGridBeads * 2 * words * sizeof(uint32_t), s, Dg, T, cur, par, rel, mode,
                        g0, g1);
-    if (cand)
-      hipLaunchKernelGGL(k_cand_filter<kFilterLanes>, dim3(blocks_for((long)(g1 - g0) * kFilterLanes)), dim3(256), 0, s,
-                         D, T, cur, par, rel, g0, g1);
   }
+  // the full scan (lists, or the candidate lists of chains too wide for the
+  // grid), then the filter of every chain planned to filter its candidates
 #define C_(X) launch_rebuild_L<X>(D, T, cur, par, rel, mode, g0, g1, s)
   SOPG_DISPATCH_L(L, C_)
 #undef C_
+  if (binned && cand)
+    hipLaunchKernelGGL(k_cand_filter<kFilterLanes>, dim3(blocks_for((long)(g1 - g0) * kFilterLanes)), dim3(256), 0, s,
+                       D, T, cur, par, rel, g0, g1);
 }
 void launch_pairfilter(int L, const Dev& D, const Tables* T, int cur, int par, int g0, int g1, hipStream_t s) {
 #define C_(X) launch_pairfilter_L<X>(D, T, cur, par, g0, g1, s)

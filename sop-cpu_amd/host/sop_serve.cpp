@@ -97,7 +97,7 @@ int ctx_fail(sopg_server* s, int rc, const char* what) {
 // chain slot and its binning grid.
 double footprint(int cap, const sopg_topology* t) {
   const double per_bead = 12 * 8 + 15 * 8 + 12 * 4 + 1 + 8.0 * cap + 12 * 8;
-  const double per_chain = 512 + 3.0 * 24 * 24 * 24 * 4 + 12 * 8;
+  const double per_chain = 512 + 3.0 * 32 * 32 * 32 * 4 + 12 * 8;
   const double nrep = t->ncon_rep > 0 ? t->ncon_rep : 0;
   const double per_native = 4 + 4 + 4 + 1 + 8 + 8 + 4 + 2 * (4 + 4 + 1 + 1 + 8 + 8 + 8);
   return per_bead * t->nbead + 24.0 * t->nbnd + 8.0 * t->nang + per_native * t->ncon_att + 17.0 * nrep + per_chain;

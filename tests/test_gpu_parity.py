@@ -305,6 +305,10 @@ def test_binned_rebuild_equals_full_scan(inputs_root, cutoff):
     for n, seed in ((80, 41), (400, 42), (1500, 43)):
         x, v, _ = chain_inputs(inputs_root, n, seed)
         chains.append((x, v, 77 + seed))
+    # self-avoiding walks wider than half the box (278 and 450 A in a 504 A box): the
+    # grid applies up to the box length less the grid edge
+    for n, seed in ((3000, 8), (4000, 9)):
+        chains.append((sopg.gen_saw(n, seed), sopg.gen_maxwell(n, 0.6, seed), 77 + seed))
     res = []
     for binned in (True, False):
         ctx = sopg.Context(binned=binned)
