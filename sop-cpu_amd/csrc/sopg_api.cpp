@@ -263,7 +263,8 @@ int grow_pool(sopg_ctx* c, int pi, size_t need) {
   size_t ncap = std::max(need, P.cap * 2 + 16);
   if (pi == P_BEAD) {
     if (need > kMaxPoolBeads) return fail(c, SOPG_ERR_ARG, "bead pool limit (2^28 beads per context) exceeded");
-    ncap = std::min(ncap, kMaxPoolBeads);
+    // whole slot tiles, so that a grown array keeps the old tiles in place
+    ncap = std::min((ncap + kSlotTile - 1) / kSlotTile * kSlotTile, kMaxPoolBeads);
   }
   HIP_OK(c, hipStreamSynchronize(c->stream));
   for (Arr& a : c->arrs[pi]) {
@@ -279,6 +280,38 @@ int grow_pool(sopg_ctx* c, int pi, size_t need) {
   }
   P.extend(ncap);
   return SOPG_OK;
+}
+
+// Slot words of beads [off, off + n) as n rows of cap words.  The device keeps
+// them in tiles of kSlotTile beads (slot_index), so a transfer covers whole
+// tiles: [off, off + n) rounded out to tile bounds, inside the pool.
+int slot_rows_get(sopg_ctx* c, size_t off, size_t n, std::vector<uint32_t>& rows) {
+  const int cap = c->D.cap;
+  const size_t t0 = off / kSlotTile * kSlotTile, t1 = (off + n + kSlotTile - 1) / kSlotTile * kSlotTile;
+  std::vector<uint32_t> tiles((t1 - t0) * (size_t)cap);
+  HIP_OK(c, hipMemcpyAsync(tiles.data(), c->D.slot + t0 * (size_t)cap, sizeof(uint32_t) * tiles.size(),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(c, hipStreamSynchronize(c->stream));
+  rows.assign(n * (size_t)cap, 0u);
+  const size_t base = t0 * (size_t)cap;
+  for (size_t b = 0; b < n; b++)
+    for (int q = 0; q < cap; q++) rows[b * cap + q] = tiles[slot_index((int)(off + b), q, cap) - base];
+  return SOPG_OK;
+}
+
+// The inverse: write n rows of cap words as beads [off, off + n)'s slot words,
+// leaving the other beads of the boundary tiles as they are.
+int slot_rows_put(sopg_ctx* c, size_t off, size_t n, const std::vector<uint32_t>& rows) {
+  const int cap = c->D.cap;
+  const size_t t0 = off / kSlotTile * kSlotTile, t1 = (off + n + kSlotTile - 1) / kSlotTile * kSlotTile;
+  std::vector<uint32_t> tiles((t1 - t0) * (size_t)cap);
+  HIP_OK(c, hipMemcpyAsync(tiles.data(), c->D.slot + t0 * (size_t)cap, sizeof(uint32_t) * tiles.size(),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(c, hipStreamSynchronize(c->stream));
+  const size_t base = t0 * (size_t)cap;
+  for (size_t b = 0; b < n; b++)
+    for (int q = 0; q < cap; q++) tiles[slot_index((int)(off + b), q, cap) - base] = rows[b * cap + q];
+  return upload_at(c, c->D.slot, base, tiles);
 }
 
 int take_range(sopg_ctx* c, int pi, size_t n, size_t& at) {
@@ -627,7 +660,7 @@ int ensure_device(sopg_ctx* c) {
     cap = 1024;
     if (any_none) cap = std::max(cap, maxn);
   }
-  D.cap = std::max(cap, 1);
+  D.cap = (std::max(cap, 1) + 3) / 4 * 4;  // whole 4-entry chunks of the slot tiles
   D.bitmap_words = 1;
   register_arrays(c);
   int rc;
@@ -1249,8 +1282,8 @@ int sopg_get_list(sopg_ctx* c, int chain, int which, int* oi, int* oj, long cap,
     std::vector<uint32_t> sl((size_t)n * cap_s);
     std::vector<int> sk;
     HIP_OK(c, hipMemcpyAsync(cnt.data(), c->D.cnt + P->off, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
-    HIP_OK(c, hipMemcpyAsync(sl.data(), c->D.slot + (size_t)P->off * cap_s, sizeof(uint32_t) * n * cap_s,
-                             hipMemcpyDeviceToHost, c->stream));
+    rc = slot_rows_get(c, (size_t)P->off, (size_t)n, sl);
+    if (rc) return rc;
     if (h->rep_explicit) {
       sk.resize((size_t)n * cap_s);
       HIP_OK(c, hipMemcpyAsync(sk.data(), c->D.slot_k + (size_t)P->off * cap_s, sizeof(int) * n * cap_s,
@@ -1327,8 +1360,8 @@ int ckpt_lists(sopg_ctx* c, int chain, std::vector<uint8_t>& nl, std::vector<int
   cnt.resize(n);
   if (nq) HIP_OK(c, hipMemcpyAsync(f.data(), c->D.att_flag + tq, nq, hipMemcpyDeviceToHost, c->stream));
   HIP_OK(c, hipMemcpyAsync(cnt.data(), c->D.cnt + P.off, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
-  HIP_OK(c, hipMemcpyAsync(sl.data(), c->D.slot + (size_t)P.off * cap, sizeof(uint32_t) * n * cap,
-                           hipMemcpyDeviceToHost, c->stream));
+  int rc = slot_rows_get(c, (size_t)P.off, (size_t)n, sl);
+  if (rc) return rc;
   if (h.rep_explicit) {
     sk.resize((size_t)n * cap);
     HIP_OK(c, hipMemcpyAsync(sk.data(), c->D.slot_k + (size_t)P.off * cap, sizeof(int) * n * cap,
@@ -1432,7 +1465,7 @@ int sopg_restore(sopg_ctx* c, int chain, const void* buf, long size) {
     f[(size_t)h->q1[a] - tq] = nl[a] ? 1 : 0;
   }
   if ((rc = upload_at(c, c->D.att_flag, tq, f)) || (rc = upload_at(c, c->D.cnt, (size_t)P->off, cnt)) ||
-      (rc = upload_at(c, c->D.slot, (size_t)P->off * cap, sl)))
+      (rc = slot_rows_put(c, (size_t)P->off, n, sl)))
     return rc;
   if (h->rep_explicit && (rc = upload_at(c, c->D.slot_k, (size_t)P->off * cap, sk))) return rc;
   h->istep = hd.istep;

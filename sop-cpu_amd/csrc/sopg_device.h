@@ -163,5 +163,14 @@ constexpr size_t kMaxPoolBeads = (size_t)1 << 28;
 SOPG_HD uint32_t slot_make(uint32_t partner, uint32_t side, uint32_t it, uint32_t jt) {
   return partner | (side << 25) | (it << 26) | (jt << 28);
 }
+// Layout of the slot words in HBM: tiles of kSlotTile consecutive beads x 4
+// consecutive entries (cap words per bead, cap a multiple of 4; the bead pool
+// a multiple of kSlotTile beads).  A wave that gives 16 consecutive beads 4
+// lanes each reads one chunk of every bead's list as one contiguous 256-byte
+// run, instead of 16 runs cap words apart.
+constexpr int kSlotTile = 16;
+SOPG_HD size_t slot_index(int g, int q, int cap) {
+  return ((size_t)(g >> 4) * (size_t)cap + (size_t)(q & ~3)) * 16 + (size_t)(g & 15) * 4 + (size_t)(q & 3);
+}
 
 }  // namespace sopg

@@ -168,6 +168,18 @@ __device__ __forceinline__ double buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t byte
   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, 0, 0));
 }
 
+// Bead g's slot words (the neighbor list: tiled, slot_index) or candidate
+// words (row-major, cap words per bead).
+struct ListRow {
+  uint32_t* base;
+  int g;
+  int cap;
+  bool tiled;
+  __device__ __forceinline__ uint32_t& operator[](int q) const {
+    return tiled ? base[slot_index(g, q, cap)] : base[(size_t)g * (size_t)cap + (size_t)q];
+  }
+};
+
 // A lane's force term cs * d, or -0.0 (which adds nothing in the folds) when
 // the lane has none: the product's sign bit is set on such lanes, whose cs is
 // 0 and whose product is therefore +0 or -0.
@@ -272,7 +284,7 @@ __device__ __forceinline__ void rebuild_scan_bead(const Dev& D, const Tables* __
 
   // non-native contacts: stable (ballot + prefix) compaction into the slots
   const int b = g - P.off;
-  uint32_t* S = (cmode ? D.cand : D.slot) + (size_t)g * D.cap;
+  const ListRow S{cmode ? D.cand : D.slot, g, D.cap, !cmode};
   int* SK = D.slot_k ? D.slot_k + (size_t)g * D.cap : nullptr;
   int cnt = 0;
   if (!P.rep_explicit) {
@@ -725,7 +737,7 @@ __global__ void __launch_bounds__(256) k_rebuild_grid(Dev D, const Tables* __res
   const ChainP& P = *Pp;
   const int off = P.off;
   const uint32_t gt = (uint32_t)D.type[g];
-  uint32_t* S = (cmode ? D.cand : D.slot) + (size_t)g * D.cap;
+  const ListRow S{cmode ? D.cand : D.slot, g, D.cap, !cmode};
   const uint32_t pbit = cmode ? 0u : 1u << (30 + par);
   int cnt = 0;
   for (int base = 0; base < words; base += LG) {
@@ -844,7 +856,7 @@ __global__ void __launch_bounds__(256) k_cand_filter(Dev D, const Tables* __rest
   // natives so that their loads overlap.
   const double cx = ux[g], cy = uy[g], cz = uz[g];
   const uint32_t* C = D.cand + (size_t)g * D.cap;
-  uint32_t* S = D.slot + (size_t)g * D.cap;
+  const ListRow S{D.slot, g, D.cap, true};
   const int n = D.ccnt[g];
   // The candidates' positions are gathered in single precision (half the
   // data of the double positions).  A single-precision coordinate is within
@@ -943,7 +955,7 @@ __device__ __forceinline__ void pairfilter_bead(const Dev& D, const Tables* __re
     D.att_flag[q] = (uint8_t)((f & ~pbit) | (pl ? pbit : 0));
   }
   const int cnt = D.cnt[g];
-  uint32_t* S = D.slot + (size_t)g * D.cap;
+  const ListRow S{D.slot, g, D.cap, true};
   const uint32_t sbit = 1u << (30 + par);
   for (int q = l; q < cnt; q += L) {
     const uint32_t w = S[q];
@@ -1223,7 +1235,7 @@ __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restric
   // non-native repulsion over the pair list (energy.cpp:313-350): F_i += f, F_j -= f
   if (flags & F_VDW) {
     const int cnt = nslot;
-    uint32_t* S = D.slot + (size_t)g * D.cap;
+    const ListRow S{D.slot, g, D.cap, true};
     const uint32_t rbit = 1u << (30 + pr), wbit = 1u << (30 + pw);
     const double thr_rep = T->thr_rep;
     // software pipeline: the partner coordinates of the next chunk and the
@@ -1370,7 +1382,7 @@ __device__ __forceinline__ void observe_bead(const Dev& D, int cur, int par, int
     if (d / (double)D.am_rf[k] < 1.25) cont++;
   }
   const int cnt = D.cnt[g];
-  const uint32_t* S = D.slot + (size_t)g * D.cap;
+  const ListRow S{D.slot, g, D.cap, true};
   const uint32_t pbit = 1u << (30 + par);
   for (int q = 0; q < cnt; q++) {
     const uint32_t w = S[q];
