@@ -1573,11 +1573,14 @@ int sopg_autotune(sopg_ctx* c, int* step_lanes) {
   HIP_OK(c, hipEventCreate(&e1));
   double best = 1e300;
   int bestL = c->lanes;
+  // The run's step kernel with its next-step pair-list test.  Without
+  // F_STORE nothing of the state is written but the pair-list bits of the
+  // other parity, which the next step of a run rewrites before any reads.
   for (int L : {1, 2, 4, 8, 16, 32, 64}) {
-    launch_force(L, c->D, c->d_T, c->cur, c->par, 0, F_ALLTERMS, false, nullptr, 0, nb, c->stream);
+    launch_force(L, c->D, c->d_T, c->cur, c->par, 0, F_ALLTERMS, true, nullptr, 0, nb, c->stream);
     (void)hipEventRecord(e0, c->stream);
     for (int k = 0; k < 5; k++)
-      launch_force(L, c->D, c->d_T, c->cur, c->par, 0, F_ALLTERMS, false, nullptr, 0, nb, c->stream);
+      launch_force(L, c->D, c->d_T, c->cur, c->par, 0, F_ALLTERMS, true, nullptr, 0, nb, c->stream);
     (void)hipEventRecord(e1, c->stream);
     HIP_OK(c, hipEventSynchronize(e1));
     float ms = 0.0f;
