@@ -39,7 +39,7 @@ int scratch_address(int device, double** out);
 void launch_resident(int L, const Dev& D, const Tables* T, const int* chains, int nchains, int threads, int cur,
                      int par, int nsteps, const int* rec_slot, double* rec, int nc, hipStream_t s);
 enum { F_STORE = 1, F_VEL = 2, F_ENERGY = 4, F_ENERGY_ALL = 8, F_NOISE = 16, F_BOND = 32, F_ANGLE = 64,
-       F_VDW = 128, F_ACCUM = 256, F_ALLTERMS = 16 | 32 | 64 | 128 };
+       F_VDW = 128, F_ACCUM = 256, F_ADVANCE = 512, F_ALLTERMS = 16 | 32 | 64 | 128 };
 enum { OBS_PHYS = 1, OBS_COUNTS = 2 };
 }  // namespace sopg
 
@@ -160,6 +160,11 @@ struct sopg_ctx {
   std::vector<GraphEntry> gcache;
   long gclock = 0;
   long graph_launches = 0, graph_captures = 0;
+  // Every call that can change positions, lists, parameters or the chain set
+  // advances gen.  A run leaves the next step's pair list (at the positions it
+  // stops at) in the other parity; the next run starts from it without a
+  // pair-list filter when gen has not moved since (next_gen == gen).
+  long gen = 0, next_gen = -1;
   // resident (persistent) kernel for small chains
   int resident_max = 0;         // chains of at most this many beads (0: off)
   int resident_lanes = 0;       // lanes per bead in it (0: automatic)
@@ -741,6 +746,7 @@ int list_counts(sopg_ctx* c, int chain, double* out) {
 }
 
 int rebuild_one(sopg_ctx* c, int chain, int mode, int* nnl_att, int* nnl_rep) {
+  if (c) c->gen++;
   HostChain* h;
   ChainP* P;
   int rc = get_chain(c, chain, &h, &P);
@@ -959,6 +965,7 @@ int sopg_default_tables(sopg_tables* t) {
 }
 
 int sopg_set_tables(sopg_ctx* c, const sopg_tables* t) {
+  if (c) c->gen++;
   if (!c || !t) return SOPG_ERR_ARG;
   c->tab = *t;
   derive_tables(c->tab, c->skin, c->T);
@@ -974,6 +981,7 @@ int sopg_set_tables(sopg_ctx* c, const sopg_tables* t) {
 
 int sopg_set_lanes(sopg_ctx* c, int step_lanes, int list_lanes) {
   if (!c) return SOPG_ERR_ARG;
+  c->gen++;
   auto ok = [](int l) { return l >= 1 && l <= 64 && !(l & (l - 1)); };
   if (step_lanes == 0) step_lanes = 8;
   if (list_lanes == 0) list_lanes = 64;
@@ -992,18 +1000,21 @@ int sopg_get_lanes(const sopg_ctx* c, int* step_lanes, int* list_lanes) {
 
 int sopg_set_binned(sopg_ctx* c, int enable) {
   if (!c) return SOPG_ERR_ARG;
+  c->gen++;
   c->binned = enable ? 1 : 0;
   return SOPG_OK;
 }
 
 int sopg_set_capacity(sopg_ctx* c, int cap) {
   if (!c) return SOPG_ERR_ARG;
+  c->gen++;
   if (c->device_ready) return fail(c, SOPG_ERR_ARG, "capacity must be set before the first device operation");
   c->cap_user = cap;
   return SOPG_OK;
 }
 
 int sopg_reserve(sopg_ctx* c, long beads, int chains) {
+  if (c) c->gen++;
   if (!c || beads < 0 || chains < 0) return SOPG_ERR_ARG;
   if (c->device_ready) {
     int rc;
@@ -1016,6 +1027,7 @@ int sopg_reserve(sopg_ctx* c, long beads, int chains) {
 }
 
 int sopg_add_chain(sopg_ctx* c, const sopg_topology* t, const sopg_chain_params* p, int* id) {
+  if (c) c->gen++;
   if (!c || !t) return SOPG_ERR_ARG;
   int rc = check_params(c, p);
   if (rc) return rc;
@@ -1041,6 +1053,7 @@ int sopg_add_chain(sopg_ctx* c, const sopg_topology* t, const sopg_chain_params*
 
 int sopg_remove_chain(sopg_ctx* c, int chain) {
   if (!c) return SOPG_ERR_ARG;
+  c->gen++;
   if (chain < 0 || chain >= (int)c->chains.size() || !c->chains[chain].active)
     return fail(c, SOPG_ERR_ARG, "bad chain id");
   HostChain& h = c->chains[chain];
@@ -1059,6 +1072,7 @@ int sopg_remove_chain(sopg_ctx* c, int chain) {
 
 int sopg_set_chain_params(sopg_ctx* c, int chain, const sopg_chain_params* p) {
   if (!c) return SOPG_ERR_ARG;
+  c->gen++;
   if (chain < 0 || chain >= (int)c->chains.size() || !c->chains[chain].active)
     return fail(c, SOPG_ERR_ARG, "bad chain id");
   int rc = check_params(c, p);
@@ -1082,6 +1096,7 @@ int sopg_chain_nbead(const sopg_ctx* c, int chain) {
 
 int sopg_put_state(sopg_ctx* c, int chain, const double* pos, const double* unc, const double* vel,
                    const double* force) {
+  if (c) c->gen++;
   HostChain* h;
   ChainP* P;
   int rc = get_chain(c, chain, &h, &P);
@@ -1139,6 +1154,7 @@ int sopg_update_cell_list(sopg_ctx* c, int chain, int* nnl_att, int* nnl_rep) {
 }
 
 int sopg_update_pair_list(sopg_ctx* c, int chain, int* nil_att, int* nil_rep) {
+  if (c) c->gen++;
   HostChain* h;
   ChainP* P;
   int rc = get_chain(c, chain, &h, &P);
@@ -1170,6 +1186,7 @@ int sopg_force_eval(sopg_ctx* c, int chain, const double* noise) {
 }
 
 int sopg_force_terms(sopg_ctx* c, int chain, int terms, const double* noise) {
+  if (c) c->gen++;
   HostChain* h;
   ChainP* P;
   int rc = get_chain(c, chain, &h, &P);
@@ -1217,6 +1234,7 @@ int sopg_energy_eval(sopg_ctx* c, int chain, sopg_energies* out) {
 }
 
 int sopg_underdamped_iteration(sopg_ctx* c, int chain, const double* noise) {
+  if (c) c->gen++;
   HostChain* h;
   ChainP* P;
   int rc = get_chain(c, chain, &h, &P);
@@ -1316,6 +1334,7 @@ int sopg_get_list(sopg_ctx* c, int chain, int which, int* oi, int* oj, long cap,
 
 int sopg_set_step(sopg_ctx* c, int chain, double istep, int inlup, int iup) {
   if (!c) return SOPG_ERR_ARG;
+  c->gen++;
   if (chain < 0 || chain >= (int)c->chains.size() || !c->chains[chain].active)
     return fail(c, SOPG_ERR_ARG, "bad chain id");
   HostChain& h = c->chains[chain];
@@ -1416,6 +1435,7 @@ int sopg_checkpoint(sopg_ctx* c, int chain, void* buf, long cap, long* size) {
 }
 
 int sopg_restore(sopg_ctx* c, int chain, const void* buf, long size) {
+  if (c) c->gen++;
   HostChain* h;
   ChainP* P;
   int rc = get_chain(c, chain, &h, &P);
@@ -1506,6 +1526,7 @@ int sopg_list_counts(sopg_ctx* c, int chain, int* nnl_att, int* nnl_rep, int* ni
 }
 
 int sopg_set_resident(sopg_ctx* c, int max_beads, int lanes) {
+  if (c) c->gen++;
   if (!c || max_beads < 0 || lanes < 0 || lanes > 8 || (lanes & (lanes - 1)))
     return c ? fail(c, SOPG_ERR_ARG, "resident: max_beads >= 0, lanes 0 or a power of two up to 8") : SOPG_ERR_ARG;
   if (max_beads * (lanes ? lanes : 1) > 512)
@@ -1517,6 +1538,7 @@ int sopg_set_resident(sopg_ctx* c, int max_beads, int lanes) {
 
 int sopg_set_skin(sopg_ctx* c, double skin) {
   if (!c) return SOPG_ERR_ARG;
+  c->gen++;
   if (!(skin >= 0.0) || skin > 100.0) return fail(c, SOPG_ERR_ARG, "skin must be in [0, 100] Angstrom");
   c->skin = skin;
   derive_tables(c->tab, c->skin, c->T);
@@ -1563,6 +1585,7 @@ int sopg_profile_read(sopg_ctx* c, int kernel, double* total_ms, long* launches)
 // depend on the lane count, so this only changes speed.
 int sopg_autotune(sopg_ctx* c, int* step_lanes) {
   if (!c) return SOPG_ERR_ARG;
+  c->gen++;
   int rc = ensure_device(c);
   if (rc) return rc;
   if ((rc = upload_chainp(c))) return rc;
@@ -1654,6 +1677,10 @@ static int run_impl(sopg_ctx* c, long nsteps, const double* noise, sopg_record* 
   // the per-step launches below serve the chains the resident kernel does not
   bool stepped = false;
   for (auto& h : c->chains) stepped |= h.active && !h.resident;
+  // the previous run's last step wrote the pair list at these positions into
+  // the other parity, and nothing has changed since: start from it
+  const bool next_pl = stepped && c->next_gen == c->gen;
+  if (next_pl) c->par ^= 1;
   auto due = [&](long r) {
     bool any = false, cell = false;
     for (auto& h : c->chains)
@@ -1745,8 +1772,9 @@ static int run_impl(sopg_ctx* c, long nsteps, const double* noise, sopg_record* 
   // pair lists at the current positions; rerunning the filter on a chain whose
   // list is fresh reproduces it, so one launch covers every chain
   bool fresh = true;
-  for (auto& h : c->chains)
-    if (h.active && !h.resident && !h.pl_fresh) fresh = false;
+  if (!next_pl)
+    for (auto& h : c->chains)
+      if (h.active && !h.resident && !h.pl_fresh) fresh = false;
   if (!fresh) {
     pe = prof_begin(c);
     launch_pairfilter(L, D, c->d_T, c->cur, c->par, 0, nb, s);  // a few dozen entries per bead: step lanes
@@ -1762,7 +1790,10 @@ static int run_impl(sopg_ctx* c, long nsteps, const double* noise, sopg_record* 
     int pe = prof_begin(c);
     // energy terms only in a step at which some chain records
     const int fe = records(r) ? F_ENERGY : 0;
-    launch_force(L, Dx, c->d_T, c->cur, c->par, k, F_STORE | F_VEL | fe | F_ALLTERMS, !last, nz, 0, nb, s);
+    // every step computes the next step's pair list; the last one leaves the
+    // positions at rest (its pair list is the one the next run starts with)
+    launch_force(L, Dx, c->d_T, c->cur, c->par, k, F_STORE | F_VEL | fe | F_ALLTERMS | (last ? 0 : F_ADVANCE), true,
+                 nz, 0, nb, s);
     prof_end(c, SOPG_KERNEL_STEP, pe);
     if (records(r)) {
       pe = prof_begin(c);
@@ -1857,6 +1888,7 @@ static int run_impl(sopg_ctx* c, long nsteps, const double* noise, sopg_record* 
     h.pl_fresh = false;
     h.resident = false;
   }
+  c->next_gen = stepped ? c->gen : -1;
   std::vector<double> hr(rec_steps.size() * nc * kRed);
   if (!hr.empty())
     HIP_OK(c, hipMemcpyAsync(hr.data(), fixed ? rec_buf : c->d_rec, sizeof(double) * hr.size(),

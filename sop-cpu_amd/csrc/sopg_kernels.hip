@@ -1009,13 +1009,14 @@ __global__ void k_posupdate(Dev D, int src, int dst, int g0, int g1) {
 //        F_VEL     velocity update (sop.cpp:320-328) when the chain's T allows
 //        F_ENERGY  energy partials (energy.cpp:115-257) on chains whose record
 //                  falls on this step (all chains if F_ENERGY_ALL)
-// NEXT:  also the next step's pair list (parity pr^1) from these positions
-//        and the next position update into buffer cur^1.
+// NEXT:  also the next step's pair list (parity pr^1) from these positions,
+//        and with F_ADVANCE the next position update into buffer cur^1
+//        (a run's last step computes the pair list its next run starts with)
 //        F_NOISE / F_BOND / F_ANGLE / F_VDW  which force terms to add (the
 //                  reference's force_term[] table, energy.cpp:78-113)
 //        F_ACCUM   add to the stored force instead of starting from zero
 enum { F_STORE = 1, F_VEL = 2, F_ENERGY = 4, F_ENERGY_ALL = 8, F_NOISE = 16, F_BOND = 32, F_ANGLE = 64,
-       F_VDW = 128, F_ACCUM = 256 };
+       F_VDW = 128, F_ACCUM = 256, F_ADVANCE = 512 };
 
 // Type-pair tables the pair loops read, staged in LDS (index it * 3 + jt).
 // Rows of 8 doubles (64 bytes): the non-native loop reads entries 0-3 of a
@@ -1311,7 +1312,8 @@ __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restric
   const double ix0 = (cur ? D.ix[1] : D.ix[0])[g], iy0 = (cur ? D.iy[1] : D.iy[0])[g];
   const double iz0 = (cur ? D.iz[1] : D.iz[0])[g];
   double psx = 0.0, psy = 0.0, psz = 0.0;
-  if constexpr (NEXT) {
+  const bool adv = NEXT && (flags & F_ADVANCE);
+  if (adv) {
     psx = (cur ? D.px[1] : D.px[0])[g];
     psy = (cur ? D.py[1] : D.py[0])[g];
     psz = (cur ? D.pz[1] : D.pz[0])[g];
@@ -1326,7 +1328,7 @@ __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restric
     vz = a3 * iz0 + a4 * Fz;
     D.vx[g] = vx; D.vy[g] = vy; D.vz[g] = vz;
   }
-  if constexpr (NEXT) {
+  if (adv) {
     // the next step's position half (sop.cpp:288-310) into buffer cur^1
     const double ix = od ? a5 * Fx : a1 * vx + a2 * Fx;
     const double iy = od ? a5 * Fy : a1 * vy + a2 * Fy;
@@ -1478,7 +1480,7 @@ __global__ void __launch_bounds__(512) k_resident(Dev D, const Tables* __restric
   const ChainP& P = D.cp[c];
   const int n = P.n, off = P.off;
   const int groups = (int)blockDim.x / L, grp = (int)threadIdx.x / L, l = (int)threadIdx.x & (L - 1);
-  const int flags = F_STORE | F_VEL | F_ENERGY | F_NOISE | F_BOND | F_ANGLE | F_VDW;
+  const int flags = F_STORE | F_VEL | F_ENERGY | F_NOISE | F_BOND | F_ANGLE | F_VDW | F_ADVANCE;
   auto rebuild = [&](int step) {
     if (P.cutoff == SOPG_CUTOFF_NONE || (P.phase + step) % P.nnlup != 0) return;
     if (P.cutoff == SOPG_CUTOFF_CELL) {

@@ -482,6 +482,68 @@ def test_graph_replay_equals_direct(inputs_root, cutoff):
     assert ra == rg
 
 
+@pytest.mark.parametrize("cutoff", [sopg.CUTOFF_NEIGHBOR, sopg.CUTOFF_CELL])
+def test_split_runs_equal_one_run(inputs_root, cutoff):
+    """A run leaves the pair list at the positions it stops at in the other
+    parity, and the next run starts from it.  Runs split at every phase of the
+    rebuild period (ends on and next to rebuild steps, single steps) give the
+    bits of one unbroken run -- also when calls between runs change the state
+    (a pair-list update, a state upload), after which the run filters again."""
+    chains = []
+    for n, seed in ((70, 111), (160, 112), (240, 113)):
+        x, v, _ = chain_inputs(inputs_root, n, seed)
+        chains.append((x, v, 60 + seed))
+
+    def make():
+        ctx = sopg.Context()
+        for (x, v, seed) in chains:
+            ctx.add_chain(sopg.chain_topology(x),
+                          sopg.chain_params(boxl=504.0, ncell=42.0, cutoff=cutoff, nnlup=20, nup=50,
+                                            noise_mode=sopg.NOISE_PHILOX, seed=seed))
+        for c, (x, v, seed) in enumerate(chains):
+            ctx.put_state(c, pos=x, unc=x, vel=v, force=np.zeros_like(x))
+            (ctx.update_cell_list if cutoff == sopg.CUTOFF_CELL else ctx.update_neighbor_list)(c)
+            ctx.update_pair_list(c)
+            ctx.force_eval(c)
+        return ctx
+
+    def snapshot(ctx):
+        out = []
+        for c in range(len(chains)):
+            st = ctx.get_state(c)
+            lists = [ctx.get_list(w, c) for w in (sopg.LIST_NL_ATT, sopg.LIST_NL_REP, sopg.LIST_PL_ATT,
+                                                  sopg.LIST_PL_REP)]
+            out.append((st, lists))
+        return out
+
+    print(f"[gpu] split runs cutoff={cutoff}", flush=True)
+    one = make()
+    rec_one = one.run(120)
+    ref = snapshot(one)
+    one.close()
+    splits = (7, 13, 20, 1, 19, 40, 20)
+    for disturb in (False, True):
+        ctx = make()
+        recs = []
+        for i, k in enumerate(splits):
+            recs += ctx.run(k)
+            if disturb and i == 1:
+                ctx.update_pair_list(0)
+            if disturb and i == 3:
+                st = ctx.get_state(2)
+                ctx.put_state(2, pos=st["pos"], unc=st["unc"], vel=st["vel"], force=st["force"])
+            if i == 4:
+                ctx.energy_eval(1)  # eager work that changes nothing
+        got = snapshot(ctx)
+        ctx.close()
+        assert recs == rec_one
+        for (sa, la), (sb, lb) in zip(ref, got):
+            for k in ("unc", "vel", "force", "pos"):
+                assert np.array_equal(sa[k], sb[k]), (disturb, k)
+            for (ai, aj), (bi, bj) in zip(la, lb):
+                assert np.array_equal(ai, bi) and np.array_equal(aj, bj), disturb
+
+
 @pytest.mark.parametrize("cutoff,lanes", [(sopg.CUTOFF_NEIGHBOR, 0), (sopg.CUTOFF_CELL, 0),
                                           (sopg.CUTOFF_NEIGHBOR, 1), (sopg.CUTOFF_NEIGHBOR, 4)])
 def test_resident_kernel_equals_per_step(inputs_root, cutoff, lanes):
