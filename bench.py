@@ -34,8 +34,9 @@ processes on the host's cores in two waves of at most --cpu-workers at once.
 roofline: the fused step kernel's algorithmic bytes per launch (per chain
 192 N + 16 nbnd + 8 nang + 40 nnl_att + 8 nnl_rep) over its average launch
 time, measured with HIP events on the library's stream over a second run of
-the same batch; the list rebuild's (per chain and rebuild 40 ncon_att + 24 N
-+ 40 nnl_att + 8 nnl_rep) likewise.  Both against 8 TB/s.
+the same batch (200 steps, ten list rebuilds); the list rebuild's (per chain
+and rebuild 40 ncon_att + 24 N + 40 nnl_att + 8 nnl_rep) likewise.  Both
+against 8 TB/s.
 """
 import argparse
 import json
@@ -415,7 +416,7 @@ def main():
     topo_mixed = [sopg.chain_topology(x) for x, _, _ in mixed]
     w = W.WORKLOADS["mixed256"]
     g, _ = gpu_batched(mixed, w, W.KT, a.steps, a.warmup, a.lanes, topo_mixed, graphs=not a.no_graphs, skin=a.skin,
-                       profile_steps=min(max(a.steps, 20), 400))
+                       profile_steps=200)
     nb = g["beads"]
     value = nb * a.steps / g["seconds"]
     log(f"mixed256 batched {value:.4g} bead-steps/s ({g['seconds'] * 1e3 / a.steps:.4f} ms/step)")
