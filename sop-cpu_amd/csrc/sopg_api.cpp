@@ -161,9 +161,11 @@ struct sopg_ctx {
   long gclock = 0;
   long graph_launches = 0, graph_captures = 0;
   // Every call that can change positions, lists, parameters or the chain set
-  // advances gen.  A run leaves the next step's pair list (at the positions it
-  // stops at) in the other parity; the next run starts from it without a
-  // pair-list filter when gen has not moved since (next_gen == gen).
+  // advances gen (the lane counts and the rebuild method do not: they change
+  // how lists are built or walked, not what they hold).  A run leaves the next
+  // step's pair list (at the positions it stops at) in the other parity; the
+  // next run starts from it without a pair-list filter when gen has not moved
+  // since (next_gen == gen).
   long gen = 0, next_gen = -1;
   // resident (persistent) kernel for small chains
   int resident_max = 0;         // chains of at most this many beads (0: off)
@@ -981,7 +983,6 @@ int sopg_set_tables(sopg_ctx* c, const sopg_tables* t) {
 
 int sopg_set_lanes(sopg_ctx* c, int step_lanes, int list_lanes) {
   if (!c) return SOPG_ERR_ARG;
-  c->gen++;
   auto ok = [](int l) { return l >= 1 && l <= 64 && !(l & (l - 1)); };
   if (step_lanes == 0) step_lanes = 8;
   if (list_lanes == 0) list_lanes = 64;
@@ -1000,7 +1001,6 @@ int sopg_get_lanes(const sopg_ctx* c, int* step_lanes, int* list_lanes) {
 
 int sopg_set_binned(sopg_ctx* c, int enable) {
   if (!c) return SOPG_ERR_ARG;
-  c->gen++;
   c->binned = enable ? 1 : 0;
   return SOPG_OK;
 }
