@@ -22,7 +22,8 @@ before the warmup).  The same line carries the other configurations:
   mixed256       batched (the headline) and plain (one chain per context)
   serve          the Poisson job stream (20 jobs/s, N log-uniform in
                  [50, 10000], steps log-uniform in [1e4, 1e6], kT in [0.5, 1.3],
-                 half folding from a self-avoiding walk) replayed in real time
+                 half folding from a self-avoiding walk that keeps native pairs
+                 at least their native distance apart) replayed in real time
                  against the continuous-batching server: throughput, p50/p99
                  latency and wait, saturation against the server's own
                  measured capacity
@@ -478,7 +479,8 @@ def main():
                             f"N log-uniform [50, 10000]; steps log-uniform [1e4, 1e6]"
                             + ("" if a.serve_step_scale == 1.0 else f" x {a.serve_step_scale:g}")
                             + ", rounded to a multiple of 1000; kT uniform [0.5, 1.3]; half unfolding (from "
-                              "native), half folding (from a self-avoiding walk); neighbor list every 20 steps; "
+                              "native), half folding (from a self-avoiding walk within 200 A that keeps every native pair at "
+                              "least its native distance apart); neighbor list every 20 steps; "
                               f"Q/Rg/energy record every {W.NUP} steps",
                      hbm_budget_gib=a.serve_budget_gib, chunk=a.serve_chunk)
         log(f"serve {serve['value']:.4g} bead-steps/s p50 {serve['p50_latency_s']:.3f} s "

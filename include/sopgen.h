@@ -17,6 +17,10 @@ extern "C" {
 
 /* Self-avoiding walk: 3.8 A bonds, non-bonded pairs >= 4.0 A.  xyz: 3n doubles. */
 int sopgen_saw(int n, uint64_t seed, double* xyz);
+/* Unfolded start of a folding run: a SAW inside radius rmax (<= 0: unbounded)
+ * that keeps every native pair of native_xyz (j-i>2, closer than 8 A there) at
+ * least its native distance apart.  xyz: 3n doubles. */
+int sopgen_unfolded(int n, uint64_t seed, const double* native_xyz, double rmax, double* xyz);
 /* Native structure: SAW + Monte Carlo collapse into a sphere of 3.0*n^(1/3) A. */
 int sopgen_native(int n, uint64_t seed, double* xyz);
 /* Maxwell velocities at temperature kT (unit mass). */

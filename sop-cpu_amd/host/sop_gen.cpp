@@ -94,8 +94,13 @@ inline void rotate(const double* p, const double* c, const double* k, double ang
   for (int a = 0; a < 3; a++) out[a] = c[a] + v[a] * cs + kx[a] * sn + k[a] * kv * (1.0 - cs);
 }
 
+// Per bead i, its native partners j < i with their squared native distances.
+using NativeFloor = std::vector<std::vector<std::pair<int, double>>>;
+
 // Grow a SAW bead by bead inside radius rmax (<=0: unbounded) with backtracking.
-bool grow_saw(int n, Rng& rng, double rmax, std::vector<double>& X) {
+// With floor, a bead is also placed no closer to an earlier native partner than
+// their native distance.
+bool grow_saw(int n, Rng& rng, double rmax, std::vector<double>& X, const NativeFloor* floor = nullptr) {
   X.assign(3 * (size_t)n, 0.0);
   Grid g;
   g.add(0, &X[0]);
@@ -111,6 +116,12 @@ bool grow_saw(int n, Rng& rng, double rmax, std::vector<double>& X) {
       for (int a = 0; a < 3; a++) p[a] = X[3 * (i - 1) + a] + kBond * u[a];
       if (rmax > 0 && norm3(p) > rmax) continue;
       ok = !g.clash(p, X, [&](int j) { return j <= i - 2; });
+      if (ok && floor)
+        for (const auto& pr : (*floor)[i]) {
+          const double* q = &X[3 * pr.first];
+          const double dx = q[0] - p[0], dy = q[1] - p[1], dz = q[2] - p[2];
+          if (dx * dx + dy * dy + dz * dz < pr.second) { ok = false; break; }
+        }
     }
     if (ok) {
       for (int a = 0; a < 3; a++) X[3 * i + a] = p[a];
@@ -232,6 +243,33 @@ int sopgen_saw(int n, uint64_t seed, double* xyz) {
   if (!grow_saw(n, rng, 0.0, X)) return -2;
   std::memcpy(xyz, X.data(), sizeof(double) * 3 * (size_t)n);
   return 0;
+}
+
+long sopgen_natives(int n, const double* xyz, double rcut, int* oi, int* oj, double* orr, long cap);
+
+// Unfolded start of a folding run: a SAW inside radius rmax (<= 0: unbounded)
+// that keeps every native pair of native_xyz (j - i > 2, closer than 8 A there)
+// at least its native distance apart.  A plain SAW puts some native pairs at
+// half their native distance, where the 12-6 native term's repulsion is
+// thousands of times its well depth and breaks the FENE bonds in the first
+// steps.
+int sopgen_unfolded(int n, uint64_t seed, const double* native_xyz, double rmax, double* xyz) {
+  if (n < 1 || !native_xyz || !xyz) return -1;
+  const long cap = 64L * n + 64;
+  std::vector<int> ni((size_t)cap), nj((size_t)cap);
+  std::vector<double> nr((size_t)cap);
+  const long nn = sopgen_natives(n, native_xyz, 8.0, ni.data(), nj.data(), nr.data(), cap);
+  if (nn < 0 || nn > cap) return -1;
+  NativeFloor fl((size_t)n);
+  for (long k = 0; k < nn; k++) fl[nj[k] - 1].push_back({ni[k] - 1, nr[k] * nr[k]});
+  Rng rng(seed);
+  std::vector<double> X;
+  for (int attempt = 0; attempt < 8; attempt++)
+    if (grow_saw(n, rng, rmax, X, &fl)) {
+      std::memcpy(xyz, X.data(), sizeof(double) * 3 * (size_t)n);
+      return 0;
+    }
+  return -2;
 }
 
 // Native structure: SAW grown in a sphere of 1.3 R, collapsed into R = 3.0*n^(1/3).

@@ -87,7 +87,7 @@ EXPORTS = [
     "sopg_update_cell_list", "sopg_update_pair_list", "sopg_force_eval", "sopg_energy_eval",
     "sopg_underdamped_iteration", "sopg_overdamped_iteration", "sopg_calculate_observables", "sopg_get_list", "sopg_run",
     "sopg_set_step", "sopg_sync", "sopg_list_counts", "sopg_profile", "sopg_profile_read",
-    "sopgen_saw", "sopgen_native", "sopgen_maxwell", "sopgen_check", "sopgen_write_topology",
+    "sopgen_saw", "sopgen_unfolded", "sopgen_native", "sopgen_maxwell", "sopgen_check", "sopgen_write_topology",
     "sopgen_write_xyz", "sopgen_natives", "sopg_force_terms", "sopg_set_binned",
     "sopg_get_lanes", "sopg_autotune", "sopg_reserve", "sopg_remove_chain", "sopg_get_step",
     "sopg_checkpoint", "sopg_restore", "sopg_set_graphs", "sopg_graph_stats", "sopg_set_resident", "sopg_set_skin",
@@ -168,6 +168,7 @@ def lib():
         L.sopg_server_evict.argtypes = [vp, C.c_long]
         L.sopg_server_stats.argtypes = [vp, _DP, C.POINTER(C.c_long), _DP]
         L.sopgen_saw.argtypes = [C.c_int, C.c_uint64, vp]
+        L.sopgen_unfolded.argtypes = [C.c_int, C.c_uint64, vp, C.c_double, vp]
         L.sopgen_native.argtypes = [C.c_int, C.c_uint64, vp]
         L.sopgen_maxwell.argtypes = [C.c_int, C.c_double, C.c_uint64, vp]
         L.sopgen_check.argtypes = [C.c_int, vp, _DP, _DP, _DP]
@@ -211,6 +212,19 @@ def gen_saw(n, seed):
     rc = lib().sopgen_saw(n, seed, _ptr(x))
     if rc:
         raise SopgError(f"sopgen_saw failed: {rc}")
+    return x
+
+
+def gen_unfolded(native, seed, rmax=200.0):
+    """Unfolded start for a folding run of the chain whose native structure is `native`: a
+    self-avoiding walk within rmax of the origin that keeps every native pair at least its
+    native distance apart."""
+    native = np.ascontiguousarray(native, dtype=np.float64)
+    n = native.size // 3
+    x = np.zeros(3 * n)
+    rc = lib().sopgen_unfolded(n, seed, _ptr(native), rmax, _ptr(x))
+    if rc:
+        raise SopgError(f"sopgen_unfolded failed: {rc}")
     return x
 
 

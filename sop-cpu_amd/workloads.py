@@ -17,7 +17,9 @@ set t_step 0.05` (h = 0.05, zeta = 50), then the temperature.
                 [50, 10000], steps log-uniform in [1e4, 1e6] (rounded to a
                 multiple of 1000), kT uniform in [0.5, 1.3]; even jobs unfold
                 (start from their native structure), odd jobs fold (start from a
-                self-avoiding walk); neighbor list every 20 steps
+                self-avoiding walk within 200 A of the origin that keeps every
+                native pair at least its native distance apart); neighbor list
+                every 20 steps
 """
 from concurrent.futures import ThreadPoolExecutor
 
@@ -106,7 +108,7 @@ def serve_jobs(njobs, step_scale, threads, seed=5, rate=20.0):
     def one(j):
         n = int(sizes[j])
         nat = sopg.gen_native(n, 20_000 + j)
-        x0 = nat if j % 2 == 0 else sopg.gen_saw(n, 30_000 + j)
+        x0 = nat if j % 2 == 0 else sopg.gen_unfolded(nat, 30_000 + j)
         return x0, sopg.gen_maxwell(n, float(kts[j]), 20_000 + j), sopg.chain_topology(nat)
     # largest first, so that the slowest structures start early
     order = sorted(range(njobs), key=lambda j: -sizes[j])

@@ -68,6 +68,26 @@ def test_generator_constraints():
     assert np.array_equal(sopg.gen_native(76, 1), sopg.gen_native(76, 1))  # seeded, deterministic
 
 
+def test_unfolded_start_keeps_native_pairs_apart():
+    """The folding runs' start: a self-avoiding walk (3.8 A bonds, non-bonded pairs >= 4 A)
+    within rmax that keeps every native pair at least its native distance apart, where a
+    plain walk puts some at about half of it."""
+    n = 1500
+    nat = sopg.gen_native(n, 7)
+    t = sopg.chain_topology(nat)
+    i, j, r0 = t["att_i"] - 1, t["att_j"] - 1, t["att_r"]
+    x = sopg.gen_unfolded(nat, 8, rmax=120.0)
+    a, b, c = (ctypes.c_double() for _ in range(3))
+    sopg.lib().sopgen_check(n, x.ctypes.data_as(ctypes.c_void_p), ctypes.byref(a), ctypes.byref(b),
+                            ctypes.byref(c))
+    assert a.value >= 4.0 - 1e-9 and b.value < 1e-9 and c.value <= 120.0 + 1e-9
+    X = x.reshape(-1, 3)
+    assert (np.linalg.norm(X[i] - X[j], axis=1) >= r0 - 1e-9).all()
+    s = sopg.gen_saw(n, 8).reshape(-1, 3)
+    assert (np.linalg.norm(s[i] - s[j], axis=1) / r0).min() < 0.8  # what the floor prevents
+    assert np.array_equal(x, sopg.gen_unfolded(nat, 8, rmax=120.0))
+
+
 def test_written_files_roundtrip(tmp_path):
     """Files written by the generator load (through the oracle's restatement of the
     reference loaders) into exactly the arrays chain_topology() hands the device."""

@@ -11,7 +11,7 @@ replayed where the configuration allows it:
                every chain's lists and state equal a full-scan context without
                candidate lists; the smallest, median and largest chains run with
                replayed noise and equal the oracle
-  serving      a stream with an N=10^4 folding job (from a self-avoiding walk) and
+  serving      a stream with an N=10^4 folding job (from an unfolded start) and
                N=50 jobs at kT 0.5 and 1.3, arriving while others run: each job's
                result equals the same job run alone
 """
@@ -237,12 +237,12 @@ SERVE_JOBS = [  # (N, kind, kT, nsteps, submitted after this many rounds)
 
 def _job_inputs(j, n, kind, kt):
     nat = sopg.gen_native(n, 40_000 + j)
-    x0 = nat if kind == "unfold" else sopg.gen_saw(n, 50_000 + j)
+    x0 = nat if kind == "unfold" else sopg.gen_unfolded(nat, 50_000 + j)
     return x0, sopg.gen_maxwell(n, kt, 40_000 + j), sopg.chain_topology(nat)
 
 
 def test_served_stream_with_large_and_small_jobs():
-    """Jobs arriving while others run, up to N=10^4 folding from a self-avoiding walk and
+    """Jobs arriving while others run, up to N=10^4 folding from an unfolded start and
     N=50 at kT 0.5 and 1.3: each served job's final state and records equal the job alone."""
     w = workloads.SERVE
     with ThreadPoolExecutor(max_workers=8) as ex:
