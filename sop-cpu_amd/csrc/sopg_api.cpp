@@ -139,6 +139,17 @@ struct sopg_ctx {
   bool device_ready = false;
   Dev D{};
   std::vector<ChainP> cp;
+  // Pinned staging of the chain records and of the error flag.  A full upload
+  // of the chain records goes through cp_pin without waiting for the copy
+  // (ev_cp marks it), and is skipped when the records equal what the device
+  // holds (cp_dev, uploaded at generation cp_dev_gen) and gen has not moved.
+  ChainP* cp_pin = nullptr;
+  size_t cp_pin_cap = 0;
+  bool cp_busy = false;
+  hipEvent_t ev_cp = nullptr;
+  std::vector<ChainP> cp_dev;
+  long cp_dev_gen = -1;
+  int* err_pin = nullptr;
   Pool pool[NPOOL];
   std::vector<Arr> arrs[NPOOL];
   std::vector<void*> allocs;  // fixed allocations (not in pools)
@@ -167,6 +178,10 @@ struct sopg_ctx {
   // next run starts from it without a pair-list filter when gen has not moved
   // since (next_gen == gen).
   long gen = 0, next_gen = -1;
+  // The last run's last step also wrote the position half that follows it
+  // into buffer cur^1 (no chain was resident); with next_gen == gen the next
+  // run starts by flipping cur instead of the in-place position update.
+  bool adv_ready = false;
   // resident (persistent) kernel for small chains
   int resident_max = 0;         // chains of at most this many beads (0: off)
   int resident_lanes = 0;       // lanes per bead in it (0: automatic)
@@ -474,7 +489,9 @@ void fill_chainp(const HostChain& h, const Tables& T, ChainP& P) {
 }
 
 // Device copy of the per-chain parameters: chain `only`, or all (only < 0).
-int upload_chainp(sopg_ctx* c, int only = -1) {
+// may_alloc false: no allocation, not even of pinned host memory (a staging
+// buffer too small for the records falls back to a waited copy).
+int upload_chainp(sopg_ctx* c, int only = -1, bool may_alloc = true) {
   if (!c->device_ready) return SOPG_OK;
   const size_t cap = c->pool[P_CHAIN].cap;
   if (c->cp.size() != cap) {
@@ -486,8 +503,32 @@ int upload_chainp(sopg_ctx* c, int only = -1) {
     if (only >= 0 && (int)i != only) continue;
     fill_chainp(i < c->chains.size() ? c->chains[i] : none, c->T, c->cp[i]);
   }
-  if (only >= 0) return upload_at(c, c->D.cp, (size_t)only, &c->cp[only], 1);
-  return upload_at(c, c->D.cp, 0, c->cp);
+  if (only >= 0) {
+    c->cp_dev_gen = -1;
+    return upload_at(c, c->D.cp, (size_t)only, &c->cp[only], 1);
+  }
+  if (c->cp_dev_gen == c->gen && c->cp_dev.size() == cap &&
+      !std::memcmp(c->cp_dev.data(), c->cp.data(), cap * sizeof(ChainP)))
+    return SOPG_OK;
+  if (cap > c->cp_pin_cap && may_alloc) {
+    if (c->cp_busy) HIP_OK(c, hipEventSynchronize(c->ev_cp));
+    c->cp_busy = false;
+    if (c->cp_pin) HIP_OK(c, hipHostFree(c->cp_pin));
+    c->cp_pin = nullptr;
+    c->cp_pin_cap = 0;
+    HIP_OK(c, hipHostMalloc((void**)&c->cp_pin, 2 * cap * sizeof(ChainP), hipHostMallocDefault));
+    c->cp_pin_cap = 2 * cap;
+  }
+  c->cp_dev_gen = -1;
+  if (cap > c->cp_pin_cap) return upload_at(c, c->D.cp, 0, c->cp);
+  if (c->cp_busy) HIP_OK(c, hipEventSynchronize(c->ev_cp));
+  std::memcpy(c->cp_pin, c->cp.data(), cap * sizeof(ChainP));
+  HIP_OK(c, hipMemcpyAsync(c->D.cp, c->cp_pin, cap * sizeof(ChainP), hipMemcpyHostToDevice, c->stream));
+  HIP_OK(c, hipEventRecord(c->ev_cp, c->stream));
+  c->cp_busy = true;
+  c->cp_dev = c->cp;
+  c->cp_dev_gen = c->gen;
+  return SOPG_OK;
 }
 
 // Per-bead incidence lists of entries (a[e], b[e]), stable in entry order:
@@ -678,6 +719,8 @@ int ensure_device(sopg_ctx* c) {
       (rc = dalloc(c, &D.plan, 3)))
     return rc;
   HIP_OK(c, hipMemsetAsync(D.err, 0, sizeof(int), c->stream));
+  HIP_OK(c, hipEventCreateWithFlags(&c->ev_cp, hipEventDisableTiming));
+  HIP_OK(c, hipHostMalloc((void**)&c->err_pin, sizeof(int), hipHostMallocDefault));
   const int no_plan[3] = {INT_MIN, INT_MIN, INT_MIN};
   HIP_OK(c, hipMemcpyAsync(D.plan, no_plan, sizeof no_plan, hipMemcpyHostToDevice, c->stream));
   HIP_OK(c, hipStreamSynchronize(c->stream));
@@ -719,16 +762,21 @@ int ensure_noise(sopg_ctx* c, size_t n) {
   return SOPG_OK;
 }
 
-int check_err_flag(sopg_ctx* c) {
-  int e = 0;
-  HIP_OK(c, hipMemcpyAsync(&e, c->D.err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  HIP_OK(c, hipStreamSynchronize(c->stream));
+// The error flag's verdict once its value e has been read.
+int err_verdict(sopg_ctx* c, int e) {
   if (e & 1) {
     HIP_OK(c, hipMemsetAsync(c->D.err, 0, sizeof(int), c->stream));
     return fail(c, SOPG_ERR_OVERFLOW,
                 "a per-bead non-native neighbor list exceeded its capacity; raise it with sopg_set_capacity");
   }
   return SOPG_OK;
+}
+
+int check_err_flag(sopg_ctx* c) {
+  int e = 0;
+  HIP_OK(c, hipMemcpyAsync(&e, c->D.err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(c, hipStreamSynchronize(c->stream));
+  return err_verdict(c, e);
 }
 
 // Chain reduction of one chain into host array out[kRed].
@@ -931,6 +979,9 @@ int sopg_destroy(sopg_ctx* c) {
   for (void* p : c->allocs) (void)hipFree(p);
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
   if (c->ev_res) (void)hipEventDestroy(c->ev_res);
+  if (c->ev_cp) (void)hipEventDestroy(c->ev_cp);
+  if (c->cp_pin) (void)hipHostFree(c->cp_pin);
+  if (c->err_pin) (void)hipHostFree(c->err_pin);
   if (c->stream2) (void)hipStreamDestroy(c->stream2);
   for (auto& ge : c->gcache) (void)hipGraphExecDestroy(ge.exec);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1657,7 +1708,7 @@ static int run_impl(sopg_ctx* c, long nsteps, const double* noise, sopg_record* 
         rmaxn = std::max(rmaxn, h.n);
       }
     }
-  if ((rc = upload_chainp(c))) return rc;
+  if ((rc = upload_chainp(c, -1, !fixed))) return rc;
   const int nb = c->D.nb, nc = (int)c->pool[P_CHAIN].cap;
   hipStream_t s = c->stream;
   if (noise) {
@@ -1749,6 +1800,7 @@ static int run_impl(sopg_ctx* c, long nsteps, const double* noise, sopg_record* 
     if (rl <= 0)
       for (rl = 8; rl > 1 && rmaxn * rl > 512; rl >>= 1) {}
     const int threads = (rmaxn * rl + 63) / 64 * 64;
+    if (c->cp_busy) HIP_OK(c, hipStreamWaitEvent(c->stream2, c->ev_cp, 0));  // the chain records it reads
     launch_resident(rl, D, c->d_T, c->d_rlist, (int)rlist.size(), threads, c->cur, c->par, (int)nsteps, c->d_rslot,
                     c->d_rec, nc, c->stream2);
     HIP_OK(c, hipGetLastError());
@@ -1780,7 +1832,12 @@ static int run_impl(sopg_ctx* c, long nsteps, const double* noise, sopg_record* 
     launch_pairfilter(L, D, c->d_T, c->cur, c->par, 0, nb, s);  // a few dozen entries per bead: step lanes
     prof_end(c, SOPG_KERNEL_PAIRLIST, pe);
   }
-  if (stepped) launch_posupdate(D, c->cur, c->cur, 0, nb, s);
+  // the position half of the run's first step: computed by the previous
+  // run's last step into buffer cur^1 when nothing has changed since
+  if (stepped) {
+    if (next_pl && c->adv_ready && rlist.empty()) c->cur ^= 1;
+    else launch_posupdate(D, c->cur, c->cur, 0, nb, s);
+  }
   // The launches of run step r.  Kernels see step k + *Dx.relb: direct
   // launches pass k = r with base 0; a captured segment passes k relative to
   // its first step, whose run step the replay writes to the base.
@@ -1790,10 +1847,10 @@ static int run_impl(sopg_ctx* c, long nsteps, const double* noise, sopg_record* 
     int pe = prof_begin(c);
     // energy terms only in a step at which some chain records
     const int fe = records(r) ? F_ENERGY : 0;
-    // every step computes the next step's pair list; the last one leaves the
-    // positions at rest (its pair list is the one the next run starts with)
-    launch_force(L, Dx, c->d_T, c->cur, c->par, k, F_STORE | F_VEL | fe | F_ALLTERMS | (last ? 0 : F_ADVANCE), true,
-                 nz, 0, nb, s);
+    // every step computes the next step's pair list and position half (into
+    // buffer cur^1); the last one leaves cur at rest, and its pair list and
+    // position half are where the next run starts
+    launch_force(L, Dx, c->d_T, c->cur, c->par, k, F_STORE | F_VEL | fe | F_ALLTERMS | F_ADVANCE, true, nz, 0, nb, s);
     prof_end(c, SOPG_KERNEL_STEP, pe);
     if (records(r)) {
       pe = prof_begin(c);
@@ -1889,13 +1946,15 @@ static int run_impl(sopg_ctx* c, long nsteps, const double* noise, sopg_record* 
     h.resident = false;
   }
   c->next_gen = stepped ? c->gen : -1;
+  c->adv_ready = stepped && rlist.empty();
   std::vector<double> hr(rec_steps.size() * nc * kRed);
   if (!hr.empty())
     HIP_OK(c, hipMemcpyAsync(hr.data(), fixed ? rec_buf : c->d_rec, sizeof(double) * hr.size(),
                              hipMemcpyDeviceToHost, s));
+  HIP_OK(c, hipMemcpyAsync(c->err_pin, c->D.err, sizeof(int), hipMemcpyDeviceToHost, s));
   HIP_OK(c, hipStreamSynchronize(s));
   prof_collect(c);
-  if ((rc = check_err_flag(c))) return rc;
+  if ((rc = err_verdict(c, *c->err_pin))) return rc;
   long nr = 0;
   for (size_t q = 0; q < rec_steps.size(); q++) {
     const long r = rec_steps[q];
@@ -1928,7 +1987,7 @@ static int run_impl(sopg_ctx* c, long nsteps, const double* noise, sopg_record* 
     h.iup = (int)(((long)h.iup - 1 + nsteps) % h.p.nup) + 1;
     h.noise_base += (uint64_t)nsteps;
   }
-  return upload_chainp(c);
+  return upload_chainp(c, -1, !fixed);
 }
 
 int sopg_run(sopg_ctx* c, long nsteps, const double* noise, sopg_record* rec, long rec_cap, long* nrec) {
