@@ -169,14 +169,16 @@ __device__ __forceinline__ double buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t byte
 }
 
 // Bead g's slot words (the neighbor list: tiled, slot_index) or candidate
-// words (row-major, cap words per bead).
+// words (row-major, cap words per bead).  The bead's first word is located
+// once; entry q of a tiled row is then (q & ~3) * 16 + (q & 3) words further,
+// a 32-bit offset.
 struct ListRow {
-  uint32_t* base;
-  int g;
-  int cap;
+  uint32_t* row;
   bool tiled;
+  __device__ __forceinline__ ListRow(uint32_t* base, int g, int cap, bool t)
+      : row(base + (t ? slot_index(g, 0, cap) : (size_t)g * (size_t)cap)), tiled(t) {}
   __device__ __forceinline__ uint32_t& operator[](int q) const {
-    return tiled ? base[slot_index(g, q, cap)] : base[(size_t)g * (size_t)cap + (size_t)q];
+    return tiled ? row[(uint32_t)(((q & ~3) << 4) | (q & 3))] : row[(uint32_t)q];
   }
 };
 
@@ -198,18 +200,23 @@ __device__ __forceinline__ void neutral3(bool c, double cs, double dx, double dy
 // symmetric, so the two are the same bits except for c == o, where they are
 // -0 and +0; mimg maps both zeros to +0 (d + 0.0 on the fast path; d -
 // boxl*rnd(d/boxl) = +0 on the full form), so the result is the reference's,
-// with one subtraction per component.  One branch covers the common case in
-// which every component takes mimg's exact fast path (|d| <= boxl/4).
+// with one subtraction per component.  The sign and the + 0.0 are one fma:
+// (o - c) * +-1 is exact, so fma((o - c), +-1, 0.0) is that product with -0
+// mapped to +0.  One branch covers the common case in which every component
+// takes mimg's exact fast path (|d| <= boxl/4); otherwise all three take the
+// full form, which equals the fast path's d + 0.0 on a component within it.
 __device__ __forceinline__ void pair_d3(int side, double cx, double cy, double cz, double ox, double oy, double oz,
                                         double boxl, double qbox, double& dx, double& dy, double& dz) {
   const double sg = side ? -1.0 : 1.0;
-  const double rx = (ox - cx) * sg;
-  const double ry = (oy - cy) * sg;
-  const double rz = (oz - cz) * sg;
+  const double rx = fma(ox - cx, sg, 0.0);
+  const double ry = fma(oy - cy, sg, 0.0);
+  const double rz = fma(oz - cz, sg, 0.0);
   if (fabs(rx) <= qbox && fabs(ry) <= qbox && fabs(rz) <= qbox) {
-    dx = rx + 0.0; dy = ry + 0.0; dz = rz + 0.0;
+    dx = rx; dy = ry; dz = rz;
   } else {
-    dx = mimg(rx, boxl, qbox); dy = mimg(ry, boxl, qbox); dz = mimg(rz, boxl, qbox);
+    dx = rx - boxl * rnd(rx / boxl);
+    dy = ry - boxl * rnd(ry / boxl);
+    dz = rz - boxl * rnd(rz / boxl);
   }
 }
 
@@ -1018,21 +1025,25 @@ __global__ void k_posupdate(Dev D, int src, int dst, int g0, int g1) {
 enum { F_STORE = 1, F_VEL = 2, F_ENERGY = 4, F_ENERGY_ALL = 8, F_NOISE = 16, F_BOND = 32, F_ANGLE = 64,
        F_VDW = 128, F_ACCUM = 256, F_ADVANCE = 512 };
 
-// Type-pair tables the pair loops read, staged in LDS (index it * 3 + jt).
-// Rows of 8 doubles (64 bytes): the non-native loop reads entries 0-3 of a
-// row as two 16-byte LDS loads.
+// Type-pair tables the pair loops read, staged in LDS: row it + 4 * jt, the
+// 4-bit type field of a slot word (bits 26-29) or of a native incidence word,
+// so that a row's address is that field times 64 bytes.  Rows of 8 doubles:
+// the non-native loop reads entries 0-3 of a row as two 16-byte LDS loads.
 enum { TP_FREP = 0, TP_S6, TP_S12, TP_PLRC2, TP_CREP, TP_FATT, TP_CATT, TP_PAD, TP_N };
+constexpr int kTypeRows = 16;
 
 __device__ __forceinline__ void load_type_tables(const Tables* __restrict__ T, double (*tp)[TP_N]) {
-  if (threadIdx.x < 9) {
-    const int it = threadIdx.x / 3, jt = threadIdx.x % 3;
-    tp[threadIdx.x][TP_FREP] = T->frep[it][jt];
-    tp[threadIdx.x][TP_S6] = T->s6[it][jt];
-    tp[threadIdx.x][TP_S12] = T->s12[it][jt];
-    tp[threadIdx.x][TP_PLRC2] = T->plrc2[it][jt];
-    tp[threadIdx.x][TP_CREP] = T->crep[it][jt];
-    tp[threadIdx.x][TP_FATT] = T->fatt[it][jt];
-    tp[threadIdx.x][TP_CATT] = T->catt[it][jt];
+  if (threadIdx.x < kTypeRows) {
+    const int it = threadIdx.x & 3, jt = threadIdx.x >> 2;
+    const bool ok = it < 3 && jt < 3;  // types are 0..2
+    const int a = ok ? it : 0, b = ok ? jt : 0;
+    tp[threadIdx.x][TP_FREP] = T->frep[a][b];
+    tp[threadIdx.x][TP_S6] = T->s6[a][b];
+    tp[threadIdx.x][TP_S12] = T->s12[a][b];
+    tp[threadIdx.x][TP_PLRC2] = T->plrc2[a][b];
+    tp[threadIdx.x][TP_CREP] = T->crep[a][b];
+    tp[threadIdx.x][TP_FATT] = T->fatt[a][b];
+    tp[threadIdx.x][TP_CATT] = T->catt[a][b];
   }
 }
 
@@ -1202,7 +1213,7 @@ __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restric
       if (q < p1) {
         if (f & 1) {
           const double r = D.att_r[q];
-          const int tt = (int)(aw >> kAttTShift), side = (tt >> 4) & 1, tpi = (tt & 3) * 3 + ((tt >> 2) & 3);
+          const int tt = (int)(aw >> kAttTShift), side = (tt >> 4) & 1, tpi = tt & 15;
           const double sg = side ? -1.0 : 1.0;
           pair_d3(side, cx, cy, cz, px, py, pz, boxl, qbox, dx, dy, dz);
           const double d2 = dx * dx + dy * dy + dz * dz;
@@ -1265,7 +1276,7 @@ __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restric
       const bool in = q < cnt;
       const int side = (w >> 25) & 1;
       const double sg = side ? -1.0 : 1.0;
-      const int tpi = (int)((w >> 26) & 3) * 3 + (int)((w >> 28) & 3);
+      const int tpi = (int)((w >> 26) & 15);
       double dx, dy, dz;
       pair_d3(side, cx, cy, cz, px, py, pz, boxl, qbox, dx, dy, dz);
       const double d2 = dx * dx + dy * dy + dz * dz;
@@ -1347,7 +1358,7 @@ template <int L, bool NEXT, bool EN>
 __global__ void __launch_bounds__(256, 5) k_force(Dev D, const Tables* __restrict__ T, int cur, int pr, int rel, int flags,
                                                const double* __restrict__ noise, int g0, int g1) {
   rel += *D.relb;
-  __shared__ __align__(16) double tp[9][TP_N];
+  __shared__ __align__(16) double tp[kTypeRows][TP_N];
   load_type_tables(T, tp);
   __syncthreads();
   const int t = xcd_block() * blockDim.x + threadIdx.x;
@@ -1473,7 +1484,7 @@ template <int L>
 __global__ void __launch_bounds__(512) k_resident(Dev D, const Tables* __restrict__ T, const int* __restrict__ chains,
                                                    int cur, int par, int nsteps, const int* __restrict__ rec_slot,
                                                    double* rec, int nc) {
-  __shared__ __align__(16) double tp[9][TP_N];
+  __shared__ __align__(16) double tp[kTypeRows][TP_N];
   __shared__ double sh[256];
   load_type_tables(T, tp);
   const int c = chains[blockIdx.x];
