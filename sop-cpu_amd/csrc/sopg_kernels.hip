@@ -1261,16 +1261,17 @@ __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restric
       pxn = buf_ld(bux, ob); pyn = buf_ld(buy, ob); pzn = buf_ld(buz, ob);
     }
     if (l + L < cnt) wn2 = S[l + L];
+    const int cnt1 = cnt - L, cnt2 = cnt - 2 * L;  // q < cnt1: chunk q + L exists
     for (int base = 0; base < cnt; base += L) {
       const int q = base + l;
       const uint32_t w = wn;
       const double px = pxn, py = pyn, pz = pzn;
-      if (q + L < cnt) {
+      if (q < cnt1) {
         const uint32_t ob = (uoff + (wn2 & kPartnerMask)) * 8u;
         pxn = buf_ld(bux, ob); pyn = buf_ld(buy, ob); pzn = buf_ld(buz, ob);
       }
       wn = wn2;
-      if (q + 2 * L < cnt) wn2 = S[q + 2 * L];
+      if (q < cnt2) wn2 = S[q + 2 * L];
       // every lane evaluates the displacement (a lane past the list end holds
       // finite stale coordinates); the branches cover only what needs the entry
       const bool in = q < cnt;
@@ -1297,9 +1298,9 @@ __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restric
         if (energy && side == 0) { e = tp[tpi][TP_CREP] * (s12 / d12 + s6 / d6); ec = true; }
       }
       if constexpr (NEXT) {
+        // the next step's bit, stored only where it changes
         const bool npl = no_cut || d2 < plrc2;
-        const uint32_t nw = (w & ~wbit) | (npl ? wbit : 0u);
-        if (in && nw != w) S[q] = nw;
+        if (in && npl != ((w & wbit) != 0u)) S[q] = w ^ wbit;
       }
       double fx, fy, fz;
       neutral3(contrib, cs, dx, dy, dz, fx, fy, fz);
