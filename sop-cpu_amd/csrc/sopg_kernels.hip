@@ -192,6 +192,13 @@ __device__ __forceinline__ void neutral3(bool c, double cs, double dx, double dy
   fy = __longlong_as_double(__double_as_longlong(cs * dy) | nz);
   fz = __longlong_as_double(__double_as_longlong(cs * dz) | nz);
 }
+// The same term where cs and d are defined only on lanes with c: a select.
+__device__ __forceinline__ void neutral3s(bool c, double cs, double dx, double dy, double dz, double& fx, double& fy,
+                                          double& fz) {
+  fx = c ? cs * dx : -0.0;
+  fy = c ? cs * dy : -0.0;
+  fz = c ? cs * dz : -0.0;
+}
 
 // Pair displacement u[j] - u[i] seen from bead g, min-imaged, all three
 // components at once: c is g's own position, o the partner's.  The reference
@@ -1196,22 +1203,27 @@ __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restric
       pxn = buf_ld(bux, ob); pyn = buf_ld(buy, ob); pzn = buf_ld(buz, ob);
     }
     if (p0 + l + L < p1) { fn2 = D.att_flag[p0 + l + L]; an2 = D.att_w[p0 + l + L]; }
+    const int p1a = p1 - L, p1b = p1 - 2 * L;  // q < p1a: entry q + L exists
     for (int base = p0; base < p1; base += L) {
       const int q = base + l;
       const uint8_t f = fn;
       const uint32_t aw = an;
       const double px = pxn, py = pyn, pz = pzn;
-      if (q + L < p1) {
+      if (q < p1a) {
         const uint32_t ob = (uoff + (an2 & kPartnerMask)) * 8u;
         pxn = buf_ld(bux, ob); pyn = buf_ld(buy, ob); pzn = buf_ld(buz, ob);
       }
       fn = fn2;
       an = an2;
-      if (q + 2 * L < p1) { fn2 = D.att_flag[q + 2 * L]; an2 = D.att_w[q + 2 * L]; }
+      if (q < p1b) { fn2 = D.att_flag[q + 2 * L]; an2 = D.att_w[q + 2 * L]; }
       bool contrib = false, ec = false;
-      double cs = 0.0, dx = 0.0, dy = 0.0, dz = 0.0, e = -0.0;
-      if (q < p1) {
-        if (f & 1) {
+      // the displacement and cs are read (by neutral3s) only where contrib:
+      // elsewhere they are left unset (no register is cleared for them)
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Wsometimes-uninitialized"
+      double cs, dx, dy, dz, e = -0.0;
+      {
+        if (q < p1 && (f & 1)) {
           const double r = D.att_r[q];
           const int tt = (int)(aw >> kAttTShift), side = (tt >> 4) & 1, tpi = tt & 15;
           const double sg = side ? -1.0 : 1.0;
@@ -1232,14 +1244,15 @@ __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restric
             if (energy && side == 0) { e = tp[tpi][TP_CATT] * (r12 / d12) - 2.0 * (r6 / d6); ec = true; }
           }
           if constexpr (NEXT) {
+            // the next step's bit, stored only where it changes
             const bool npl = no_cut || d2 < nat_plrc2(r);
-            const uint8_t nf = (uint8_t)((f & ~(2 << pw)) | (npl ? (2 << pw) : 0));
-            if (nf != f) D.att_flag[q] = nf;
+            if (npl != (((f >> (1 + pw)) & 1) != 0)) D.att_flag[q] = (uint8_t)(f ^ (2 << pw));
           }
         }
       }
       double fx, fy, fz;
-      neutral3(contrib, cs, dx, dy, dz, fx, fy, fz);
+      neutral3s(contrib, cs, dx, dy, dz, fx, fy, fz);
+#pragma clang diagnostic pop
       fold3<L>(Fx, Fy, Fz, contrib, fx, fy, fz);
       if (energy) fold1<L>(eatt, ec, e);
     }
@@ -1285,7 +1298,10 @@ __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restric
       const double frep = tp[tpi][TP_FREP], s6 = tp[tpi][TP_S6], s12 = tp[tpi][TP_S12];
       const double plrc2 = tp[tpi][TP_PLRC2];
       bool contrib = false, ec = false;
-      double cs = 0.0, e = -0.0;
+      // cs is read (by neutral3s) only where contrib
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Wsometimes-uninitialized"
+      double cs, e = -0.0;
       if (in && (w & rbit)) {
         const double d6 = d2 * d2 * d2;
         const double d12 = d6 * d6;
@@ -1303,7 +1319,8 @@ __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restric
         if (in && npl != ((w & wbit) != 0u)) S[q] = w ^ wbit;
       }
       double fx, fy, fz;
-      neutral3(contrib, cs, dx, dy, dz, fx, fy, fz);
+      neutral3s(contrib, cs, dx, dy, dz, fx, fy, fz);
+#pragma clang diagnostic pop
       fold3<L>(Fx, Fy, Fz, contrib, fx, fy, fz);
       if (energy) fold1<L>(erep, ec, e);
     }
