@@ -234,6 +234,25 @@ int sopg_sync(sopg_ctx* ctx);
  * reference's counters, global.h:96-101) without changing any state. */
 int sopg_list_counts(sopg_ctx* ctx, int chain, int* nnl_att, int* nnl_rep, int* nil_att, int* nil_rep);
 
+/* How the latest list rebuild that planned this chain built its lists (no
+ * counterpart in the reference, whose rebuild always scans every master pair):
+ *   SOPG_PLAN_NONE      no rebuild was due for the chain in it
+ *   SOPG_PLAN_LOCAL     binned on a grid over the chain's extent
+ *   SOPG_PLAN_PERIODIC  binned on a grid over the box (a chain wider than the
+ *                       box less one grid edge)
+ *   SOPG_PLAN_SCAN      the full scan (a cell-list chain too wide for the local
+ *                       grid, a box under three grid edges, a chain beyond the
+ *                       binned rebuild's bead limit, non-finite positions)
+ *   SOPG_PLAN_FILTER    the neighbor list filtered from the chain's candidate
+ *                       list, which was still valid
+ * cells: cells per axis of the grid (0 without one). */
+#define SOPG_PLAN_NONE 0
+#define SOPG_PLAN_LOCAL 1
+#define SOPG_PLAN_PERIODIC 2
+#define SOPG_PLAN_SCAN 3
+#define SOPG_PLAN_FILTER 4
+int sopg_rebuild_plan(sopg_ctx* ctx, int chain, int* plan, int* cells);
+
 /* ------------------------------------------------------------- measurement */
 #define SOPG_KERNEL_STEP 0     /* fused force / velocity / next-position kernel */
 #define SOPG_KERNEL_REBUILD 1  /* neighbor / cell list rebuild */

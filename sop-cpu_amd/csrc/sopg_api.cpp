@@ -1562,6 +1562,23 @@ int sopg_sync(sopg_ctx* c) {
   return SOPG_OK;
 }
 
+int sopg_rebuild_plan(sopg_ctx* c, int chain, int* plan, int* cells) {
+  HostChain* h;
+  ChainP* P;
+  int rc = get_chain(c, chain, &h, &P);
+  if (rc) return rc;
+  GridP G;
+  HIP_OK(c, hipMemcpyAsync(&G, c->D.grid + chain, sizeof G, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(c, hipStreamSynchronize(c->stream));
+  int p = SOPG_PLAN_NONE, n = 0;
+  if (G.work & W_SCAN) p = SOPG_PLAN_SCAN;
+  else if (G.work & W_BIN) { p = G.periodic ? SOPG_PLAN_PERIODIC : SOPG_PLAN_LOCAL; n = G.nx; }
+  else if (G.work & W_FILTER) p = SOPG_PLAN_FILTER;
+  if (plan) *plan = p;
+  if (cells) *cells = n;
+  return SOPG_OK;
+}
+
 int sopg_list_counts(sopg_ctx* c, int chain, int* nnl_att, int* nnl_rep, int* nil_att, int* nil_rep) {
   HostChain* h;
   ChainP* P;

@@ -75,7 +75,7 @@ struct GridP {
   int nx, ny, nz;
   int work;            // W_* of the current rebuild
   int cand_valid;      // the candidate list holds the pairs within 3.2 sigma + skin of u0
-  int pad;
+  int periodic;        // the grid covers the box with wrapped coordinates (cells adjacent modulo n)
 };
 
 // Word of one native-contact incidence that the step kernel reads:

@@ -15,6 +15,7 @@
 // and therefore produce the same bits.
 #include <hip/hip_runtime.h>
 #include <mutex>
+#include <type_traits>
 #include "sopg_device.h"
 #include "sopg_internal.h"
 #include "../../include/sopg.h"
@@ -394,8 +395,11 @@ __global__ void __launch_bounds__(256) k_rebuild(Dev D, const Tables* __restrict
 // O(N): beads are binned into a per-chain grid; a wave per bead tests the 27
 // neighbouring cells, marks accepted partners in an LDS bitmap (one bit per
 // chain bead) and emits the bitmap in ascending order -- the reference's
-// lexicographic order -- with a wave prefix sum.  Chains whose extent leaves
-// the exact min-image fast path keep the full scan of k_rebuild.
+// lexicographic order -- with a wave prefix sum.  The grid covers the chain's
+// extent, or, for a distance-criterion chain wider than the box less one grid
+// edge, the box itself with wrapped coordinates (neighbouring cells modulo the
+// cell count).  A cell-list chain too wide for the local grid keeps the full
+// scan of k_rebuild.
 
 __device__ __forceinline__ bool rebuild_due(const ChainP& P, int rel, int mode, int& crit) {
   crit = mode;
@@ -446,8 +450,8 @@ __global__ void __launch_bounds__(kGridThreads) k_grid(Dev D, const Tables* __re
   } u;
   auto& sh = u.sh;
   __shared__ int part[kGridThreads];
-  __shared__ double gf[4];
-  __shared__ int gi[4];
+  __shared__ double gf[5];
+  __shared__ int gi[5];
   if (cand && G.cand_valid) {
     // largest squared displacement from the candidate lists' positions
     double dmax = 0.0;
@@ -509,26 +513,48 @@ __global__ void __launch_bounds__(kGridThreads) k_grid(Dev D, const Tables* __re
     // min image.
     const double gedge = cell ? 1.0 : (cand ? P.gcand : P.gcell);
     const double limit = cell ? P.ncell - 2.0 : P.boxl - gedge;
-    const double inv = 1.0 / gedge;
+    double inv = 1.0 / gedge;
     int dims[3];
-    bool use = true;
+    bool use = true, fin = true;
     for (int a = 0; a < 3; a++) {
       const double ext = sh[3 + a][0] - sh[a][0];
+      fin = fin && isfinite(ext);
       use = use && ext <= limit && isfinite(ext);
       dims[a] = use ? (int)floor(ext * inv) + 1 : 0;
       use = use && dims[a] <= kGridDim;
     }
     G.ox = sh[0][0]; G.oy = sh[1][0]; G.oz = sh[2][0];
+    // A distance-criterion chain too wide for that grid is binned periodically:
+    // coordinates wrapped into [0, boxl), n cells of edge boxl / n >= the grid
+    // edge per axis (3 <= n <= 32), neighbouring cells taken modulo n.  A pair
+    // within the cutoff (under the reference's min image) has wrapped
+    // coordinates that differ by less than one cell edge modulo boxl, so its
+    // beads lie in cells adjacent modulo n (n >= 3 keeps the 27 cells distinct).
+    int per = 0;
+    if (!use && !cell && fin && P.boxl > 0.0) {
+      int n = (int)fmin(floor(P.boxl / gedge), (double)kGridDim);
+      while (n >= 3 && P.boxl / n < gedge) n--;
+      if (n >= 3) {
+        per = 1;
+        use = true;
+        dims[0] = dims[1] = dims[2] = n;
+        inv = (double)n / P.boxl;
+        G.ox = G.oy = G.oz = 0.0;
+      }
+    }
     G.inv = inv;
     G.nx = dims[0]; G.ny = dims[1]; G.nz = dims[2];
-    // a chain too wide for the grid builds its candidate list by the full scan
+    G.periodic = per;
+    // a chain that neither grid takes (a cell-list chain too wide for the local
+    // grid, a box under three grid edges, non-finite positions) builds its lists
+    // by the full scan
     G.work = (use ? W_BIN : W_SCAN) | (cand ? (W_CAND | W_FILTER) : 0);
     if (G.work & W_BIN) D.plan[0] = rel;
     if (G.work & W_FILTER) D.plan[1] = rel;
     if (G.work & W_SCAN) D.plan[2] = rel;
     if (cand) G.cand_valid = 1;
-    gf[0] = G.ox; gf[1] = G.oy; gf[2] = G.oz; gf[3] = inv;
-    gi[0] = dims[0]; gi[1] = dims[1]; gi[2] = dims[2]; gi[3] = G.work;
+    gf[0] = G.ox; gf[1] = G.oy; gf[2] = G.oz; gf[3] = inv; gf[4] = P.boxl;
+    gi[0] = dims[0]; gi[1] = dims[1]; gi[2] = dims[2]; gi[3] = G.work; gi[4] = per;
   }
   __syncthreads();
   if (!(gi[3] & W_BIN)) return;  // block-uniform
@@ -537,15 +563,19 @@ __global__ void __launch_bounds__(kGridThreads) k_grid(Dev D, const Tables* __re
   // the beads of each cell, contiguous in cell order (any order inside a cell:
   // the rebuild's bitmaps make the lists independent of it).
   const int nx = gi[0], ny = gi[1], nz = gi[2], ncell = nx * ny * nz;
-  const double gox = gf[0], goy = gf[1], goz = gf[2], ginv = gf[3];
+  const double gox = gf[0], goy = gf[1], goz = gf[2], ginv = gf[3], gbox = gf[4];
+  const bool gper = gi[4] != 0;
+  // periodic: the coordinate wrapped into [0, boxl) (boxl itself, by rounding,
+  // falls in the last cell, adjacent to the first)
+  auto wrap = [&](double v) { return gper ? v - gbox * floor(v / gbox) : v; };
   int* cells = u.cells;
   const int t = (int)threadIdx.x, nt = (int)blockDim.x;
   for (int k = t; k < ncell; k += nt) cells[k] = 0;
   __syncthreads();
   for (int b = t; b < P.n; b += nt) {
     const int g = P.off + b;
-    const int id = (grid_coord(az[g], goz, ginv, nz) * ny + grid_coord(ay[g], goy, ginv, ny)) * nx +
-                   grid_coord(ax[g], gox, ginv, nx);
+    const int id = (grid_coord(wrap(az[g]), goz, ginv, nz) * ny + grid_coord(wrap(ay[g]), goy, ginv, ny)) * nx +
+                   grid_coord(wrap(ax[g]), gox, ginv, nx);
     D.bead_cell[g] = id;
     atomicAdd(&cells[id], 1);
   }
@@ -654,7 +684,6 @@ __global__ void __launch_bounds__(256) k_rebuild_grid(Dev D, const Tables* __res
     const GridP& G = D.grid[c];
     const int id = D.bead_cell[g];
     const int cxi = id % G.nx, cyi = (id / G.nx) % G.ny, czi = id / (G.nx * G.ny);
-    const int x0 = cxi > 0 ? cxi - 1 : 0, x1 = cxi + 1 < G.nx ? cxi + 1 : G.nx - 1;
     const size_t cbase = (size_t)c * kGridCells;
     const int* cb = D.cell_beads + P.off;
     const int off = P.off;
@@ -662,78 +691,109 @@ __global__ void __launch_bounds__(256) k_rebuild_grid(Dev D, const Tables* __res
     const double ox = ux[g], oy = uy[g], oz = uz[g];
     const int gt = D.type[g];
     const bool nlcrit = crit == SOPG_CUTOFF_NEIGHBOR;
-    // the 27 neighbouring cells as up to 9 x-runs, contiguous in the sorted
-    // array, walked as one candidate sequence (LG candidates per chunk, the
-    // next chunk loaded before the current one is tested)
-    int rs[9], pre[10];
-    pre[0] = 0;
+    // The 27 neighbouring cells as x-runs, contiguous in the sorted array,
+    // walked as one candidate sequence (LG candidates per chunk, the next chunk
+    // loaded before the current one is tested): 9 runs on the local grid (rows
+    // outside it empty), and on the periodic grid 18, the two parts of each
+    // row's x-range cx - 1 .. cx + 1 modulo n (the second empty unless it wraps).
+    auto walk = [&](auto nruns) {
+      constexpr int NR = decltype(nruns)::value;
+      constexpr bool PER = NR == 18;
+      int rs[NR], pre[NR + 1];
+      pre[0] = 0;
 #pragma unroll
-    for (int k = 0; k < 9; k++) {
-      const int z = czi + k / 3 - 1, y = cyi + k % 3 - 1;
-      int s0 = 0, len = 0;
-      if (z >= 0 && z < G.nz && y >= 0 && y < G.ny) {
-        const int row = (z * G.ny + y) * G.nx;
-        s0 = D.cell_start[cbase + row + x0];
-        len = D.cell_start[cbase + row + x1] + D.cell_fill[cbase + row + x1] - s0;
+      for (int k = 0; k < 9; k++) {
+        int z = czi + k / 3 - 1, y = cyi + k % 3 - 1;
+        if constexpr (!PER) {
+          const int x0 = cxi > 0 ? cxi - 1 : 0, x1 = cxi + 1 < G.nx ? cxi + 1 : G.nx - 1;
+          int s0 = 0, len = 0;
+          if (z >= 0 && z < G.nz && y >= 0 && y < G.ny) {
+            const int row = (z * G.ny + y) * G.nx;
+            s0 = D.cell_start[cbase + row + x0];
+            len = D.cell_start[cbase + row + x1] + D.cell_fill[cbase + row + x1] - s0;
+          }
+          rs[k] = s0;
+          pre[k + 1] = pre[k] + len;
+        } else {
+          const int n = G.nx;  // nx = ny = nz
+          z += z < 0 ? n : (z >= n ? -n : 0);
+          y += y < 0 ? n : (y >= n ? -n : 0);
+          const int row = (z * n + y) * n;
+          const int a0 = cxi == 0 ? 0 : cxi - 1, a1 = cxi == n - 1 ? n - 1 : cxi + 1;
+          const int bx = cxi == 0 ? n - 1 : 0;  // the wrapped cell, if any
+          const bool wr = cxi == 0 || cxi == n - 1;
+          const int s0 = D.cell_start[cbase + row + a0];
+          rs[2 * k] = s0;
+          pre[2 * k + 1] = pre[2 * k] + D.cell_start[cbase + row + a1] + D.cell_fill[cbase + row + a1] - s0;
+          rs[2 * k + 1] = D.cell_start[cbase + row + bx];
+          pre[2 * k + 2] = pre[2 * k + 1] + (wr ? D.cell_fill[cbase + row + bx] : 0);
+        }
       }
-      rs[k] = s0;
-      pre[k + 1] = pre[k] + len;
-    }
-    const int total = pre[9];
-    // position in the sorted array of candidate t: the last run starting at or before t
-    auto locate = [&](int t) {
-      int at = rs[0] + t;
+      const int total = pre[NR];
+      // position in the sorted array of candidate t: the last run starting at or before t
+      auto locate = [&](int t) {
+        int at = rs[0] + t;
 #pragma unroll
-      for (int k = 1; k < 9; k++) at = (pre[k] <= t) ? rs[k] + (t - pre[k]) : at;
-      return at;
-    };
-    // A candidate-list build gathers the single-precision positions (and
-    // types) of the planning step and keeps a superset of the pairs within
-    // 3.2 sigma + skin: the cutoff is widened by the bound eps2 on the error of
-    // the squared distance (as in the candidate filter), and the filter decides
-    // every kept pair exactly.  A neighbor-list build uses the double positions.
-    const double R = T->cand_reach;
-    const double oxs = (double)(float)ox, oys = (double)(float)oy, ozs = (double)(float)oz;
-    const double ec = 0x1p-24 * (2.0 * fmax(fabs(ox), fmax(fabs(oy), fabs(oz))) + R);
-    const double eps2 = 2.0 * R * 1.7320508075688773 * ec + 3.0 * ec * ec + 1e-9;
-    const double qlim = qbox - 1.0;
-    int ccn = -1;
-    double pxn = 0.0, pyn = 0.0, pzn = 0.0;
-    float4 pfn = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    auto fetch = [&](int t) {
-      ccn = cb[locate(t)];
-      const int gc = off + ccn;
-      if (cmode) pfn = D.fpos[gc];
-      else { pxn = ux[gc]; pyn = uy[gc]; pzn = uz[gc]; }
-    };
-    if (l < total) fetch(l);
-    for (int base = 0; base < total; base += LG) {
-      const int cc = ccn;
-      const double px = pxn, py = pyn, pz = pzn;
-      const float4 pf = pfn;
-      if (base + LG + l < total) fetch(base + LG + l);
-      if (base + l >= total || !(cc - b > 2 || b - cc > 2)) continue;
-      const int gc = off + cc;
-      const int side = gc < g ? 1 : 0;  // 1: the partner is the pair's first bead
-      bool keep;
-      if (cmode) {
-        const int ct = (int)pf.w;
-        const int tpi = side ? ct * 3 + gt : gt * 3 + ct;
-        const double sx = (double)pf.x - oxs, sy = (double)pf.y - oys, sz = (double)pf.z - ozs;
-        // beyond the min-image fast path: kept (the filter decides exactly)
-        keep = !(fabs(sx) <= qlim && fabs(sy) <= qlim && fabs(sz) <= qlim) ||
-               sx * sx + sy * sy + sz * sz < cut2[tpi][2] + eps2;
-      } else {
-        const int ct = D.type[gc];
-        const int tpi = side ? ct * 3 + gt : gt * 3 + ct;
-        double ddx, ddy, ddz;
-        pair_d3(side, ox, oy, oz, px, py, pz, boxl, qbox, ddx, ddy, ddz);
-        const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-        keep = nlcrit ? d2 < cut2[tpi][0] : cell_adjacent(D, P, side ? gc : g, side ? g : gc);
-        if (keep && d2 < cut2[tpi][1]) atomicOr(&plb[cc >> 5], 1u << (cc & 31));
+        for (int k = 1; k < NR; k++) at = (pre[k] <= t) ? rs[k] + (t - pre[k]) : at;
+        return at;
+      };
+      // A candidate-list build gathers the single-precision positions (and
+      // types) of the planning step and keeps a superset of the pairs within
+      // 3.2 sigma + skin: the cutoff is widened by the bound eps2 on the error of
+      // the squared distance (as in the candidate filter), and the filter decides
+      // every kept pair exactly.  A neighbor-list build, and a candidate build on
+      // the periodic grid (whose partners' coordinates are not bounded by the
+      // bead's own plus the reach, as eps2 assumes), use the double positions.
+      constexpr bool FP = !PER;  // single-precision candidate gathers
+      const double R = T->cand_reach;
+      const double oxs = (double)(float)ox, oys = (double)(float)oy, ozs = (double)(float)oz;
+      const double ec = 0x1p-24 * (2.0 * fmax(fabs(ox), fmax(fabs(oy), fabs(oz))) + R);
+      const double eps2 = 2.0 * R * 1.7320508075688773 * ec + 3.0 * ec * ec + 1e-9;
+      const double qlim = qbox - 1.0;
+      int ccn = -1;
+      double pxn = 0.0, pyn = 0.0, pzn = 0.0;
+      float4 pfn = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      auto fetch = [&](int t) {
+        ccn = cb[locate(t)];
+        const int gc = off + ccn;
+        if (FP && cmode) pfn = D.fpos[gc];
+        else { pxn = ux[gc]; pyn = uy[gc]; pzn = uz[gc]; }
+      };
+      if (l < total) fetch(l);
+      for (int base = 0; base < total; base += LG) {
+        const int cc = ccn;
+        const double px = pxn, py = pyn, pz = pzn;
+        const float4 pf = pfn;
+        if (base + LG + l < total) fetch(base + LG + l);
+        if (base + l >= total || !(cc - b > 2 || b - cc > 2)) continue;
+        const int gc = off + cc;
+        const int side = gc < g ? 1 : 0;  // 1: the partner is the pair's first bead
+        bool keep;
+        if (FP && cmode) {
+          const int ct = (int)pf.w;
+          const int tpi = side ? ct * 3 + gt : gt * 3 + ct;
+          const double sx = (double)pf.x - oxs, sy = (double)pf.y - oys, sz = (double)pf.z - ozs;
+          // beyond the min-image fast path: kept (the filter decides exactly)
+          keep = !(fabs(sx) <= qlim && fabs(sy) <= qlim && fabs(sz) <= qlim) ||
+                 sx * sx + sy * sy + sz * sz < cut2[tpi][2] + eps2;
+        } else {
+          const int ct = D.type[gc];
+          const int tpi = side ? ct * 3 + gt : gt * 3 + ct;
+          double ddx, ddy, ddz;
+          pair_d3(side, ox, oy, oz, px, py, pz, boxl, qbox, ddx, ddy, ddz);
+          const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+          if (cmode) {
+            keep = d2 < cut2[tpi][2];  // periodic grid: the exact candidate test
+          } else {
+            keep = nlcrit ? d2 < cut2[tpi][0] : cell_adjacent(D, P, side ? gc : g, side ? g : gc);
+            if (keep && d2 < cut2[tpi][1]) atomicOr(&plb[cc >> 5], 1u << (cc & 31));
+          }
+        }
+        if (keep) atomicOr(&nlb[cc >> 5], 1u << (cc & 31));
       }
-      if (keep) atomicOr(&nlb[cc >> 5], 1u << (cc & 31));
-    }
+    };
+    if (G.periodic) walk(std::integral_constant<int, 18>{});
+    else walk(std::integral_constant<int, 9>{});
   }
   __syncthreads();
   if (active) {

@@ -16,6 +16,7 @@ OK = 0
 CUTOFF_NONE, CUTOFF_NEIGHBOR, CUTOFF_CELL = 0, 1, 2
 NOISE_REPLAY, NOISE_PHILOX = 0, 1
 LIST_NL_ATT, LIST_NL_REP, LIST_PL_ATT, LIST_PL_REP = 0, 1, 2, 3
+PLAN_NONE, PLAN_LOCAL, PLAN_PERIODIC, PLAN_SCAN, PLAN_FILTER = 0, 1, 2, 3, 4
 KERNEL_STEP, KERNEL_REBUILD, KERNEL_PAIRLIST, KERNEL_OBSERVE = 0, 1, 2, 3
 
 _D33 = (C.c_double * 3) * 3
@@ -94,7 +95,7 @@ EXPORTS = [
     "sopg_server_create", "sopg_server_destroy", "sopg_server_last_error", "sopg_server_now",
     "sopg_server_footprint", "sopg_server_submit", "sopg_server_poll", "sopg_server_job",
     "sopg_server_job_records", "sopg_server_job_state", "sopg_server_release", "sopg_server_stats",
-    "sopg_server_evict", "sopg_run_noalloc",
+    "sopg_server_evict", "sopg_run_noalloc", "sopg_rebuild_plan",
 ]
 
 _lib = None
@@ -148,6 +149,7 @@ def lib():
         L.sopg_set_step.argtypes = [vp, C.c_int, C.c_double, C.c_int, C.c_int]
         L.sopg_sync.argtypes = [vp]
         L.sopg_list_counts.argtypes = [vp, C.c_int, _IP, _IP, _IP, _IP]
+        L.sopg_rebuild_plan.argtypes = [vp, C.c_int, _IP, _IP]
         L.sopg_profile.argtypes = [vp, C.c_int]
         L.sopg_profile_read.argtypes = [vp, C.c_int, _DP, C.POINTER(C.c_long)]
         L.sopg_server_create.argtypes = [C.c_int, C.POINTER(ServerConfig), C.POINTER(vp)]
@@ -521,6 +523,12 @@ class Context:
         v = [C.c_int() for _ in range(4)]
         self._chk(self.L.sopg_list_counts(self.h, chain, *[C.byref(x) for x in v]))
         return dict(zip(("nnl_att", "nnl_rep", "nil_att", "nil_rep"), (x.value for x in v)))
+
+    def rebuild_plan(self, chain=0):
+        """(plan, cells per axis) of the chain's latest list rebuild: one of PLAN_*."""
+        p, n = C.c_int(), C.c_int()
+        self._chk(self.L.sopg_rebuild_plan(self.h, chain, C.byref(p), C.byref(n)))
+        return p.value, n.value
 
     def profile(self, enable=True):
         self._chk(self.L.sopg_profile(self.h, 1 if enable else 0))

@@ -333,6 +333,108 @@ def test_binned_rebuild_equals_full_scan(inputs_root, cutoff):
         assert np.array_equal(pa[0], pb[0]) and np.array_equal(pa[1], pb[1])
 
 
+def zigzag_chain(n, direction=(1.0, 0.05, 0.0), step=3.6, wiggle=1.0, origin=(0.0, 0.0, 0.0)):
+    """n beads along a tilted axis, step A apart along it and +-wiggle across: in a box
+    shorter than the chain, consecutive wraps along x lie 6 A apart (direction's y / x x the
+    box), so wrapped segments meet within the list cutoffs only through the min image."""
+    u = np.asarray(direction, float)
+    u /= np.linalg.norm(u)
+    w = np.cross(u, (0.0, 0.0, 1.0))
+    w /= np.linalg.norm(w)
+    k = np.arange(n)[:, None]
+    x = np.asarray(origin, float) + k * step * u + wiggle * np.where(k % 2 == 0, 1.0, -1.0) * w
+    return x.reshape(-1).copy()
+
+
+def _lists(ctx, c):
+    return [ctx.get_list(w, c) for w in (sopg.LIST_NL_ATT, sopg.LIST_NL_REP, sopg.LIST_PL_ATT, sopg.LIST_PL_REP)]
+
+
+@pytest.mark.parametrize("skin", [4.0, 0.0])
+def test_periodic_grid_lists(tmp_path, skin):
+    """Chains wider than the box less a grid edge are binned on a grid over the box (wrapped
+    coordinates, neighbouring cells taken modulo the cell count): the same lists as the full
+    scan, pairs that only the min image brings within the cutoffs included, and the oracle's
+    lists for one of them.  Box 120 A: a 900 A zig-zag, a 1500-bead self-avoiding walk (whose
+    periodic images overlap it), and a native chain 10^4 A from the origin (local grid)."""
+    box = 120.0
+    zz = zigzag_chain(250)
+    saw = sopg.gen_saw(1500, 21)
+    nat = sopg.gen_native(400, 22) + np.tile([1.0e4, -3.0e3, 50.0], 400)
+    chains = [zz, saw, nat]
+    res = []
+    for binned in (True, False):
+        print(f"[gpu] periodic grid lists skin={skin} binned={binned}", flush=True)
+        ctx = sopg.Context(binned=binned)
+        ctx.set_skin(skin)
+        for c, x in enumerate(chains):
+            ctx.add_chain(sopg.chain_topology(x),
+                          sopg.chain_params(boxl=box, ncell=10.0, cutoff=sopg.CUTOFF_NEIGHBOR, nnlup=10, nup=50,
+                                            noise_mode=sopg.NOISE_PHILOX, seed=5 + c))
+        plans = []
+        for c, x in enumerate(chains):
+            ctx.put_state(c, pos=x, unc=x, vel=np.zeros_like(x), force=np.zeros_like(x))
+            ctx.update_neighbor_list(c)
+            plans.append(ctx.rebuild_plan(c))
+            ctx.update_pair_list(c)
+        # cells of at least the grid edge (3.2 sigma_rep, plus the skin for candidates)
+        n = int(box // (3.2 * 3.8 * 1.001 + skin))
+        want = [(sopg.PLAN_PERIODIC, n), (sopg.PLAN_PERIODIC, n), (sopg.PLAN_LOCAL, None)] if binned else \
+            [(sopg.PLAN_SCAN, 0)] * 3
+        for (p, k), (wp, wk) in zip(plans, want):
+            assert p == wp and (wk is None or k == wk), (plans, want)
+        res.append([_lists(ctx, c) for c in range(len(chains))])
+        ctx.close()
+    for la, lb in zip(*res):
+        for (ai, aj), (bi, bj) in zip(la, lb):
+            assert np.array_equal(ai, bi) and np.array_equal(aj, bj)
+    # pairs that only the min image brings together: plain separation beyond half the box
+    zi, zj = res[0][0][1]
+    assert len(zi) > 0
+    X = zz.reshape(-1, 3)
+    plain = np.abs(X[zj - 1] - X[zi - 1]).max(axis=1)
+    assert (plain > box / 2).sum() > 100
+    # the oracle (the reference's algorithm) on the zig-zag alone
+    d = str(tmp_path / "zz")
+    sopg.write_inputs(zz, d)
+    orc = Oracle()
+    setup_system(orc, d, std_cmds(nnlup=10, nup=50, extra=[("boxl", "120"), ("ncell", "10")]))
+    orc.call("update_neighbor_list")
+    orc.call("update_pair_list")
+    for k in range(4):
+        oi, oj = orc.lst(k)
+        di, dj = res[0][0][k]
+        assert np.array_equal(di, oi) and np.array_equal(dj, oj), k
+
+
+def test_periodic_grid_run_equals_full_scan():
+    """A run on the periodic grid (a zig-zag wrapping the 120 A box about seven times, beside a
+    chain on the local grid): lists and trajectories bit-identical to the full scan's."""
+    box = 120.0
+    chains = [zigzag_chain(250), sopg.gen_native(400, 23) + np.tile([-2.0e3, 40.0, 7.0e3], 400)]
+    res = []
+    for binned in (True, False):
+        print(f"[gpu] periodic grid run binned={binned}", flush=True)
+        ctx = sopg.Context(binned=binned)
+        for c, x in enumerate(chains):
+            ctx.add_chain(sopg.chain_topology(x),
+                          sopg.chain_params(boxl=box, ncell=10.0, cutoff=sopg.CUTOFF_NEIGHBOR, nnlup=10, nup=50,
+                                            noise_mode=sopg.NOISE_PHILOX, seed=31 + c))
+        for c, x in enumerate(chains):
+            ctx.put_state(c, pos=x, unc=x, vel=sopg.gen_maxwell(len(x) // 3, 0.6, 40 + c), force=np.zeros_like(x))
+            ctx.update_neighbor_list(c)
+            ctx.update_pair_list(c)
+            ctx.force_eval(c)
+        ctx.run(60)
+        res.append([(ctx.get_state(c), _lists(ctx, c)) for c in range(len(chains))])
+        ctx.close()
+    for (sa, la), (sb, lb) in zip(*res):
+        for k in ("unc", "vel", "force"):
+            assert np.array_equal(sa[k], sb[k]), k
+        for (ai, aj), (bi, bj) in zip(la, lb):
+            assert np.array_equal(ai, bi) and np.array_equal(aj, bj)
+
+
 def test_philox_noise_statistics(inputs_root):
     """Device noise: zero force field replaced by pure noise statistics check."""
     n = 2000
