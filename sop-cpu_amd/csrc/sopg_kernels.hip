@@ -750,21 +750,29 @@ __global__ void __launch_bounds__(256) k_rebuild_grid(Dev D, const Tables* __res
       const double ec = 0x1p-24 * (2.0 * fmax(fabs(ox), fmax(fabs(oy), fabs(oz))) + R);
       const double eps2 = 2.0 * R * 1.7320508075688773 * ec + 3.0 * ec * ec + 1e-9;
       const double qlim = qbox - 1.0;
-      int ccn = -1;
+      // software pipeline: a chunk's candidate bead indices are loaded two
+      // chunks ahead and their positions one chunk ahead, so neither load of
+      // the dependent pair (index, then position) waits in the pass that issues it
+      int ccn = -1, ccn2 = -1;
       double pxn = 0.0, pyn = 0.0, pzn = 0.0;
       float4 pfn = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      auto fetch = [&](int t) {
-        ccn = cb[locate(t)];
-        const int gc = off + ccn;
+      auto fetch_pos = [&](int cc) {
+        const int gc = off + cc;
         if (FP && cmode) pfn = D.fpos[gc];
         else { pxn = ux[gc]; pyn = uy[gc]; pzn = uz[gc]; }
       };
-      if (l < total) fetch(l);
+      if (l < total) {
+        ccn = cb[locate(l)];
+        fetch_pos(ccn);
+      }
+      if (l + LG < total) ccn2 = cb[locate(l + LG)];
       for (int base = 0; base < total; base += LG) {
         const int cc = ccn;
         const double px = pxn, py = pyn, pz = pzn;
         const float4 pf = pfn;
-        if (base + LG + l < total) fetch(base + LG + l);
+        if (base + LG + l < total) fetch_pos(ccn2);
+        ccn = ccn2;
+        if (base + 2 * LG + l < total) ccn2 = cb[locate(base + 2 * LG + l)];
         if (base + l >= total || !(cc - b > 2 || b - cc > 2)) continue;
         const int gc = off + cc;
         const int side = gc < g ? 1 : 0;  // 1: the partner is the pair's first bead
