@@ -513,14 +513,22 @@ __global__ void __launch_bounds__(kGridThreads) k_grid(Dev D, const Tables* __re
     // min image.
     const double gedge = cell ? 1.0 : (cand ? P.gcand : P.gcell);
     const double limit = cell ? P.ncell - 2.0 : P.boxl - gedge;
-    double inv = 1.0 / gedge;
-    int dims[3];
-    bool use = true, fin = true;
+    // cells of the grid edge, or wider where the extent would need more than
+    // kGridDim of them (cubic cells of at least the edge keep the adjacency)
+    double ext[3], emax = 0.0;
+    bool fin = true;
     for (int a = 0; a < 3; a++) {
-      const double ext = sh[3 + a][0] - sh[a][0];
-      fin = fin && isfinite(ext);
-      use = use && ext <= limit && isfinite(ext);
-      dims[a] = use ? (int)floor(ext * inv) + 1 : 0;
+      ext[a] = sh[3 + a][0] - sh[a][0];
+      fin = fin && isfinite(ext[a]);
+      emax = fmax(emax, ext[a]);
+    }
+    const double edge = fmax(gedge, emax / (kGridDim - 1));
+    double inv = 1.0 / edge;
+    int dims[3];
+    bool use = fin;
+    for (int a = 0; a < 3; a++) {
+      use = use && ext[a] <= limit;
+      dims[a] = use ? (int)floor(ext[a] * inv) + 1 : 0;
       use = use && dims[a] <= kGridDim;
     }
     G.ox = sh[0][0]; G.oy = sh[1][0]; G.oz = sh[2][0];

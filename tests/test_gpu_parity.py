@@ -407,6 +407,30 @@ def test_periodic_grid_lists(tmp_path, skin):
         assert np.array_equal(di, oi) and np.array_equal(dj, oj), k
 
 
+def test_wide_local_grid_lists():
+    """A chain that needs more than 32 grid edges per axis but fits the box (a 900 A zig-zag in
+    a 2000 A box) is binned on a local grid of wider cubic cells: the full scan's lists."""
+    zz = zigzag_chain(250)
+    res = []
+    for binned in (True, False):
+        print(f"[gpu] wide local grid binned={binned}", flush=True)
+        ctx = sopg.Context(binned=binned)
+        ctx.add_chain(sopg.chain_topology(zz),
+                      sopg.chain_params(boxl=2000.0, ncell=100.0, cutoff=sopg.CUTOFF_NEIGHBOR, nnlup=10, nup=50,
+                                        noise_mode=sopg.NOISE_PHILOX, seed=3))
+        ctx.put_state(0, pos=zz, unc=zz, vel=np.zeros_like(zz), force=np.zeros_like(zz))
+        ctx.update_neighbor_list(0)
+        plan = ctx.rebuild_plan(0)
+        # 895 A along x over at most 32 cells (the 16.2 A edge alone would need 56)
+        assert (plan[0] == sopg.PLAN_LOCAL and 30 <= plan[1] <= 32) if binned else plan == (sopg.PLAN_SCAN, 0), plan
+        ctx.update_pair_list(0)
+        res.append(_lists(ctx, 0))
+        ctx.close()
+    for (ai, aj), (bi, bj) in zip(*res):
+        assert np.array_equal(ai, bi) and np.array_equal(aj, bj)
+    assert len(res[0][1][0]) > 0
+
+
 def test_periodic_grid_run_equals_full_scan():
     """A run on the periodic grid (a zig-zag wrapping the 120 A box about seven times, beside a
     chain on the local grid): lists and trajectories bit-identical to the full scan's."""
