@@ -239,10 +239,11 @@ int sopg_list_counts(sopg_ctx* ctx, int chain, int* nnl_att, int* nnl_rep, int* 
  *   SOPG_PLAN_NONE      no rebuild was due for the chain in it
  *   SOPG_PLAN_LOCAL     binned on a grid over the chain's extent
  *   SOPG_PLAN_PERIODIC  binned on a grid over the box (a chain wider than the
- *                       box less one grid edge)
- *   SOPG_PLAN_SCAN      the full scan (a cell-list chain too wide for the local
- *                       grid, a box under three grid edges, a chain beyond the
- *                       binned rebuild's bead limit, non-finite positions)
+ *                       box less one grid edge; for the cell list, than ncell
+ *                       less two cells)
+ *   SOPG_PLAN_SCAN      the full scan (a box under three grid edges, a chain
+ *                       beyond the binned rebuild's bead limit, non-finite
+ *                       positions, an explicit master list)
  *   SOPG_PLAN_FILTER    the neighbor list filtered from the chain's candidate
  *                       list, which was still valid
  * cells: cells per axis of the grid (0 without one). */

@@ -396,10 +396,8 @@ __global__ void __launch_bounds__(256) k_rebuild(Dev D, const Tables* __restrict
 // neighbouring cells, marks accepted partners in an LDS bitmap (one bit per
 // chain bead) and emits the bitmap in ascending order -- the reference's
 // lexicographic order -- with a wave prefix sum.  The grid covers the chain's
-// extent, or, for a distance-criterion chain wider than the box less one grid
-// edge, the box itself with wrapped coordinates (neighbouring cells modulo the
-// cell count).  A cell-list chain too wide for the local grid keeps the full
-// scan of k_rebuild.
+// extent, or, for a chain wider than the period less one grid edge, the period
+// itself with wrapped coordinates (neighbouring cells modulo the cell count).
 
 __device__ __forceinline__ bool rebuild_due(const ChainP& P, int rel, int mode, int& crit) {
   crit = mode;
@@ -532,36 +530,38 @@ __global__ void __launch_bounds__(kGridThreads) k_grid(Dev D, const Tables* __re
       use = use && dims[a] <= kGridDim;
     }
     G.ox = sh[0][0]; G.oy = sh[1][0]; G.oz = sh[2][0];
-    // A distance-criterion chain too wide for that grid is binned periodically:
-    // coordinates wrapped into [0, boxl), n cells of edge boxl / n >= the grid
-    // edge per axis (3 <= n <= 32), neighbouring cells taken modulo n.  A pair
-    // within the cutoff (under the reference's min image) has wrapped
-    // coordinates that differ by less than one cell edge modulo boxl, so its
-    // beads lie in cells adjacent modulo n (n >= 3 keeps the 27 cells distinct).
+    // A chain too wide for that grid is binned periodically: coordinates
+    // wrapped into [0, period) (boxl; ncell for the cell list, whose
+    // coordinates are the reference's cell indices), n cells of edge period / n
+    // >= the grid edge per axis (3 <= n <= 32), neighbouring cells taken modulo
+    // n.  A pair within the cutoff (under the reference's min image; for the
+    // cell list, cells adjacent modulo ncell) has wrapped coordinates that
+    // differ by less than one cell edge modulo the period, so its beads lie in
+    // cells adjacent modulo n (n >= 3 keeps the 27 cells distinct).
+    const double period = cell ? P.ncell : P.boxl;
     int per = 0;
-    if (!use && !cell && fin && P.boxl > 0.0) {
-      int n = (int)fmin(floor(P.boxl / gedge), (double)kGridDim);
-      while (n >= 3 && P.boxl / n < gedge) n--;
+    if (!use && fin && period > 0.0) {
+      int n = (int)fmin(floor(period / gedge), (double)kGridDim);
+      while (n >= 3 && period / n < gedge) n--;
       if (n >= 3) {
         per = 1;
         use = true;
         dims[0] = dims[1] = dims[2] = n;
-        inv = (double)n / P.boxl;
+        inv = (double)n / period;
         G.ox = G.oy = G.oz = 0.0;
       }
     }
     G.inv = inv;
     G.nx = dims[0]; G.ny = dims[1]; G.nz = dims[2];
     G.periodic = per;
-    // a chain that neither grid takes (a cell-list chain too wide for the local
-    // grid, a box under three grid edges, non-finite positions) builds its lists
-    // by the full scan
+    // a chain that neither grid takes (a period under three grid edges,
+    // non-finite positions) builds its lists by the full scan
     G.work = (use ? W_BIN : W_SCAN) | (cand ? (W_CAND | W_FILTER) : 0);
     if (G.work & W_BIN) D.plan[0] = rel;
     if (G.work & W_FILTER) D.plan[1] = rel;
     if (G.work & W_SCAN) D.plan[2] = rel;
     if (cand) G.cand_valid = 1;
-    gf[0] = G.ox; gf[1] = G.oy; gf[2] = G.oz; gf[3] = inv; gf[4] = P.boxl;
+    gf[0] = G.ox; gf[1] = G.oy; gf[2] = G.oz; gf[3] = inv; gf[4] = period;
     gi[0] = dims[0]; gi[1] = dims[1]; gi[2] = dims[2]; gi[3] = G.work; gi[4] = per;
   }
   __syncthreads();
@@ -573,8 +573,8 @@ __global__ void __launch_bounds__(kGridThreads) k_grid(Dev D, const Tables* __re
   const int nx = gi[0], ny = gi[1], nz = gi[2], ncell = nx * ny * nz;
   const double gox = gf[0], goy = gf[1], goz = gf[2], ginv = gf[3], gbox = gf[4];
   const bool gper = gi[4] != 0;
-  // periodic: the coordinate wrapped into [0, boxl) (boxl itself, by rounding,
-  // falls in the last cell, adjacent to the first)
+  // periodic: the coordinate wrapped into [0, period) (the period itself, by
+  // rounding, falls in the last cell, adjacent to the first)
   auto wrap = [&](double v) { return gper ? v - gbox * floor(v / gbox) : v; };
   int* cells = u.cells;
   const int t = (int)threadIdx.x, nt = (int)blockDim.x;

@@ -350,35 +350,38 @@ def _lists(ctx, c):
     return [ctx.get_list(w, c) for w in (sopg.LIST_NL_ATT, sopg.LIST_NL_REP, sopg.LIST_PL_ATT, sopg.LIST_PL_REP)]
 
 
-@pytest.mark.parametrize("skin", [4.0, 0.0])
-def test_periodic_grid_lists(tmp_path, skin):
-    """Chains wider than the box less a grid edge are binned on a grid over the box (wrapped
-    coordinates, neighbouring cells taken modulo the cell count): the same lists as the full
-    scan, pairs that only the min image brings within the cutoffs included, and the oracle's
-    lists for one of them.  Box 120 A: a 900 A zig-zag, a 1500-bead self-avoiding walk (whose
-    periodic images overlap it), and a native chain 10^4 A from the origin (local grid)."""
+@pytest.mark.parametrize("crit,skin", [("nl", 4.0), ("nl", 0.0), ("cl", 0.0)])
+def test_periodic_grid_lists(tmp_path, crit, skin):
+    """Chains wider than the period less a grid edge are binned on a grid over the period
+    (wrapped coordinates, neighbouring cells taken modulo the cell count): the same lists as the
+    full scan, pairs that only the min image brings within the cutoffs (or the periodic cell
+    adjacency) included, and the oracle's lists for one of them.  Box 120 A with 12 A cells: a
+    900 A zig-zag, a 1500-bead self-avoiding walk (whose periodic images overlap it), and a
+    native chain 10^4 A from the origin (local grid)."""
     box = 120.0
+    cutoff = sopg.CUTOFF_NEIGHBOR if crit == "nl" else sopg.CUTOFF_CELL
     zz = zigzag_chain(250)
     saw = sopg.gen_saw(1500, 21)
     nat = sopg.gen_native(400, 22) + np.tile([1.0e4, -3.0e3, 50.0], 400)
     chains = [zz, saw, nat]
     res = []
     for binned in (True, False):
-        print(f"[gpu] periodic grid lists skin={skin} binned={binned}", flush=True)
+        print(f"[gpu] periodic grid lists {crit} skin={skin} binned={binned}", flush=True)
         ctx = sopg.Context(binned=binned)
         ctx.set_skin(skin)
         for c, x in enumerate(chains):
             ctx.add_chain(sopg.chain_topology(x),
-                          sopg.chain_params(boxl=box, ncell=10.0, cutoff=sopg.CUTOFF_NEIGHBOR, nnlup=10, nup=50,
+                          sopg.chain_params(boxl=box, ncell=10.0, cutoff=cutoff, nnlup=10, nup=50,
                                             noise_mode=sopg.NOISE_PHILOX, seed=5 + c))
         plans = []
         for c, x in enumerate(chains):
             ctx.put_state(c, pos=x, unc=x, vel=np.zeros_like(x), force=np.zeros_like(x))
-            ctx.update_neighbor_list(c)
+            (ctx.update_neighbor_list if crit == "nl" else ctx.update_cell_list)(c)
             plans.append(ctx.rebuild_plan(c))
             ctx.update_pair_list(c)
-        # cells of at least the grid edge (3.2 sigma_rep, plus the skin for candidates)
-        n = int(box // (3.2 * 3.8 * 1.001 + skin))
+        # cells of at least the grid edge (3.2 sigma_rep, plus the skin for candidates; one
+        # reference cell for the cell list)
+        n = int(box // (3.2 * 3.8 * 1.001 + skin)) if crit == "nl" else 10
         want = [(sopg.PLAN_PERIODIC, n), (sopg.PLAN_PERIODIC, n), (sopg.PLAN_LOCAL, None)] if binned else \
             [(sopg.PLAN_SCAN, 0)] * 3
         for (p, k), (wp, wk) in zip(plans, want):
@@ -398,8 +401,9 @@ def test_periodic_grid_lists(tmp_path, skin):
     d = str(tmp_path / "zz")
     sopg.write_inputs(zz, d)
     orc = Oracle()
-    setup_system(orc, d, std_cmds(nnlup=10, nup=50, extra=[("boxl", "120"), ("ncell", "10")]))
-    orc.call("update_neighbor_list")
+    setup_system(orc, d, std_cmds(cutoff="neighborlist" if crit == "nl" else "celllist", nnlup=10, nup=50,
+                                  extra=[("boxl", "120"), ("ncell", "10")]))
+    orc.call("update_neighbor_list" if crit == "nl" else "update_cell_list")
     orc.call("update_pair_list")
     for k in range(4):
         oi, oj = orc.lst(k)
