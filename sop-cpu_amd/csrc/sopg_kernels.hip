@@ -1704,7 +1704,7 @@ void launch_cellcoords(const Dev& D, int cur, int g0, int g1, hipStream_t s) {
 // chains whose grid applies (binned != 0), the full scan for the others.
 constexpr int kGridLanes = 16;                       // lanes per bead in the binned rebuild
 constexpr int kGridBeads = 256 / kGridLanes;         // beads per block
-constexpr int kFilterLanes = 16;                     // lanes per bead in the candidate filter
+constexpr int kFilterLanes = 8;                     // lanes per bead in the candidate filter
 constexpr int kMaxBitmapWords = 512;                 // 16 beads x 2 bitmaps x 2 KiB = 64 KiB of LDS
 void launch_rebuild(int L, const Dev& D, const Tables* T, int cur, int par, int rel, int mode, int g0, int g1,
                     int c0, int nc, int binned, int cand, hipStream_t s) {
