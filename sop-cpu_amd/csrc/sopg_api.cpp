@@ -516,8 +516,13 @@ int upload_chainp(sopg_ctx* c, int only = -1, bool may_alloc = true) {
     if (c->cp_pin) HIP_OK(c, hipHostFree(c->cp_pin));
     c->cp_pin = nullptr;
     c->cp_pin_cap = 0;
-    HIP_OK(c, hipHostMalloc((void**)&c->cp_pin, 2 * cap * sizeof(ChainP), hipHostMallocDefault));
-    c->cp_pin_cap = 2 * cap;
+    // without pinned memory the records go through the waited pageable copy below
+    if (hipHostMalloc((void**)&c->cp_pin, 2 * cap * sizeof(ChainP), hipHostMallocDefault) == hipSuccess) {
+      c->cp_pin_cap = 2 * cap;
+    } else {
+      c->cp_pin = nullptr;
+      (void)hipGetLastError();  // not an error of a later launch
+    }
   }
   c->cp_dev_gen = -1;
   if (cap > c->cp_pin_cap) return upload_at(c, c->D.cp, 0, c->cp);
