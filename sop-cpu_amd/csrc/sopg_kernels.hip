@@ -1463,13 +1463,59 @@ __global__ void __launch_bounds__(256, 5) k_force(Dev D, const Tables* __restric
 
 // Observables (sop.cpp:200-280) and list counts, per bead.
 enum { OBS_PHYS = 1, OBS_COUNTS = 2 };
-__device__ __forceinline__ void observe_bead(const Dev& D, int cur, int par, int rel, int all, int what, int g) {
+// Rg^2 pair sum (sop.cpp:245-258) of bead b over its partners c > b, in
+// kObsLanes strided parts (part l: c = b + 1 + l, b + 1 + l + kObsLanes, ...)
+// combined by a fixed tree (part l adds part l + 8, then l + 4, l + 2, l + 1):
+// one lane per part in k_observe, all parts in one thread in the resident
+// kernel, the same bits either way.
+constexpr int kObsLanes = 16;
+__device__ __forceinline__ double rg_part(const double* ux, const double* uy, const double* uz, int off, int n,
+                                          int b, int l, double boxl, double qbox) {
+  const int g = off + b;
+  const double x = ux[g], y = uy[g], z = uz[g];
+  double rg = 0.0;
+  for (int c = b + 1 + l; c < n; c += kObsLanes) {
+    const int h = off + c;
+    const double dx = mimg(x - ux[h], boxl, qbox);
+    const double dy = mimg(y - uy[h], boxl, qbox);
+    const double dz = mimg(z - uz[h], boxl, qbox);
+    rg += (dx * dx + dy * dy + dz * dz);
+  }
+  return rg;
+}
+
+// Observables of bead g.  LO = kObsLanes: the bead's group of lanes shares the
+// Rg^2 pairs (lane l its part l) and lane 0 does the rest; LO = 1: one thread.
+template <int LO>
+__device__ __forceinline__ void observe_bead(const Dev& D, int cur, int par, int rel, int all, int what, int g,
+                                             int l) {
   const ChainP& P = D.cp[D.chain[g]];
   if (!all && (P.iup0 + rel) % P.nup != 0) return;
   const double* ux = cur ? D.ux[1] : D.ux[0];
   const double* uy = cur ? D.uy[1] : D.uy[0];
   const double* uz = cur ? D.uz[1] : D.uz[0];
   const double boxl = P.boxl, qbox = P.qbox;
+  double rg = 0.0;
+  if (what & OBS_PHYS) {
+    const int b = g - P.off;
+    if constexpr (LO == 1) {
+      double s[kObsLanes];
+      for (int k = 0; k < kObsLanes; k++) s[k] = rg_part(ux, uy, uz, P.off, P.n, b, k, boxl, qbox);
+      for (int w = kObsLanes / 2; w > 0; w >>= 1)
+        for (int k = 0; k < w; k++) s[k] += s[k + w];
+      rg = s[0];
+    } else {
+      static_assert(LO == kObsLanes, "one lane per part");
+      double s = rg_part(ux, uy, uz, P.off, P.n, b, l, boxl, qbox);
+#pragma unroll
+      for (int w = kObsLanes / 2; w > 0; w >>= 1) {
+        const double o = __shfl_down(s, w, kObsLanes);
+        if (l < w) s += o;
+      }
+      rg = s;
+    }
+  }
+  if (l != 0) return;
   double* r = D.red + (size_t)g * kRed;
   int cont = 0, nla = 0, pla = 0, nlr = 0, plr = 0;
   const int a0 = D.att_beg[g], a1 = D.att_end[g];
@@ -1499,16 +1545,6 @@ __device__ __forceinline__ void observe_bead(const Dev& D, int cur, int par, int
   }
   if (what & OBS_COUNTS) { r[R_NLA] = nla; r[R_PLA] = pla; r[R_NLR] = nlr; r[R_PLR] = plr; }
   if (!(what & OBS_PHYS)) return;
-  // Rg^2 pair sum over j > i (sop.cpp:245-258)
-  double rg = 0.0;
-  const int b = g - P.off;
-  for (int c = b + 1; c < P.n; c++) {
-    const int h = P.off + c;
-    const double dx = mimg(ux[g] - ux[h], boxl, qbox);
-    const double dy = mimg(uy[g] - uy[h], boxl, qbox);
-    const double dz = mimg(uz[g] - uz[h], boxl, qbox);
-    rg += (dx * dx + dy * dy + dz * dz);
-  }
   // kinT source: velocities (underdamped, sop.cpp:263-269) or position
   // increments (overdamped, sop.cpp:271-278; scaled by zeta/(2h) on the host)
   const bool od = P.sim_type == 2;
@@ -1522,12 +1558,14 @@ __device__ __forceinline__ void observe_bead(const Dev& D, int cur, int par, int
   r[R_CONT] = cont; r[R_RG] = rg; r[R_V2] = v2;
 }
 
+// kObsLanes lanes per bead (the Rg^2 pairs split between them).
 __global__ void k_observe(Dev D, int cur, int par, int rel, int all, int what, int g0, int g1) {
   rel += *D.relb;
-  const int g = g0 + blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= g1 || D.chain[g] < 0) return;  // past the end, or a free bead
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = g0 + t / kObsLanes, l = t & (kObsLanes - 1);
+  if (g >= g1 || D.chain[g] < 0) return;  // past the end, or a free bead: the whole group
   if (!all && D.cp[D.chain[g]].resident) return;
-  observe_bead(D, cur, par, rel, all, what, g);
+  observe_bead<kObsLanes>(D, cur, par, rel, all, what, g, l);
 }
 
 // Sums of chain c's per-bead fields into out[kRed], in a fixed order: 256
@@ -1610,7 +1648,7 @@ __global__ void __launch_bounds__(512) k_resident(Dev D, const Tables* __restric
     }
     __syncthreads();
     if (rec_slot[s] >= 0 && (P.iup0 + s) % P.nup == 0) {
-      for (int b = threadIdx.x; b < n; b += blockDim.x) observe_bead(D, cur, par, s, 0, OBS_PHYS, off + b);
+      for (int b = threadIdx.x; b < n; b += blockDim.x) observe_bead<1>(D, cur, par, s, 0, OBS_PHYS, off + b, 0);
       __syncthreads();
       chain_reduce_block(D, c, rec + ((size_t)rec_slot[s] * nc + c) * kRed, sh);
     }
@@ -1741,7 +1779,8 @@ void launch_posupdate(const Dev& D, int src, int dst, int g0, int g1, hipStream_
   hipLaunchKernelGGL(k_posupdate, dim3(blocks_for(g1 - g0)), dim3(256), 0, s, D, src, dst, g0, g1);
 }
 void launch_observe(const Dev& D, int cur, int par, int rel, int all, int what, int g0, int g1, hipStream_t s) {
-  hipLaunchKernelGGL(k_observe, dim3(blocks_for(g1 - g0)), dim3(256), 0, s, D, cur, par, rel, all, what, g0, g1);
+  hipLaunchKernelGGL(k_observe, dim3(blocks_for((long)(g1 - g0) * kObsLanes)), dim3(256), 0, s, D, cur, par, rel, all,
+                     what, g0, g1);
 }
 void launch_resident(int L, const Dev& D, const Tables* T, const int* chains, int nchains, int threads, int cur,
                      int par, int nsteps, const int* rec_slot, double* rec, int nc, hipStream_t s) {
