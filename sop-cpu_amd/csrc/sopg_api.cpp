@@ -1754,20 +1754,34 @@ static int run_impl(sopg_ctx* c, long nsteps, const double* noise, sopg_record* 
   // the other parity, and nothing has changed since: start from it
   const bool next_pl = stepped && c->next_gen == c->gen;
   if (next_pl) c->par ^= 1;
-  auto due = [&](long r) {
+  // The chains whose list rebuild (rec: record) falls on run step r, as the
+  // bead and chain-slot ranges that cover them: a step's rebuild and record
+  // launches cover only these, so chains on other rebuild phases (jobs that
+  // joined at different steps) cost no blocks there.
+  struct Span {
     bool any = false, cell = false;
-    for (auto& h : c->chains)
-      if (h.active && !h.resident && h.p.cutoff != SOPG_CUTOFF_NONE && (h.inlup + r) % h.p.nnlup == 0) {
-        any = true;
-        cell |= h.p.cutoff == SOPG_CUTOFF_CELL;
-      }
-    return std::make_pair(any, cell);
+    int g0 = 0, g1 = 0, c0 = 0, c1 = 0;
   };
-  auto records = [&](long r) {
-    for (auto& h : c->chains)
-      if (h.active && !h.resident && (h.iup + r) % h.p.nup == 0) return true;
-    return false;
+  auto span_of = [&](long r, bool rec) {
+    Span sp;
+    for (int id = 0; id < (int)c->chains.size(); id++) {
+      const HostChain& h = c->chains[id];
+      if (!h.active || h.resident) continue;
+      const bool hit = rec ? (h.iup + r) % h.p.nup == 0
+                           : h.p.cutoff != SOPG_CUTOFF_NONE && (h.inlup + r) % h.p.nnlup == 0;
+      if (!hit) continue;
+      const int b0 = (int)h.at[P_BEAD], b1 = b0 + h.n;
+      sp.g0 = sp.any ? std::min(sp.g0, b0) : b0;
+      sp.g1 = sp.any ? std::max(sp.g1, b1) : b1;
+      sp.c0 = sp.any ? std::min(sp.c0, id) : id;
+      sp.c1 = sp.any ? std::max(sp.c1, id + 1) : id + 1;
+      sp.any = true;
+      sp.cell |= h.p.cutoff == SOPG_CUTOFF_CELL;
+    }
+    return sp;
   };
+  auto due = [&](long r) { return span_of(r, false); };
+  auto records = [&](long r) { return span_of(r, true).any; };
   // record buffer rows: one per run step at which any chain records
   std::vector<long> rec_steps;
   std::vector<int> row_of(nsteps, -1);
@@ -1837,10 +1851,11 @@ static int run_impl(sopg_ctx* c, long nsteps, const double* noise, sopg_record* 
       c->cur ^= 1;
       c->par ^= 1;
     }
-  } else if (dr.first) {
+  } else if (dr.any) {
     pe = prof_begin(c);
-    if (dr.second) launch_cellcoords(D, c->cur, 0, nb, s);
-    launch_rebuild(c->list_lanes, D, c->d_T, c->cur, c->par, 0, -1, 0, nb, 0, nc, c->binned, c->skin > 0.0, s);
+    if (dr.cell) launch_cellcoords(D, c->cur, dr.g0, dr.g1, s);
+    launch_rebuild(c->list_lanes, D, c->d_T, c->cur, c->par, 0, -1, dr.g0, dr.g1, dr.c0, dr.c1 - dr.c0, c->binned,
+                   c->skin > 0.0, s);
     prof_end(c, SOPG_KERNEL_REBUILD, pe);
   }
   // pair lists at the current positions; rerunning the filter on a chain whose
@@ -1874,20 +1889,23 @@ static int run_impl(sopg_ctx* c, long nsteps, const double* noise, sopg_record* 
     // position half are where the next run starts
     launch_force(L, Dx, c->d_T, c->cur, c->par, k, F_STORE | F_VEL | fe | F_ALLTERMS | F_ADVANCE, true, nz, 0, nb, s);
     prof_end(c, SOPG_KERNEL_STEP, pe);
-    if (records(r)) {
+    const Span rs = span_of(r, true);
+    if (rs.any) {
+      // chain slot c's sums land in row row_of[r], column c
       pe = prof_begin(c);
-      launch_observe(Dx, c->cur, c->par, k, 0, OBS_PHYS, 0, nb, s);
-      launch_chain_reduce(Dx, 0, nc, (fixed ? rec_buf : c->d_rec) + (size_t)row_of[r] * nc * kRed, s);
+      launch_observe(Dx, c->cur, c->par, k, 0, OBS_PHYS, rs.g0, rs.g1, s);
+      launch_chain_reduce(Dx, rs.c0, rs.c1 - rs.c0,
+                          (fixed ? rec_buf : c->d_rec) + ((size_t)row_of[r] * nc + (size_t)rs.c0) * kRed, s);
       prof_end(c, SOPG_KERNEL_OBSERVE, pe);
     }
     if (!last) {
       c->par ^= 1;
-      const auto d = due(r + 1);
-      if (d.first) {
+      const Span d = due(r + 1);
+      if (d.any) {
         pe = prof_begin(c);
-        if (d.second) launch_cellcoords(Dx, c->cur, 0, nb, s);
-        launch_rebuild(c->list_lanes, Dx, c->d_T, c->cur, c->par, k + 1, -1, 0, nb, 0, nc, c->binned, c->skin > 0.0,
-                       s);
+        if (d.cell) launch_cellcoords(Dx, c->cur, d.g0, d.g1, s);
+        launch_rebuild(c->list_lanes, Dx, c->d_T, c->cur, c->par, k + 1, -1, d.g0, d.g1, d.c0, d.c1 - d.c0, c->binned,
+                       c->skin > 0.0, s);
         prof_end(c, SOPG_KERNEL_REBUILD, pe);
       }
       c->cur ^= 1;
@@ -1926,9 +1944,9 @@ static int run_impl(sopg_ctx* c, long nsteps, const double* noise, sopg_record* 
     const int hdr[7] = {(int)S, c->cur, c->par, L, c->list_lanes, c->binned, c->skin > 0.0 ? 1 : 0};
     add(hdr, sizeof hdr);
     for (long k = 0; k < S; k++) {
-      const auto d = due(r + k + 1);
-      const char f = (char)((d.first ? 1 : 0) | (d.second ? 2 : 0));
-      add(&f, 1);
+      const Span d = due(r + k + 1);
+      const int f[5] = {(d.any ? 1 : 0) | (d.cell ? 2 : 0), d.g0, d.g1, d.c0, d.c1};
+      add(f, sizeof f);
     }
     add(&Dg, sizeof Dg);
     add(&c->d_T, sizeof c->d_T);
