@@ -29,8 +29,10 @@ void launch_cellcoords(const Dev& D, int cur, int g0, int g1, hipStream_t s);
 void launch_rebuild(int L, const Dev& D, const Tables* T, int cur, int par, int rel, int mode, int g0, int g1,
                     int c0, int nc, int binned, int cand, hipStream_t s);
 void launch_pairfilter(int L, const Dev& D, const Tables* T, int cur, int par, int g0, int g1, hipStream_t s);
+// blk: nblk block numbers of 256 / L beads each (blocks of [0, g1)), or null
+// for every block of [g0, g1)
 void launch_force(int L, const Dev& D, const Tables* T, int cur, int pr, int rel, int flags, bool next,
-                  const double* noise, int g0, int g1, hipStream_t s);
+                  const double* noise, int g0, int g1, hipStream_t s, const int* blk = nullptr, int nblk = 0);
 void launch_posupdate(const Dev& D, int src, int dst, int g0, int g1, hipStream_t s);
 void launch_observe(const Dev& D, int cur, int par, int rel, int all, int what, int g0, int g1, hipStream_t s);
 void launch_chain_reduce(const Dev& D, int c0, int nc, double* out, hipStream_t s);
@@ -67,6 +69,8 @@ struct Pool {
     if (n == 0) return;
     fr.push_back({at, at + n});
     merge();
+    // a free range that reaches the end lowers the high-water mark
+    if (!fr.empty() && fr.back().second == cap) hw = std::min(hw, fr.back().first);
   }
   void extend(size_t ncap) {
     if (ncap > cap) fr.push_back({cap, ncap});
@@ -191,6 +195,10 @@ struct sopg_ctx {
   size_t rlist_cap = 0;
   int* d_rslot = nullptr;
   size_t rslot_cap = 0;
+  // block table of the step kernel (step_blocks): what d_blk holds
+  int* d_blk = nullptr;
+  size_t blk_cap = 0;
+  std::vector<int> blk_dev;
   // kernel timing (sopg_profile)
   bool prof = false;
   std::vector<hipEvent_t> ev_pool;
@@ -935,6 +943,47 @@ int drop_graphs(sopg_ctx* c) {
   return SOPG_OK;
 }
 
+// The step kernel's blocks of 256 / L beads (block k: beads [k, k + 1) * 256 / L)
+// that hold a bead of an active chain (not a resident one, unless
+// with_resident).  Chains that left free beads behind (a retired job) leave
+// whole blocks of [0, nb) empty; the step launch then covers the table's
+// blocks only.  *blk = null (every block) when no block is empty.
+int step_blocks(sopg_ctx* c, int L, bool with_resident, const int** blk, int* nblk) {
+  const int per = 256 / L, total = (c->D.nb + per - 1) / per;
+  *blk = nullptr;
+  *nblk = total;
+  std::vector<char> hit((size_t)total, 0);
+  for (auto& h : c->chains)
+    if (h.active && h.on_device && h.n > 0 && (with_resident || !h.resident)) {
+      const int b0 = (int)h.at[P_BEAD];
+      for (int k = b0 / per; k <= (b0 + h.n - 1) / per; k++) hit[(size_t)k] = 1;
+    }
+  std::vector<int> t;
+  for (int k = 0; k < total; k++)
+    if (hit[(size_t)k]) t.push_back(k);
+  if (t.empty() || (int)t.size() == total) return SOPG_OK;
+  int rc;
+  if (t != c->blk_dev) {
+    if (t.size() > c->blk_cap) {
+      // cached graphs hold the old table's address
+      if ((rc = drop_graphs(c))) return rc;
+      HIP_OK(c, hipStreamSynchronize(c->stream));
+      dfree(c, c->d_blk);
+      c->d_blk = nullptr;
+      c->blk_cap = 0;
+      c->blk_dev.clear();
+      const size_t cap = std::max<size_t>(t.size(), (size_t)total);
+      if ((rc = dalloc(c, &c->d_blk, cap))) return rc;
+      c->blk_cap = cap;
+    }
+    if ((rc = upload_at(c, c->d_blk, 0, t))) return rc;
+    c->blk_dev = t;
+  }
+  *blk = c->d_blk;
+  *nblk = (int)t.size();
+  return SOPG_OK;
+}
+
 // Candidate lists built with other cutoffs or another skin are rebuilt at
 // each chain's next neighbor-list rebuild.
 int invalidate_candidates(sopg_ctx* c, int only = -1) {
@@ -1673,10 +1722,13 @@ int sopg_autotune(sopg_ctx* c, int* step_lanes) {
   // F_STORE nothing of the state is written but the pair-list bits of the
   // other parity, which the next step of a run rewrites before any reads.
   for (int L : {1, 2, 4, 8, 16, 32, 64}) {
-    launch_force(L, c->D, c->d_T, c->cur, c->par, 0, F_ALLTERMS, true, nullptr, 0, nb, c->stream);
+    const int* blk = nullptr;
+    int nblk = 0;
+    if ((rc = step_blocks(c, L, true, &blk, &nblk))) break;
+    launch_force(L, c->D, c->d_T, c->cur, c->par, 0, F_ALLTERMS, true, nullptr, 0, nb, c->stream, blk, nblk);
     (void)hipEventRecord(e0, c->stream);
     for (int k = 0; k < 5; k++)
-      launch_force(L, c->D, c->d_T, c->cur, c->par, 0, F_ALLTERMS, true, nullptr, 0, nb, c->stream);
+      launch_force(L, c->D, c->d_T, c->cur, c->par, 0, F_ALLTERMS, true, nullptr, 0, nb, c->stream, blk, nblk);
     (void)hipEventRecord(e1, c->stream);
     HIP_OK(c, hipEventSynchronize(e1));
     float ms = 0.0f;
@@ -1685,6 +1737,7 @@ int sopg_autotune(sopg_ctx* c, int* step_lanes) {
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  if (rc) return rc;
   HIP_OK(c, hipGetLastError());
   c->lanes = bestL;
   if (step_lanes) *step_lanes = bestL;
@@ -1842,6 +1895,11 @@ static int run_impl(sopg_ctx* c, long nsteps, const double* noise, sopg_record* 
     HIP_OK(c, hipGetLastError());
     HIP_OK(c, hipEventRecord(c->ev_res, c->stream2));
   }
+  // the step kernel's blocks: those holding a bead that it steps (no block
+  // table in sopg_run_noalloc, which allocates nothing)
+  const int* blk = nullptr;
+  int nblk = 0;
+  if (stepped && !fixed && (rc = step_blocks(c, L, false, &blk, &nblk))) return rc;
   // step 1 of the run: rebuild if due, pair list, position half
   auto dr = due(0);
   int pe;
@@ -1887,7 +1945,8 @@ static int run_impl(sopg_ctx* c, long nsteps, const double* noise, sopg_record* 
     // every step computes the next step's pair list and position half (into
     // buffer cur^1); the last one leaves cur at rest, and its pair list and
     // position half are where the next run starts
-    launch_force(L, Dx, c->d_T, c->cur, c->par, k, F_STORE | F_VEL | fe | F_ALLTERMS | F_ADVANCE, true, nz, 0, nb, s);
+    launch_force(L, Dx, c->d_T, c->cur, c->par, k, F_STORE | F_VEL | fe | F_ALLTERMS | F_ADVANCE, true, nz, 0, nb, s,
+                 blk, nblk);
     prof_end(c, SOPG_KERNEL_STEP, pe);
     const Span rs = span_of(r, true);
     if (rs.any) {
@@ -1941,8 +2000,9 @@ static int run_impl(sopg_ctx* c, long nsteps, const double* noise, sopg_record* 
     // key: launch shape, parities, the segment's rebuild pattern, device layout
     std::vector<char> key;
     auto add = [&](const void* p, size_t n) { key.insert(key.end(), (const char*)p, (const char*)p + n); };
-    const int hdr[7] = {(int)S, c->cur, c->par, L, c->list_lanes, c->binned, c->skin > 0.0 ? 1 : 0};
+    const int hdr[8] = {(int)S, c->cur, c->par, L, c->list_lanes, c->binned, c->skin > 0.0 ? 1 : 0, nblk};
     add(hdr, sizeof hdr);
+    add(&blk, sizeof blk);
     for (long k = 0; k < S; k++) {
       const Span d = due(r + k + 1);
       const int f[5] = {(d.any ? 1 : 0) | (d.cell ? 2 : 0), d.g0, d.g1, d.c0, d.c1};

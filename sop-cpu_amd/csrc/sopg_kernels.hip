@@ -1450,15 +1450,18 @@ __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restric
 
 template <int L, bool NEXT, bool EN>
 __global__ void __launch_bounds__(256, 5) k_force(Dev D, const Tables* __restrict__ T, int cur, int pr, int rel, int flags,
-                                               const double* __restrict__ noise, int g0, int g1) {
+                                               const double* __restrict__ noise, int g0, int g1,
+                                               const int* __restrict__ blk) {
   rel += *D.relb;
   __shared__ __align__(16) double tp[kTypeRows][TP_N];
   load_type_tables(T, tp);
   __syncthreads();
-  const int t = xcd_block() * blockDim.x + threadIdx.x;
-  const int g = g0 + t / L;
+  // block b serves beads g0 + [b, b + 1) * per, or with a block table (the
+  // blocks of [0, g1) that hold a bead to step) beads [blk[b], blk[b] + 1) * per
+  const int per = (int)blockDim.x / L, b = xcd_block();
+  const int g = (blk ? blk[b] * per : g0 + b * per) + (int)threadIdx.x / L;
   if (g >= g1) return;
-  force_bead<L, NEXT, true, EN>(D, T, tp, cur, pr, rel, flags, noise, g, t & (L - 1));
+  force_bead<L, NEXT, true, EN>(D, T, tp, cur, pr, rel, flags, noise, g, (int)threadIdx.x & (L - 1));
 }
 
 // Observables (sop.cpp:200-280) and list counts, per bead.
@@ -1711,17 +1714,17 @@ static void launch_pairfilter_L(const Dev& D, const Tables* T, int cur, int par,
 }
 template <int L>
 static void launch_force_L(const Dev& D, const Tables* T, int cur, int pr, int rel, int flags, bool next,
-                           const double* noise, int g0, int g1, hipStream_t s) {
-  const dim3 grid(blocks_for((long)(g1 - g0) * L)), block(256);
+                           const double* noise, int g0, int g1, const int* blk, int nblk, hipStream_t s) {
+  const dim3 grid(blk ? (unsigned)nblk : blocks_for((long)(g1 - g0) * L)), block(256);
   const bool en = (flags & (F_ENERGY | F_ENERGY_ALL)) != 0;
   if (next && en)
-    hipLaunchKernelGGL((k_force<L, true, true>), grid, block, 0, s, D, T, cur, pr, rel, flags, noise, g0, g1);
+    hipLaunchKernelGGL((k_force<L, true, true>), grid, block, 0, s, D, T, cur, pr, rel, flags, noise, g0, g1, blk);
   else if (next)
-    hipLaunchKernelGGL((k_force<L, true, false>), grid, block, 0, s, D, T, cur, pr, rel, flags, noise, g0, g1);
+    hipLaunchKernelGGL((k_force<L, true, false>), grid, block, 0, s, D, T, cur, pr, rel, flags, noise, g0, g1, blk);
   else if (en)
-    hipLaunchKernelGGL((k_force<L, false, true>), grid, block, 0, s, D, T, cur, pr, rel, flags, noise, g0, g1);
+    hipLaunchKernelGGL((k_force<L, false, true>), grid, block, 0, s, D, T, cur, pr, rel, flags, noise, g0, g1, blk);
   else
-    hipLaunchKernelGGL((k_force<L, false, false>), grid, block, 0, s, D, T, cur, pr, rel, flags, noise, g0, g1);
+    hipLaunchKernelGGL((k_force<L, false, false>), grid, block, 0, s, D, T, cur, pr, rel, flags, noise, g0, g1, blk);
 }
 
 #define SOPG_DISPATCH_L(L, CALL)          \
@@ -1770,8 +1773,8 @@ void launch_pairfilter(int L, const Dev& D, const Tables* T, int cur, int par, i
 #undef C_
 }
 void launch_force(int L, const Dev& D, const Tables* T, int cur, int pr, int rel, int flags, bool next,
-                  const double* noise, int g0, int g1, hipStream_t s) {
-#define C_(X) launch_force_L<X>(D, T, cur, pr, rel, flags, next, noise, g0, g1, s)
+                  const double* noise, int g0, int g1, hipStream_t s, const int* blk, int nblk) {
+#define C_(X) launch_force_L<X>(D, T, cur, pr, rel, flags, next, noise, g0, g1, blk, nblk, s)
   SOPG_DISPATCH_L(L, C_)
 #undef C_
 }

@@ -522,6 +522,33 @@ def test_continuous_batching_join_leave(inputs_root):
     assert [r for r in r2 if r["chain"] == d] == [dict(r, chain=d) for r in rd]
 
 
+@pytest.mark.parametrize("lanes", [1, 4, 64])
+def test_freed_beads_skipped(inputs_root, lanes):
+    """Chains that leave free beads behind: a hole of whole step-kernel blocks
+    (which the step launch then skips) and a freed last chain (the high-water
+    mark falls), with a resident chain beside them.  The chains that stay
+    evolve bit-identically to the same chains alone."""
+    data = []
+    for n, seed in ((90, 71), (1500, 72), (150, 73), (700, 74)):
+        x, v, _ = chain_inputs(inputs_root, n, seed)
+        data.append((x, v, 700 + seed))
+    print(f"[gpu] freed beads L={lanes}", flush=True)
+    mix = _philox_ctx(data, lanes=lanes)
+    mix.set_resident(100)  # chain 0 runs in the resident kernel
+    r1 = mix.run(40)
+    mix.remove_chain(1)
+    mix.remove_chain(3)
+    r2 = mix.run(70)
+    for c, d in ((0, data[0]), (2, data[2])):
+        alone = _philox_ctx([d], lanes=lanes)
+        alone.set_resident(100)
+        ra = alone.run(40) + alone.run(70)
+        got, ref = mix.get_state(c), alone.get_state(0)
+        for k in ("unc", "vel", "force", "pos"):
+            assert np.array_equal(got[k], ref[k]), (c, k)
+        assert [r for r in r1 + r2 if r["chain"] == c] == [dict(r, chain=c) for r in ra]
+
+
 def test_autotune_keeps_results(inputs_root):
     """The autotuner changes the launch shape only: results stay bit-identical."""
     chains = []
