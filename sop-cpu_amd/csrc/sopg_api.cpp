@@ -1831,6 +1831,20 @@ static int run_impl(sopg_ctx* c, long nsteps, const double* noise, sopg_record* 
       sp.any = true;
       sp.cell |= h.p.cutoff == SOPG_CUTOFF_CELL;
     }
+    // The binned kernels follow each bead's chain plan, which k_grid writes
+    // for the chain slots of its launch (the plan of a chain not due: no
+    // work).  Slot and bead orders differ once chains have left and joined,
+    // so the slot range grows to every chain with beads in the bead range.
+    if (sp.any && !rec)
+      for (int id = 0; id < (int)c->chains.size(); id++) {
+        const HostChain& h = c->chains[id];
+        if (!h.active || !h.on_device) continue;
+        const int b0 = (int)h.at[P_BEAD];
+        if (b0 < sp.g1 && b0 + h.n > sp.g0) {
+          sp.c0 = std::min(sp.c0, id);
+          sp.c1 = std::max(sp.c1, id + 1);
+        }
+      }
     return sp;
   };
   auto due = [&](long r) { return span_of(r, false); };

@@ -549,6 +549,39 @@ def test_freed_beads_skipped(inputs_root, lanes):
         assert [r for r in r1 + r2 if r["chain"] == c] == [dict(r, chain=c) for r in ra]
 
 
+def test_rebuild_span_leaves_chains_not_due(inputs_root):
+    """A step's list rebuild covers the bead range of the chains due at that step.  A
+    chain inside that range that is not due keeps its lists until its own rebuild step,
+    also when its slot id lies outside the due chains' slots (slot and bead orders differ
+    after a chain left and a larger one took its slot at the end of the bead range)."""
+    data = []
+    for n, seed in ((100, 81), (400, 82), (900, 83), (600, 84)):
+        x, v, _ = chain_inputs(inputs_root, n, seed)
+        data.append((x, v, 800 + seed))
+    print("[gpu] rebuild span", flush=True)
+    mix = _philox_ctx(data[:3])
+    mix.remove_chain(0)
+    x, v, seed = data[3]
+    d = mix.add_chain(sopg.chain_topology(x), sopg.chain_params(cutoff=sopg.CUTOFF_NEIGHBOR, nnlup=20, nup=100,
+                                                                noise_mode=sopg.NOISE_PHILOX, seed=seed))
+    assert d == 0  # slot 0, beads after chain 2's
+    _setup_chain(mix, d, x, v)
+    # chains 0 and 1 rebuild at run steps 19 and 39, chain 2 at steps 9 and 29
+    phase = {0: 1, 1: 1, 2: 11}
+    for c, ph in phase.items():
+        mix.set_step(c, 1.0, ph, 1)
+    mix.run(25)
+    for c, k in ((0, 3), (1, 1), (2, 2)):
+        alone = _philox_ctx([data[k]])
+        alone.set_step(0, 1.0, phase[c], 1)
+        alone.run(25)
+        got, ref = mix.get_state(c), alone.get_state(0)
+        for key in ("unc", "vel", "force", "pos"):
+            assert np.array_equal(got[key], ref[key]), (c, key)
+        for k, ((mi, mj), (ai, aj)) in enumerate(zip(_lists(mix, c), _lists(alone, 0))):
+            assert np.array_equal(mi, ai) and np.array_equal(mj, aj), (c, k)
+
+
 def test_autotune_keeps_results(inputs_root):
     """The autotuner changes the launch shape only: results stay bit-identical."""
     chains = []
