@@ -26,8 +26,10 @@
 
 namespace sopg {
 void launch_cellcoords(const Dev& D, int cur, int g0, int g1, hipStream_t s);
+// tiles: ntiles bead tiles (kBeadTile beads each) that cover the beads of
+// [g0, g1) to rebuild, or null for every bead of [g0, g1)
 void launch_rebuild(int L, const Dev& D, const Tables* T, int cur, int par, int rel, int mode, int g0, int g1,
-                    int c0, int nc, int binned, int cand, hipStream_t s);
+                    int c0, int nc, int binned, int cand, hipStream_t s, const int* tiles = nullptr, int ntiles = 0);
 void launch_pairfilter(int L, const Dev& D, const Tables* T, int cur, int par, int g0, int g1, hipStream_t s);
 // blk: nblk block numbers of 256 / L beads each (blocks of [0, g1)), or null
 // for every block of [g0, g1)
@@ -195,10 +197,14 @@ struct sopg_ctx {
   size_t rlist_cap = 0;
   int* d_rslot = nullptr;
   size_t rslot_cap = 0;
-  // block table of the step kernel (step_blocks): what d_blk holds
-  int* d_blk = nullptr;
-  size_t blk_cap = 0;
-  std::vector<int> blk_dev;
+  // block tables (block_table) of the step kernel and of the binned rebuild:
+  // device array, its capacity, and what it holds
+  struct BlockTable {
+    int* d = nullptr;
+    size_t cap = 0;
+    std::vector<int> dev;
+  };
+  BlockTable step_tab, tile_tab;
   // kernel timing (sopg_profile)
   bool prof = false;
   std::vector<hipEvent_t> ev_pool;
@@ -943,13 +949,14 @@ int drop_graphs(sopg_ctx* c) {
   return SOPG_OK;
 }
 
-// The step kernel's blocks of 256 / L beads (block k: beads [k, k + 1) * 256 / L)
-// that hold a bead of an active chain (not a resident one, unless
-// with_resident).  Chains that left free beads behind (a retired job) leave
-// whole blocks of [0, nb) empty; the step launch then covers the table's
-// blocks only.  *blk = null (every block) when no block is empty.
-int step_blocks(sopg_ctx* c, int L, bool with_resident, const int** blk, int* nblk) {
-  const int per = 256 / L, total = (c->D.nb + per - 1) / per;
+// Tables of bead blocks (block k: beads [k, k + 1) * per) that hold a bead of
+// an active chain (not a resident one, unless with_resident).  Chains that
+// left free beads behind (a retired job) leave whole blocks of [0, nb) empty;
+// a launch given the table covers its blocks only: the step kernel (256 / L
+// beads per block) and the binned rebuild (kBeadTile beads per tile).
+// *blk = null (every block) when no block is empty.
+int block_table(sopg_ctx* c, sopg_ctx::BlockTable& bt, int per, bool with_resident, const int** blk, int* nblk) {
+  const int total = (c->D.nb + per - 1) / per;
   *blk = nullptr;
   *nblk = total;
   std::vector<char> hit((size_t)total, 0);
@@ -963,25 +970,28 @@ int step_blocks(sopg_ctx* c, int L, bool with_resident, const int** blk, int* nb
     if (hit[(size_t)k]) t.push_back(k);
   if (t.empty() || (int)t.size() == total) return SOPG_OK;
   int rc;
-  if (t != c->blk_dev) {
-    if (t.size() > c->blk_cap) {
+  if (t != bt.dev) {
+    if (t.size() > bt.cap) {
       // cached graphs hold the old table's address
       if ((rc = drop_graphs(c))) return rc;
       HIP_OK(c, hipStreamSynchronize(c->stream));
-      dfree(c, c->d_blk);
-      c->d_blk = nullptr;
-      c->blk_cap = 0;
-      c->blk_dev.clear();
+      dfree(c, bt.d);
+      bt.d = nullptr;
+      bt.cap = 0;
+      bt.dev.clear();
       const size_t cap = std::max<size_t>(t.size(), (size_t)total);
-      if ((rc = dalloc(c, &c->d_blk, cap))) return rc;
-      c->blk_cap = cap;
+      if ((rc = dalloc(c, &bt.d, cap))) return rc;
+      bt.cap = cap;
     }
-    if ((rc = upload_at(c, c->d_blk, 0, t))) return rc;
-    c->blk_dev = t;
+    if ((rc = upload_at(c, bt.d, 0, t))) return rc;
+    bt.dev = t;
   }
-  *blk = c->d_blk;
+  *blk = bt.d;
   *nblk = (int)t.size();
   return SOPG_OK;
+}
+int step_blocks(sopg_ctx* c, int L, bool with_resident, const int** blk, int* nblk) {
+  return block_table(c, c->step_tab, 256 / L, with_resident, blk, nblk);
 }
 
 // Candidate lists built with other cutoffs or another skin are rebuilt at
@@ -1914,6 +1924,27 @@ static int run_impl(sopg_ctx* c, long nsteps, const double* noise, sopg_record* 
   const int* blk = nullptr;
   int nblk = 0;
   if (stepped && !fixed && (rc = step_blocks(c, L, false, &blk, &nblk))) return rc;
+  // and the binned rebuild's tiles: a rebuild of bead range [g0, g1) takes the
+  // table's run of tiles that cover it
+  const int* tiles = nullptr;
+  int ntiles = 0;
+  if (stepped && !fixed && (rc = block_table(c, c->tile_tab, kBeadTile, false, &tiles, &ntiles))) return rc;
+  struct TileRun {
+    const int* p = nullptr;
+    int n = 0;
+  };
+  auto tiles_of = [&](const Span& d) {
+    TileRun tr;
+    if (!tiles || !d.any) return tr;
+    const std::vector<int>& t = c->tile_tab.dev;
+    const long i0 = std::lower_bound(t.begin(), t.end(), d.g0 / kBeadTile) - t.begin();
+    const long i1 = std::lower_bound(t.begin(), t.end(), (d.g1 + kBeadTile - 1) / kBeadTile) - t.begin();
+    if (i1 > i0) {
+      tr.p = tiles + i0;
+      tr.n = (int)(i1 - i0);
+    }
+    return tr;
+  };
   // step 1 of the run: rebuild if due, pair list, position half
   auto dr = due(0);
   int pe;
@@ -1926,8 +1957,9 @@ static int run_impl(sopg_ctx* c, long nsteps, const double* noise, sopg_record* 
   } else if (dr.any) {
     pe = prof_begin(c);
     if (dr.cell) launch_cellcoords(D, c->cur, dr.g0, dr.g1, s);
+    const TileRun tr = tiles_of(dr);
     launch_rebuild(c->list_lanes, D, c->d_T, c->cur, c->par, 0, -1, dr.g0, dr.g1, dr.c0, dr.c1 - dr.c0, c->binned,
-                   c->skin > 0.0, s);
+                   c->skin > 0.0, s, tr.p, tr.n);
     prof_end(c, SOPG_KERNEL_REBUILD, pe);
   }
   // pair lists at the current positions; rerunning the filter on a chain whose
@@ -1977,8 +2009,9 @@ static int run_impl(sopg_ctx* c, long nsteps, const double* noise, sopg_record* 
       if (d.any) {
         pe = prof_begin(c);
         if (d.cell) launch_cellcoords(Dx, c->cur, d.g0, d.g1, s);
+        const TileRun tr = tiles_of(d);
         launch_rebuild(c->list_lanes, Dx, c->d_T, c->cur, c->par, k + 1, -1, d.g0, d.g1, d.c0, d.c1 - d.c0, c->binned,
-                       c->skin > 0.0, s);
+                       c->skin > 0.0, s, tr.p, tr.n);
         prof_end(c, SOPG_KERNEL_REBUILD, pe);
       }
       c->cur ^= 1;
@@ -2019,8 +2052,10 @@ static int run_impl(sopg_ctx* c, long nsteps, const double* noise, sopg_record* 
     add(&blk, sizeof blk);
     for (long k = 0; k < S; k++) {
       const Span d = due(r + k + 1);
-      const int f[5] = {(d.any ? 1 : 0) | (d.cell ? 2 : 0), d.g0, d.g1, d.c0, d.c1};
+      const TileRun tr = tiles_of(d);
+      const int f[6] = {(d.any ? 1 : 0) | (d.cell ? 2 : 0), d.g0, d.g1, d.c0, d.c1, tr.n};
       add(f, sizeof f);
+      add(&tr.p, sizeof tr.p);
     }
     add(&Dg, sizeof Dg);
     add(&c->d_T, sizeof c->d_T);

@@ -86,6 +86,12 @@ constexpr int kAttTShift = 25;
 // Per-bead reduction fields (red[g * kRed + field]).
 enum { R_EBND = 0, R_EANG, R_EATT, R_EREP, R_CONT, R_RG, R_V2, R_NLA, R_PLA, R_NLR, R_PLR, kRed = 12 };
 
+// Bead tiles of the binned list-rebuild launches: with free beads in a
+// rebuild's bead range, the launch takes a table of the tiles (kBeadTile
+// consecutive beads, tile k = beads [k, k + 1) * kBeadTile) that hold a bead
+// of an active chain.
+constexpr int kBeadTile = 16;
+
 // Records of sopg_run_noalloc are staged in a file-scope device array of
 // this many doubles (4 MiB; one copy per GPU, held until the process ends).
 constexpr size_t kScratchDoubles = (size_t)1 << 19;

@@ -627,10 +627,11 @@ __global__ void __launch_bounds__(kGridThreads) k_grid(Dev D, const Tables* __re
 // consecutive in the cell-sorted bead array.
 template <int LG>
 __global__ void __launch_bounds__(256) k_rebuild_grid(Dev D, const Tables* __restrict__ T, int cur, int par, int rel,
-                                                      int mode, int g0, int g1) {
+                                                      int mode, int g0, int g1, const int* __restrict__ tiles) {
   rel += *D.relb;
   if (D.plan[0] != rel) return;  // block-uniform: before any barrier
   constexpr int BPB = 256 / LG;
+  static_assert(BPB == kBeadTile, "one block per bead tile");
   extern __shared__ uint32_t bitmaps[];
   // per type pair (it * 3 + jt): (3.2 sigma)^2, (2.5 sigma)^2, (3.2 sigma + skin)^2
   __shared__ double cut2[9][3];
@@ -640,11 +641,14 @@ __global__ void __launch_bounds__(256) k_rebuild_grid(Dev D, const Tables* __res
     cut2[threadIdx.x][2] = T->candc2[threadIdx.x / 3][threadIdx.x % 3];
   }
   const int grp = (int)threadIdx.x / LG, l = (int)threadIdx.x & (LG - 1);
-  const int g = g0 + xcd_block() * BPB + grp;
+  // block blk: beads g0 + [blk, blk + 1) * BPB, or with a tile table (the
+  // tiles of kBeadTile beads that hold a bead of [g0, g1)) tile tiles[blk]
+  const int blk = xcd_block();
+  const int g = (tiles ? tiles[blk] * kBeadTile : g0 + blk * BPB) + grp;
   const int W = D.bitmap_words;
   uint32_t* nlb = bitmaps + (size_t)grp * 2 * W;
   uint32_t* plb = nlb + W;
-  bool active = g < g1;
+  bool active = g >= g0 && g < g1;
   if (active) active = D.chain[g] >= 0;
   int crit = 0, work = 0;
   const ChainP* Pp = nullptr;
@@ -920,7 +924,7 @@ __device__ __forceinline__ V wave_chain_fields(const ChainP* __restrict__ cp, in
 // natives as in the full scan.  One group of LF lanes per bead.
 template <int LF>
 __global__ void __launch_bounds__(256) k_cand_filter(Dev D, const Tables* __restrict__ T, int cur, int par, int rel,
-                                                     int g0, int g1) {
+                                                     int g0, int g1, const int* __restrict__ tiles, int ntiles) {
   rel += *D.relb;
   if (D.plan[1] != rel) return;  // block-uniform: before the barrier
   __shared__ double cut2[9][2];  // per type pair (it * 3 + jt): (3.2 sigma)^2, (2.5 sigma)^2
@@ -929,9 +933,20 @@ __global__ void __launch_bounds__(256) k_cand_filter(Dev D, const Tables* __rest
     cut2[threadIdx.x][1] = T->plrc2[threadIdx.x / 3][threadIdx.x % 3];
   }
   __syncthreads();
+  // thread t: bead g0 + t / LF, or with a tile table (the tiles of kBeadTile
+  // beads that hold a bead of [g0, g1)) bead (t / LF) % kBeadTile of tile
+  // tiles[t / (LF * kBeadTile)]
   const int t = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
-  const int g = g0 + t / LF, l = t & (LF - 1);
-  if (g >= g1) return;
+  int g;
+  if (tiles) {
+    const int k = t / (LF * kBeadTile);
+    if (k >= ntiles) return;
+    g = tiles[k] * kBeadTile + (t / LF) % kBeadTile;
+  } else {
+    g = g0 + t / LF;
+  }
+  const int l = t & (LF - 1);
+  if (g < g0 || g >= g1) return;
   const int c = D.chain[g];
   if (c < 0 || !(D.grid[c].work & W_FILTER)) return;
   const ChainF cf = wave_chain_fields<ChainF>(D.cp, c);
@@ -1748,24 +1763,27 @@ constexpr int kGridBeads = 256 / kGridLanes;         // beads per block
 constexpr int kFilterLanes = 8;                     // lanes per bead in the candidate filter
 constexpr int kMaxBitmapWords = 512;                 // 16 beads x 2 bitmaps x 2 KiB = 64 KiB of LDS
 void launch_rebuild(int L, const Dev& D, const Tables* T, int cur, int par, int rel, int mode, int g0, int g1,
-                    int c0, int nc, int binned, int cand, hipStream_t s) {
+                    int c0, int nc, int binned, int cand, hipStream_t s, const int* tiles, int ntiles) {
   const int words = D.bitmap_words < kMaxBitmapWords ? D.bitmap_words : kMaxBitmapWords;
   hipLaunchKernelGGL(k_grid, dim3(nc), dim3(kGridThreads), 0, s, D, T, cur, rel, mode, c0, binned ? 32 * words : 0);
   if (binned) {
     Dev Dg = D;
     Dg.bitmap_words = words;
-    hipLaunchKernelGGL(k_rebuild_grid<kGridLanes>, dim3((unsigned)((g1 - g0 + kGridBeads - 1) / kGridBeads)),
-                       dim3(256), (size_t)kGridBeads * 2 * words * sizeof(uint32_t), s, Dg, T, cur, par, rel, mode,
-                       g0, g1);
+    const unsigned blocks = tiles ? (unsigned)ntiles : (unsigned)((g1 - g0 + kGridBeads - 1) / kGridBeads);
+    hipLaunchKernelGGL(k_rebuild_grid<kGridLanes>, dim3(blocks), dim3(256),
+                       (size_t)kGridBeads * 2 * words * sizeof(uint32_t), s, Dg, T, cur, par, rel, mode, g0, g1,
+                       tiles);
   }
   // the full scan (lists, or the candidate lists of chains too wide for the
   // grid), then the filter of every chain planned to filter its candidates
 #define C_(X) launch_rebuild_L<X>(D, T, cur, par, rel, mode, g0, g1, s)
   SOPG_DISPATCH_L(L, C_)
 #undef C_
-  if (binned && cand)
-    hipLaunchKernelGGL(k_cand_filter<kFilterLanes>, dim3(blocks_for((long)(g1 - g0) * kFilterLanes)), dim3(256), 0, s,
-                       D, T, cur, par, rel, g0, g1);
+  if (binned && cand) {
+    const long threads = tiles ? (long)ntiles * kBeadTile * kFilterLanes : (long)(g1 - g0) * kFilterLanes;
+    hipLaunchKernelGGL(k_cand_filter<kFilterLanes>, dim3(blocks_for(threads)), dim3(256), 0, s, D, T, cur, par, rel,
+                       g0, g1, tiles, ntiles);
+  }
 }
 void launch_pairfilter(int L, const Dev& D, const Tables* T, int cur, int par, int g0, int g1, hipStream_t s) {
 #define C_(X) launch_pairfilter_L<X>(D, T, cur, par, g0, g1, s)
