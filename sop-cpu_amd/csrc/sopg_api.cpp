@@ -203,6 +203,12 @@ struct sopg_ctx {
     int* d = nullptr;
     size_t cap = 0;
     std::vector<int> dev;
+    // the last call's block size, block count, chain block ranges and result
+    bool valid = false;
+    int per = 0, total = 0;
+    std::vector<std::pair<int, int>> ranges;
+    const int* out = nullptr;
+    int nout = 0;
   };
   BlockTable step_tab, tile_tab;
   // kernel timing (sopg_profile)
@@ -957,37 +963,58 @@ int drop_graphs(sopg_ctx* c) {
 // *blk = null (every block) when no block is empty.
 int block_table(sopg_ctx* c, sopg_ctx::BlockTable& bt, int per, bool with_resident, const int** blk, int* nblk) {
   const int total = (c->D.nb + per - 1) / per;
-  *blk = nullptr;
-  *nblk = total;
-  std::vector<char> hit((size_t)total, 0);
-  for (auto& h : c->chains)
+  // the block range [k0, k1] of each chain the launch serves, in chain order:
+  // the table of the last call stands while these are unchanged
+  std::vector<std::pair<int, int>> r;
+  for (const HostChain& h : c->chains)
     if (h.active && h.on_device && h.n > 0 && (with_resident || !h.resident)) {
       const int b0 = (int)h.at[P_BEAD];
-      for (int k = b0 / per; k <= (b0 + h.n - 1) / per; k++) hit[(size_t)k] = 1;
+      r.push_back({b0 / per, (b0 + h.n - 1) / per});
     }
-  std::vector<int> t;
-  for (int k = 0; k < total; k++)
-    if (hit[(size_t)k]) t.push_back(k);
-  if (t.empty() || (int)t.size() == total) return SOPG_OK;
-  int rc;
-  if (t != bt.dev) {
-    if (t.size() > bt.cap) {
-      // cached graphs hold the old table's address
-      if ((rc = drop_graphs(c))) return rc;
-      HIP_OK(c, hipStreamSynchronize(c->stream));
-      dfree(c, bt.d);
-      bt.d = nullptr;
-      bt.cap = 0;
-      bt.dev.clear();
-      const size_t cap = std::max<size_t>(t.size(), (size_t)total);
-      if ((rc = dalloc(c, &bt.d, cap))) return rc;
-      bt.cap = cap;
-    }
-    if ((rc = upload_at(c, bt.d, 0, t))) return rc;
-    bt.dev = t;
+  if (bt.valid && bt.per == per && bt.total == total && bt.ranges == r) {
+    *blk = bt.out;
+    *nblk = bt.out ? bt.nout : total;
+    return SOPG_OK;
   }
-  *blk = bt.d;
-  *nblk = (int)t.size();
+  bt.valid = false;  // set again once the table below is in place
+  bt.ranges = r;
+  *blk = nullptr;
+  *nblk = total;
+  std::sort(r.begin(), r.end());
+  std::vector<std::pair<int, int>> m;  // merged ranges
+  for (auto& x : r) {
+    if (!m.empty() && x.first <= m.back().second + 1) m.back().second = std::max(m.back().second, x.second);
+    else m.push_back(x);
+  }
+  int rc;
+  if (!m.empty() && !(m.size() == 1 && m[0].first == 0 && m[0].second == total - 1)) {
+    std::vector<int> t;
+    for (auto& x : m)
+      for (int k = x.first; k <= x.second; k++) t.push_back(k);
+    if (t != bt.dev) {
+      if (t.size() > bt.cap) {
+        // cached graphs hold the old table's address
+        if ((rc = drop_graphs(c))) return rc;
+        HIP_OK(c, hipStreamSynchronize(c->stream));
+        dfree(c, bt.d);
+        bt.d = nullptr;
+        bt.cap = 0;
+        bt.dev.clear();
+        const size_t cap = std::max<size_t>(t.size(), (size_t)total);
+        if ((rc = dalloc(c, &bt.d, cap))) return rc;
+        bt.cap = cap;
+      }
+      if ((rc = upload_at(c, bt.d, 0, t))) return rc;
+      bt.dev.swap(t);
+    }
+    *blk = bt.d;
+    *nblk = (int)bt.dev.size();
+  }
+  bt.valid = true;
+  bt.per = per;
+  bt.total = total;
+  bt.out = *blk;
+  bt.nout = *nblk;
   return SOPG_OK;
 }
 int step_blocks(sopg_ctx* c, int L, bool with_resident, const int** blk, int* nblk) {
