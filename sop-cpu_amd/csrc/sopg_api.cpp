@@ -1885,17 +1885,23 @@ static int run_impl(sopg_ctx* c, long nsteps, const double* noise, sopg_record* 
     return sp;
   };
   auto due = [&](long r) { return span_of(r, false); };
-  auto records = [&](long r) { return span_of(r, true).any; };
+  // the run steps at which some chain records (bit 0), some chain on the
+  // per-step path (bit 1): chain h records at the steps r with
+  // (iup + r) % nup == 0, every nup steps from the first
+  std::vector<uint8_t> rec_at((size_t)nsteps, 0);
+  for (auto& h : c->chains)
+    if (h.active)
+      for (long r = (h.p.nup - h.iup % h.p.nup) % h.p.nup; r < nsteps; r += h.p.nup)
+        rec_at[(size_t)r] |= h.resident ? 1 : 3;
+  auto records = [&](long r) { return r < nsteps && (rec_at[(size_t)r] & 2) != 0; };
   // record buffer rows: one per run step at which any chain records
   std::vector<long> rec_steps;
   std::vector<int> row_of(nsteps, -1);
   for (long r = 0; r < nsteps; r++)
-    for (auto& h : c->chains)
-      if (h.active && (h.iup + r) % h.p.nup == 0) {
-        row_of[r] = (int)rec_steps.size();
-        rec_steps.push_back(r);
-        break;
-      }
+    if (rec_at[(size_t)r]) {
+      row_of[r] = (int)rec_steps.size();
+      rec_steps.push_back(r);
+    }
   const long nrs = (long)rec_steps.size();
   if (fixed && (size_t)nrs * nc * kRed > kScratchDoubles) {
     char m[256];
@@ -2021,7 +2027,7 @@ static int run_impl(sopg_ctx* c, long nsteps, const double* noise, sopg_record* 
     launch_force(L, Dx, c->d_T, c->cur, c->par, k, F_STORE | F_VEL | fe | F_ALLTERMS | F_ADVANCE, true, nz, 0, nb, s,
                  blk, nblk);
     prof_end(c, SOPG_KERNEL_STEP, pe);
-    const Span rs = span_of(r, true);
+    const Span rs = fe ? span_of(r, true) : Span();
     if (rs.any) {
       // chain slot c's sums land in row row_of[r], column c
       pe = prof_begin(c);
