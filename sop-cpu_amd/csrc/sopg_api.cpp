@@ -1207,6 +1207,7 @@ int sopg_remove_chain(sopg_ctx* c, int chain) {
     const size_t nb = h.bi.size(), na = h.gi.size(), nt = h.ai.size(), nr = h.rep_explicit ? h.ri.size() : 0;
     const size_t len[NPOOL] = {(size_t)h.n, 2 * nb, 2 * na, nt, 2 * nt, nr, 2 * nr, 0};
     for (int pi = 0; pi < P_CHAIN; pi++) c->pool[pi].give(h.at[pi], len[pi]);
+    c->D.nb = (int)c->pool[P_BEAD].hw;  // lower when the chain was the last in the bead range
   }
   h = HostChain();
   return upload_chainp(c, chain);
