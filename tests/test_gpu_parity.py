@@ -538,6 +538,8 @@ def test_freed_beads_skipped(inputs_root, lanes):
     r1 = mix.run(40)
     mix.remove_chain(1)
     mix.remove_chain(3)
+    if lanes == 4:
+        mix.autotune()  # times every lane count on the block tables of the holed batch
     r2 = mix.run(70)
     for c, d in ((0, data[0]), (2, data[2])):
         alone = _philox_ctx([d], lanes=lanes)
