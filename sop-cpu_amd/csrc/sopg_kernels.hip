@@ -1151,38 +1151,65 @@ __device__ __forceinline__ void load_type_tables(const Tables* __restrict__ T, d
 
 // EN false: the launch records no energy (no chain's record step), and the
 // energy terms are compiled out.
+// The bead's own fields.  bead_head_load issues the loads and bead_head_share
+// completes them, so that a kernel can put the loads out before its block
+// barrier.  L = 4: the group's four lanes share the fields: each lane loads
+// two of the eight index fields and one coordinate (into chain, bond0 and cx)
+// and the group exchanges them (a quarter of the vector-memory data of every
+// lane loading every field).
+struct BeadHead {
+  int chain, bond0, bond1, ang0, ang1, att0, att1, nslot;
+  double cx, cy, cz;
+};
+template <int L>
+__device__ __forceinline__ void bead_head_load(const Dev& D, const double* __restrict__ ux,
+                                               const double* __restrict__ uy, const double* __restrict__ uz, int g,
+                                               int l, BeadHead& h) {
+  if constexpr (L == 4) {
+    const int* __restrict__ pa = l == 0 ? D.chain : l == 1 ? D.bond_end : l == 2 ? D.ang_end : D.att_end;
+    const int* __restrict__ pb = l == 0 ? D.bond_beg : l == 1 ? D.ang_beg : l == 2 ? D.att_beg : D.cnt;
+    const double* __restrict__ pc = l == 0 ? ux : l == 1 ? uy : uz;
+    h.chain = pa[g];
+    h.bond0 = pb[g];
+    h.cx = pc[g];
+  } else {
+    h.chain = D.chain[g];
+    h.cx = ux[g]; h.cy = uy[g]; h.cz = uz[g];
+    h.bond0 = D.bond_beg[g]; h.bond1 = D.bond_end[g]; h.ang0 = D.ang_beg[g]; h.ang1 = D.ang_end[g];
+    h.att0 = D.att_beg[g]; h.att1 = D.att_end[g]; h.nslot = D.cnt[g];
+  }
+}
+template <int L>
+__device__ __forceinline__ void bead_head_share(BeadHead& h) {
+  if constexpr (L == 4) {
+    const int va = h.chain, vb = h.bond0;
+    const double vc = h.cx;
+    h.chain = group_bcast_i<0, 4>(va); h.bond0 = group_bcast_i<0, 4>(vb);
+    h.bond1 = group_bcast_i<1, 4>(va); h.ang0 = group_bcast_i<1, 4>(vb);
+    h.ang1 = group_bcast_i<2, 4>(va); h.att0 = group_bcast_i<2, 4>(vb);
+    h.att1 = group_bcast_i<3, 4>(va); h.nslot = group_bcast_i<3, 4>(vb);
+    h.cx = group_bcast<0, 4>(vc); h.cy = group_bcast<1, 4>(vc); h.cz = group_bcast<2, 4>(vc);
+  }
+}
+
 template <int L, bool NEXT, bool SKIP_RES = false, bool EN = true>
 __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restrict__ T, const double (*tp)[TP_N],
                                            int cur, int pr, int rel, int flags, const double* __restrict__ noise,
-                                           int g, int l) {
+                                           int g, int l, const BeadHead* pre = nullptr) {
   // loads in two batches: bead-indexed ones (any bead below the high-water
-  // mark is valid memory), then the chain's fields
+  // mark is valid memory; pre: already issued by the caller), then the
+  // chain's fields
   const double* __restrict__ ux = cur ? D.ux[1] : D.ux[0];
   const double* __restrict__ uy = cur ? D.uy[1] : D.uy[0];
   const double* __restrict__ uz = cur ? D.uz[1] : D.uz[0];
   const __amdgpu_buffer_rsrc_t bux = buf_view(ux), buy = buf_view(uy), buz = buf_view(uz);
-  int chain, bond0, bond1, ang0, ang1, att0, att1, nslot;
-  double cx, cy, cz;
-  if constexpr (L == 4) {
-    // the group's four lanes share the bead's fields: each lane loads two of
-    // the eight index fields and one coordinate, and the group exchanges them
-    // (a quarter of the vector-memory data of every lane loading every field)
-    const int* __restrict__ pa = l == 0 ? D.chain : l == 1 ? D.bond_end : l == 2 ? D.ang_end : D.att_end;
-    const int* __restrict__ pb = l == 0 ? D.bond_beg : l == 1 ? D.ang_beg : l == 2 ? D.att_beg : D.cnt;
-    const double* __restrict__ pc = l == 0 ? ux : l == 1 ? uy : uz;
-    const int va = pa[g], vb = pb[g];
-    const double vc = pc[g];
-    chain = group_bcast_i<0, 4>(va); bond0 = group_bcast_i<0, 4>(vb);
-    bond1 = group_bcast_i<1, 4>(va); ang0 = group_bcast_i<1, 4>(vb);
-    ang1 = group_bcast_i<2, 4>(va); att0 = group_bcast_i<2, 4>(vb);
-    att1 = group_bcast_i<3, 4>(va); nslot = group_bcast_i<3, 4>(vb);
-    cx = group_bcast<0, 4>(vc); cy = group_bcast<1, 4>(vc); cz = group_bcast<2, 4>(vc);
-  } else {
-    chain = D.chain[g];
-    cx = ux[g]; cy = uy[g]; cz = uz[g];
-    bond0 = D.bond_beg[g]; bond1 = D.bond_end[g]; ang0 = D.ang_beg[g]; ang1 = D.ang_end[g];
-    att0 = D.att_beg[g]; att1 = D.att_end[g]; nslot = D.cnt[g];
-  }
+  BeadHead h;
+  if (pre) h = *pre;
+  else bead_head_load<L>(D, ux, uy, uz, g, l, h);
+  bead_head_share<L>(h);
+  const int chain = h.chain, bond0 = h.bond0, bond1 = h.bond1, ang0 = h.ang0, ang1 = h.ang1, att0 = h.att0,
+            att1 = h.att1, nslot = h.nslot;
+  const double cx = h.cx, cy = h.cy, cz = h.cz;
   if (chain < 0) return;  // a free bead
   // chain fields, in registers (the loops' stores could alias them)
   const ChainV cv = wave_chain_fields<ChainV>(D.cp, chain);
@@ -1469,14 +1496,18 @@ __global__ void __launch_bounds__(256, 5) k_force(Dev D, const Tables* __restric
                                                const int* __restrict__ blk) {
   rel += *D.relb;
   __shared__ __align__(16) double tp[kTypeRows][TP_N];
-  load_type_tables(T, tp);
-  __syncthreads();
   // block b serves beads g0 + [b, b + 1) * per, or with a block table (the
   // blocks of [0, g1) that hold a bead to step) beads [blk[b], blk[b] + 1) * per
   const int per = (int)blockDim.x / L, b = xcd_block();
   const int g = (blk ? blk[b] * per : g0 + b * per) + (int)threadIdx.x / L;
+  const int l = (int)threadIdx.x & (L - 1);
+  // the bead's own loads go out with the type tables', before the barrier
+  BeadHead h;
+  if (g < g1) bead_head_load<L>(D, cur ? D.ux[1] : D.ux[0], cur ? D.uy[1] : D.uy[0], cur ? D.uz[1] : D.uz[0], g, l, h);
+  load_type_tables(T, tp);
+  __syncthreads();
   if (g >= g1) return;
-  force_bead<L, NEXT, true, EN>(D, T, tp, cur, pr, rel, flags, noise, g, (int)threadIdx.x & (L - 1));
+  force_bead<L, NEXT, true, EN>(D, T, tp, cur, pr, rel, flags, noise, g, l, &h);
 }
 
 // Observables (sop.cpp:200-280) and list counts, per bead.
