@@ -1195,24 +1195,28 @@ __device__ __forceinline__ void bead_head_share(BeadHead& h) {
 template <int L, bool NEXT, bool SKIP_RES = false, bool EN = true>
 __device__ __forceinline__ void force_bead(const Dev& D, const Tables* __restrict__ T, const double (*tp)[TP_N],
                                            int cur, int pr, int rel, int flags, const double* __restrict__ noise,
-                                           int g, int l, const BeadHead* pre = nullptr) {
+                                           int g, int l, const BeadHead* pre = nullptr,
+                                           const ChainV* pre_cv = nullptr) {
   // loads in two batches: bead-indexed ones (any bead below the high-water
-  // mark is valid memory; pre: already issued by the caller), then the
-  // chain's fields
+  // mark is valid memory), then the chain's fields; pre, pre_cv: loaded (and
+  // shared) by the caller
   const double* __restrict__ ux = cur ? D.ux[1] : D.ux[0];
   const double* __restrict__ uy = cur ? D.uy[1] : D.uy[0];
   const double* __restrict__ uz = cur ? D.uz[1] : D.uz[0];
   const __amdgpu_buffer_rsrc_t bux = buf_view(ux), buy = buf_view(uy), buz = buf_view(uz);
   BeadHead h;
-  if (pre) h = *pre;
-  else bead_head_load<L>(D, ux, uy, uz, g, l, h);
-  bead_head_share<L>(h);
+  if (pre) {
+    h = *pre;
+  } else {
+    bead_head_load<L>(D, ux, uy, uz, g, l, h);
+    bead_head_share<L>(h);
+  }
   const int chain = h.chain, bond0 = h.bond0, bond1 = h.bond1, ang0 = h.ang0, ang1 = h.ang1, att0 = h.att0,
             att1 = h.att1, nslot = h.nslot;
   const double cx = h.cx, cy = h.cy, cz = h.cz;
   if (chain < 0) return;  // a free bead
   // chain fields, in registers (the loops' stores could alias them)
-  const ChainV cv = wave_chain_fields<ChainV>(D.cp, chain);
+  const ChainV cv = pre_cv ? *pre_cv : wave_chain_fields<ChainV>(D.cp, chain);
   const int off = cv.off, att_off = cv.att_off, natt = cv.natt;
   const int cutoff = cv.cutoff, noise_mode = cv.noise_mode, iup0 = cv.iup0, nup = cv.nup;
   const int resident = cv.resident;
@@ -1501,13 +1505,19 @@ __global__ void __launch_bounds__(256, 5) k_force(Dev D, const Tables* __restric
   const int per = (int)blockDim.x / L, b = xcd_block();
   const int g = (blk ? blk[b] * per : g0 + b * per) + (int)threadIdx.x / L;
   const int l = (int)threadIdx.x & (L - 1);
-  // the bead's own loads go out with the type tables', before the barrier
+  // the bead's own loads go out with the type tables', and its chain's
+  // fields once the chain id is in, all before the barrier
   BeadHead h;
+  ChainV cv;
   if (g < g1) bead_head_load<L>(D, cur ? D.ux[1] : D.ux[0], cur ? D.uy[1] : D.uy[0], cur ? D.uz[1] : D.uz[0], g, l, h);
   load_type_tables(T, tp);
+  if (g < g1) {
+    bead_head_share<L>(h);
+    if (h.chain >= 0) cv = wave_chain_fields<ChainV>(D.cp, h.chain);
+  }
   __syncthreads();
   if (g >= g1) return;
-  force_bead<L, NEXT, true, EN>(D, T, tp, cur, pr, rel, flags, noise, g, l, &h);
+  force_bead<L, NEXT, true, EN>(D, T, tp, cur, pr, rel, flags, noise, g, l, &h, &cv);
 }
 
 // Observables (sop.cpp:200-280) and list counts, per bead.
