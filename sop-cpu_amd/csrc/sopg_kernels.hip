@@ -927,42 +927,46 @@ __global__ void __launch_bounds__(256) k_cand_filter(Dev D, const Tables* __rest
                                                      int g0, int g1, const int* __restrict__ tiles, int ntiles) {
   rel += *D.relb;
   if (D.plan[1] != rel) return;  // block-uniform: before the barrier
+  // thread t: bead g0 + t / LF, or with a tile table (the tiles of kBeadTile
+  // beads that hold a bead of [g0, g1)) bead (t / LF) % kBeadTile of tile
+  // tiles[t / (LF * kBeadTile)]
+  const int t = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
+  int g = g0 + t / LF;
+  bool act = true;
+  if (tiles) {
+    const int k = t / (LF * kBeadTile);
+    act = k < ntiles;
+    if (act) g = tiles[k] * kBeadTile + (t / LF) % kBeadTile;
+  }
+  act = act && g >= g0 && g < g1;
+  const int l = t & (LF - 1);
+  const double* ux = cur ? D.ux[1] : D.ux[0];
+  const double* uy = cur ? D.uy[1] : D.uy[0];
+  const double* uz = cur ? D.uz[1] : D.uz[0];
+  // the bead's own loads go out with the cutoff table's, before the barrier
+  int c = -1, n = 0;
+  double cx = 0.0, cy = 0.0, cz = 0.0;
+  if (act) {
+    c = D.chain[g];
+    n = D.ccnt[g];
+    cx = ux[g]; cy = uy[g]; cz = uz[g];
+  }
   __shared__ double cut2[9][2];  // per type pair (it * 3 + jt): (3.2 sigma)^2, (2.5 sigma)^2
   if (threadIdx.x < 9) {
     cut2[threadIdx.x][0] = T->nlrc2[threadIdx.x / 3][threadIdx.x % 3];
     cut2[threadIdx.x][1] = T->plrc2[threadIdx.x / 3][threadIdx.x % 3];
   }
   __syncthreads();
-  // thread t: bead g0 + t / LF, or with a tile table (the tiles of kBeadTile
-  // beads that hold a bead of [g0, g1)) bead (t / LF) % kBeadTile of tile
-  // tiles[t / (LF * kBeadTile)]
-  const int t = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
-  int g;
-  if (tiles) {
-    const int k = t / (LF * kBeadTile);
-    if (k >= ntiles) return;
-    g = tiles[k] * kBeadTile + (t / LF) % kBeadTile;
-  } else {
-    g = g0 + t / LF;
-  }
-  const int l = t & (LF - 1);
-  if (g < g0 || g >= g1) return;
-  const int c = D.chain[g];
-  if (c < 0 || !(D.grid[c].work & W_FILTER)) return;
+  if (!act || c < 0 || !(D.grid[c].work & W_FILTER)) return;
   const ChainF cf = wave_chain_fields<ChainF>(D.cp, c);
-  const double* ux = cur ? D.ux[1] : D.ux[0];
-  const double* uy = cur ? D.uy[1] : D.uy[0];
-  const double* uz = cur ? D.uz[1] : D.uz[0];
   const double boxl = cf.boxl, qbox = cf.qbox;
   const int off = cf.off;
   // non-natives (neighbor_list.cpp:97-103): stable compaction of the
   // candidates.  Software pipeline as in the step kernel (candidate words two
   // chunks ahead, partner coordinates one chunk ahead), started before the
   // natives so that their loads overlap.
-  const double cx = ux[g], cy = uy[g], cz = uz[g];
   const uint32_t* C = D.cand + (size_t)g * D.cap;
   const ListRow S{D.slot, g, D.cap, true};
-  const int n = D.ccnt[g];
   // The candidates' positions are gathered in single precision (half the
   // data of the double positions).  A single-precision coordinate is within
   // 2^-24 |x| of the double one, so for a partner within R = cand_reach (every
